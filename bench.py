@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark: compressible viscous RHS on one 256^3 FP64 subdomain (MI355X).
+
+Workload: 256^3 FP64, 5 conserved variables, uniform flow at Mach 0.5 in x
+(gamma 1.4, mu = 1/Re = 1e-4), seeded Gaussian acoustic pulse (amplitude
+1e-3, width 8 cells), non-reflective characteristic BCs on all six faces.
+One timed step = one RK4 step: time step, four stages (each = full RHS
+through the HIP library + stage update) and the compact filter of the five
+conserved variables along x, y, z.  value = RHS grid points per second =
+4 * 256^3 / t_step (filter and time step included, i.e. conservative).
+
+Prints ONE JSON line.  --gpus N > 1 is not supported: this build runs the
+subdomain of one GPU.
+"""
+import argparse
+import json
+import os
+import statistics
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256, help="grid points per axis")
+    ap.add_argument("--path", default="ref", choices=["fused", "ref"],
+                    help="fused sweeps (default) or the reference-ordered per-operator path")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    return ap.parse_args()
+
+
+def built():
+    return os.path.exists(os.path.join(ROOT, "compact_schemes_amd", "libcompact_schemes_amd.so"))
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return "unknown"
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs one GPU's subdomain only; multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+    if not built():
+        import __graft_entry__
+        __graft_entry__.build()
+    import numpy as np
+    import torch
+    import compact_schemes_amd as cs
+    from compact_schemes_amd import workloads as wl
+
+    torch.cuda.set_device(0)
+    n = args.n
+    dims = (n, n, n)
+    lmx = n ** 3
+    st = wl.flow_state(dims, seed=2)
+    q = torch.from_numpy(st["qa"]).cuda()
+    q_init = q.clone()
+
+    if args.path == "ref":
+        from compact_schemes_amd.stepper import Stepper
+        stepper = Stepper(dims, wl.NBC_NON_REFLECTIVE, True, st["umf"], st)
+        step = lambda: stepper.rk4_step(q)
+        rhs = lambda: stepper.rhs(q)
+        num = stepper.num
+    else:
+        from compact_schemes_amd.fused import FusedSolver
+        solver = FusedSolver(dims, wl.NBC_NON_REFLECTIVE, st)
+        step = lambda: solver.rk4_step(q)
+        rhs = lambda: solver.rhs(q)
+        num = solver.num
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    q.copy_(q_init)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t0
+    ms_step = 1e3 * t_total / args.steps
+    value = 4.0 * lmx * args.steps / t_total / 1e9
+    assert torch.isfinite(q).all().item(), "non-finite state after the timed steps"
+
+    # RHS alone (HIP events on the launch stream)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    q.copy_(q_init)
+    rhs()
+    e0.record(stream)
+    nr = 5
+    for _ in range(nr):
+        rhs()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_rhs = e0.elapsed_time(e1) / nr
+
+    # dominant kernel: the y-direction line sweep, timed in isolation
+    kern = {}
+    if args.path == "ref":
+        fin = q[0:lmx].clone()
+        fout = torch.empty_like(fin)
+        e0.record(stream)
+        nk = 20
+        for _ in range(nk):
+            num.deriv(1, fin, fout)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_k = e0.elapsed_time(e1) / nk
+        bytes_per_launch = 16.0 * lmx          # read + write one FP64 field
+        kern = {"name": "cs::line_kernel<16,0,1> (y derivative)", "ms": ms_k, "bytes": bytes_per_launch}
+    else:
+        kern = solver.time_dominant_kernel(q_init, stream)
+
+    achieved = kern["bytes"] / (kern["ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": kern.get("traffic"),
+                "kernel": kern["name"], "kernel_ms": round(kern["ms"], 5),
+                "algorithmic_bytes_per_launch": kern["bytes"]}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(st, dims, args.cpu_reps)
+
+    line = {
+        "metric": "RHS Gpts/s at 256^3 FP64",
+        "value": round(value, 4),
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded acoustic pulse on uniform Mach 0.5 flow)",
+        "config": {"workload": f"viscous compressible RHS, {n}^3 FP64, non-reflective gcbc on all faces, "
+                               "one RK4 step (4 RHS + stage updates + GV10 filter)",
+                   "grid": [n, n, n], "path": args.path, "ms_per_rhs": round(ms_rhs, 3),
+                   "rhs_gpts_s_rhs_only": round(lmx / (ms_rhs * 1e-3) / 1e9, 4)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(st, dims, reps):
+    """The CPU oracle's full RHS (OpenMP over lines, all host cores) on the
+    same workload; median of `reps` RHS evaluations after one warm-up."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.lib()
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    # the host share of one GPU: OMP_NUM_THREADS where the launcher sets it
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    pyoracle.set_num_threads(cores)
+    lmx = dims[0] * dims[1] * dims[2]
+    ctx = pyoracle.Context(dims, [[10, 10, 10], [10, 10, 10]])
+    de, p = np.zeros(5 * lmx), np.zeros(lmx)
+    ts = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        ctx.rhs(1, st["qa"], de, p, st["ssk"], st["xim"], st["etm"], st["zem"], st["yaco"], st["cm"], st["sbcc"],
+                st["umf"])
+        if r > 0:
+            ts.append(time.perf_counter() - t0)
+    t = statistics.median(ts)
+    return {"value": round(lmx / t / 1e9, 5), "unit": "Gpts/s", "cores": pyoracle.num_threads(), "kind": "port",
+            "sample": f"{reps} full viscous RHS evaluations at {dims[0]}^3 (median, after 1 warm-up)",
+            "cpu": cpu_model(), "s_per_rhs": round(t, 3)}
+
+
+if __name__ == "__main__":
+    main()

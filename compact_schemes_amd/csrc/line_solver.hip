@@ -1,0 +1,482 @@
+// line_solver.hip -- compact derivative and compact filter along one axis.
+//
+// Math: the tridiagonal compact first derivative of the reference
+// (build_system, src/cuda/kernels/numerics_device_pc.hpp:176-293: closure
+// rows for nstart/nend = 0, quasi-disjoint seam rows for 1) and a 10th-order
+// compact filter (absent upstream).  Solve: not the reference's PCR
+// (numerics_device_pc.hpp:121-174, power-of-two lines only).  The system
+// matrix is constant per line length and end condition, so its LU
+// factorisation is computed once on the host and the per-line solve is two
+// first-order linear recurrences (forward with nu/mu, backward with rho).
+// A line is cut into P chunks of K points, one thread per chunk: each thread
+// runs both recurrences on its chunk with zero carry-in, and the carries are
+// propagated across chunks through LDS using the precomputed chunk products
+// (Mf, Rb).  Any line length 8..512 works.
+//
+// Layouts (one workgroup = TL lines x P chunks, threads line-fastest):
+//   y/z lines (AX = 1, 2): TL = 64, lanes of a wave are 64 consecutive
+//     lines, so every load and store of a wave touches 64 consecutive
+//     doubles of one row; the chunk index is wave-uniform and the tables are
+//     read with scalar loads;
+//   x lines   (AX = 0): TL = 16, the 16 lines (contiguous in memory) are
+//     staged through LDS with coalesced loads and stores, transposed so
+//     that a thread reads its chunk conflict-free.
+#include "cs_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace cs {
+
+constexpr int kXLines = 16;   // lines per workgroup for x lines
+constexpr int kMaxP   = 16;   // chunks per line
+
+// lines per workgroup: 16 for x (LDS tile), 64 for y/z, 32 for y/z with
+// 32-point chunks so that the register file holds them (512 threads)
+__host__ __device__ constexpr int lines_per_block(int K, int AX) {
+    return AX == 0 ? kXLines : (K == 32 ? 32 : 64);
+}
+__host__ __device__ constexpr int max_threads(int K) { return K == 32 ? 512 : 1024; }
+
+// ---------------------------------------------------------------------------
+// host: tables
+// ---------------------------------------------------------------------------
+int choose_chunk(int n, int* K, int* P) {
+    static const int ks[] = {8, 16, 32};
+    for (int k : ks) {
+        int p = (n + k - 1) / k;
+        if (p <= kMaxP) { *K = k; *P = p; return CS_OK; }
+    }
+    return fail(CS_ERR_SHAPE, "line length " + std::to_string(n) + " exceeds 512");
+}
+
+// Order-2M centred filter coefficients: sum a_n = 1+2af, sum_{n>=1} a_n n^{2m}
+// = 2af (m = 1..M-1), sum (-1)^n a_n = 0; solved by Gaussian elimination.
+void filter_coefficients(int M, double af, double* an) {
+    const int n = M + 1;
+    double A[6][6] = {}, r[6] = {};
+    for (int j = 0; j <= M; j++) A[0][j] = 1.0;
+    r[0] = 1.0 + 2.0 * af;
+    for (int m = 1; m < M; m++) {
+        for (int j = 1; j <= M; j++) A[m][j] = std::pow((double)j, 2.0 * m);
+        r[m] = 2.0 * af;
+    }
+    for (int j = 0; j <= M; j++) A[M][j] = (j % 2 == 0) ? 1.0 : -1.0;
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        for (int i = k + 1; i < n; i++) if (std::fabs(A[i][k]) > std::fabs(A[p][k])) p = i;
+        if (p != k) { for (int j = 0; j < n; j++) std::swap(A[k][j], A[p][j]); std::swap(r[k], r[p]); }
+        for (int i = k + 1; i < n; i++) {
+            double f = A[i][k] / A[k][k];
+            for (int j = k; j < n; j++) A[i][j] -= f * A[k][j];
+            r[i] -= f * r[k];
+        }
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double s = r[i];
+        for (int j = i + 1; j < n; j++) s -= A[i][j] * an[j];
+        an[i] = s / A[i][i];
+    }
+}
+
+static void thomas_lu(int n, const std::vector<double>& a, const std::vector<double>& b,
+                      const std::vector<double>& c, double* nu, double* mu, double* rho) {
+    double cp_prev = 0.0;
+    for (int i = 0; i < n; i++) {
+        double w = 1.0 / (b[i] - (i > 0 ? a[i] * cp_prev : 0.0));
+        nu[i] = w;
+        mu[i] = (i > 0) ? -a[i] * w : 0.0;
+        double cp = c[i] * w;
+        rho[i] = -cp;
+        cp_prev = cp;
+    }
+    rho[n - 1] = 0.0;
+}
+
+int build_line_table(int op, int n, int start, int end, double af, int K, std::vector<double>& h) {
+    h.assign(6 * (size_t)n + 8 + 36, 0.0);
+    std::vector<double> a(n), b(n, 1.0), c(n);
+    bool cyclic = false;
+    if (op == OP_DERIV) {
+        for (int i = 0; i < n; i++) { a[i] = kAlpha; c[i] = kAlpha; }
+        if (start == 0) { a[0] = 0; c[0] = kAlpha01; a[1] = kAlpha10; c[1] = kAlpha10; }
+        else            { a[0] = 0; c[0] = kAlpha; }
+        if (end == 0)   { a[n - 1] = kAlpha01; c[n - 1] = 0; a[n - 2] = kAlpha10; c[n - 2] = kAlpha10; }
+        else            { a[n - 1] = kAlpha; c[n - 1] = 0; }
+    } else {
+        cyclic = (start == 1 && end == 1);
+        if (!cyclic && (start == 1 || end == 1))
+            return fail(CS_ERR_UNSUPPORTED, "filter: a seam on one face only has no single-GPU wrap");
+        for (int i = 0; i < n; i++) { a[i] = af; c[i] = af; }
+        if (cyclic) {
+            // Sherman-Morrison: A = A' + u v^T, gamma = -b_0.
+            double gamma = -1.0;
+            b[0] = 1.0 - gamma;
+            b[n - 1] = 1.0 - af * af / gamma;
+            a[0] = 0.0; c[n - 1] = 0.0;
+        } else {
+            a[0] = 0; c[0] = 0; a[n - 1] = 0; c[n - 1] = 0;   // identity end rows
+        }
+        for (int M = 1; M <= 5; M++) filter_coefficients(M, af, &h[6 * (size_t)n + 8 + 6 * M]);
+    }
+    double* nu = &h[0];
+    double* mu = &h[n];
+    double* rho = &h[2 * (size_t)n];
+    double* Mf = &h[3 * (size_t)n];
+    double* Rb = &h[4 * (size_t)n];
+    double* z = &h[5 * (size_t)n];
+    thomas_lu(n, a, b, c, nu, mu, rho);
+    for (int s = 0; s < n; s += K) {
+        int e = std::min(s + K, n) - 1;
+        double p = 1.0;
+        for (int j = s; j <= e; j++) { p *= mu[j]; Mf[j] = p; }
+        p = 1.0;
+        for (int j = e; j >= s; j--) { p *= rho[j]; Rb[j] = p; }
+    }
+    if (cyclic) {
+        double gamma = -1.0;
+        std::vector<double> u(n, 0.0), g(n);
+        u[0] = gamma; u[n - 1] = af;
+        for (int i = 0; i < n; i++) g[i] = nu[i] * u[i] + (i > 0 ? mu[i] * g[i - 1] : 0.0);
+        for (int i = n - 1; i >= 0; i--) z[i] = g[i] + (i < n - 1 ? rho[i] * z[i + 1] : 0.0);
+        double kappa = af / gamma;
+        h[6 * (size_t)n + 0] = 1.0 / (1.0 + z[0] + kappa * z[n - 1]);
+        h[6 * (size_t)n + 1] = kappa;
+    }
+    return CS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device
+// ---------------------------------------------------------------------------
+template <int K, int OP, int AX>
+__global__ void __launch_bounds__(max_threads(K)) line_kernel(LineArgs a) {
+    constexpr int H = (OP == OP_DERIV) ? 2 : 5;     // stencil half-width
+    constexpr int W = K + 2 * H;
+    constexpr int TL = lines_per_block(K, AX);
+    constexpr int TP = TL + 1;                      // padded LDS row (x tile)
+
+    extern __shared__ double smem[];
+    double* gend = smem;                            // [kMaxP][TL]
+    double* xbeg = smem + kMaxP * TL;               // [kMaxP][TL]
+    double* tile = smem + 2 * kMaxP * TL;           // x only: [N][TP]
+
+    const int tid = threadIdx.x;
+    const int lloc = tid % TL;
+    int c = tid / TL;
+    if (AX != 0) c = __builtin_amdgcn_readfirstlane(c);
+    const int N = a.N;
+    const int P = a.P;
+    const long line0 = (long)blockIdx.x * TL;
+    const long line = line0 + lloc;
+    const bool active = line < a.nlines;
+    const int s = c * K;
+    const bool cvalid = s < N;
+    const double* tab = a.tab;
+    const double* nu = tab;
+    const double* mu = tab + N;
+    const double* rho = tab + 2 * N;
+    const double* Mf = tab + 3 * N;
+    const double* Rb = tab + 4 * N;
+
+    long base = 0;
+    if (AX != 0) base = line_base(AX, active ? line : 0, a.lxi, a.plane);
+
+    // ---- stage x lines in LDS (coalesced, transposed) ----
+    if (AX == 0) {
+        const int nl = (int)min((long)TL, (long)a.nlines - line0);
+        const double* src = a.in + line0 * N;
+        for (int l = 0; l < nl; l++)
+            for (int i = tid; i < N; i += blockDim.x) tile[i * TP + l] = src[(long)l * N + i];
+        __syncthreads();
+    }
+
+    auto fetch = [&](int row) -> double {
+        if (AX == 0) return tile[row * TP + lloc];
+        return active ? a.in[base + (long)row * a.st] : 0.0;
+    };
+
+    // ---- right-hand side of the chunk rows ----
+    double r[K];
+    if (cvalid) {
+        double w[W];
+#pragma unroll
+        for (int t = 0; t < W; t++) {
+            int row = s - H + t;
+            if (OP == OP_FILTER && a.cyclic) row = (row + N) % N;
+            w[t] = (row >= 0 && row < N) ? fetch(row) : 0.0;
+        }
+        if (OP == OP_DERIV) {
+#pragma unroll
+            for (int t = 0; t < K; t++)
+                r[t] = kAa * (w[t + H + 1] - w[t + H - 1]) + kBb * (w[t + H + 2] - w[t + H - 2]);
+            const double* pbco = a.pb;
+            const double* pbci = a.pb + 16;
+            if (c == 0) {
+                if (a.seam0) {
+                    double own[kLmd], oth[kLmd];
+#pragma unroll
+                    for (int i = 0; i < kLmd; i++) { own[i] = fetch(i); oth[i] = fetch(N - 1 - i); }
+                    double rv0, rv1;
+                    if (a.recv) {
+                        rv0 = active ? a.recv[line] : 0.0;
+                        rv1 = active ? a.recv[a.nlines + line] : 0.0;
+                    } else {
+                        rv0 = 0.0; rv1 = 0.0;
+#pragma unroll
+                        for (int i = 0; i < kLmd; i++) { rv0 += pbco[i] * oth[i]; rv1 += pbco[kLmd + i] * oth[i]; }
+                    }
+                    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+                    for (int i = 0; i < kLmd; i++) { s0 += pbci[i] * own[i]; s1 += pbci[kLmd + i] * own[i]; }
+                    r[0] = s0 + rv0;
+                    r[1] = s1 + rv1;
+                } else {
+                    r[0] = kAb00 * w[H] + kAb01 * w[H + 1] + kAb02 * w[H + 2];
+                    r[1] = kAb10 * (w[H + 2] - w[H]);
+                }
+            }
+            if (s + K > N - 2) {            // the chunk holds row N-2 and/or N-1
+                const int tl = N - 1 - s;   // chunk-local index of row N-1
+                double r1, r2;
+                if (a.seam1) {
+                    double own[kLmd], oth[kLmd];
+#pragma unroll
+                    for (int i = 0; i < kLmd; i++) { own[i] = fetch(N - 1 - i); oth[i] = fetch(i); }
+                    double rv2, rv3;
+                    if (a.recv) {
+                        rv2 = active ? a.recv[2L * a.nlines + line] : 0.0;
+                        rv3 = active ? a.recv[3L * a.nlines + line] : 0.0;
+                    } else {
+                        rv2 = 0.0; rv3 = 0.0;
+#pragma unroll
+                        for (int i = 0; i < kLmd; i++) { rv2 += pbco[i] * oth[i]; rv3 += pbco[kLmd + i] * oth[i]; }
+                    }
+                    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+                    for (int i = 0; i < kLmd; i++) { s0 += pbci[i] * own[i]; s1 += pbci[kLmd + i] * own[i]; }
+                    r1 = -(s0 + rv2);
+                    r2 = -(s1 + rv3);
+                } else {
+                    double fN1 = fetch(N - 1), fN2 = fetch(N - 2), fN3 = fetch(N - 3);
+                    r1 = -(kAb00 * fN1 + kAb01 * fN2 + kAb02 * fN3);
+                    r2 = -kAb10 * (fN3 - fN1);
+                }
+#pragma unroll
+                for (int t = 0; t < K; t++) {
+                    if (t == tl) r[t] = r1;
+                    if (t == tl - 1) r[t] = r2;
+                }
+            }
+        } else {
+            const double* an = tab + 6 * N + 8;
+            if (a.cyclic) {
+#pragma unroll
+                for (int t = 0; t < K; t++) {
+                    double v = an[30] * w[t + H];
+#pragma unroll
+                    for (int m = 1; m <= 5; m++) v += 0.5 * an[30 + m] * (w[t + H + m] + w[t + H - m]);
+                    r[t] = v;
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < K; t++) {
+                    const int j = s + t;
+                    const int dist = min(j, N - 1 - j);
+                    if (dist >= 5) {
+                        double v = an[30] * w[t + H];
+#pragma unroll
+                        for (int m = 1; m <= 5; m++) v += 0.5 * an[30 + m] * (w[t + H + m] + w[t + H - m]);
+                        r[t] = v;
+                    } else if (dist <= 0) {
+                        r[t] = w[t + H];
+                    } else {
+                        const double* cf = an + 6 * dist;
+                        double v = cf[0] * w[t + H];
+                        for (int m = 1; m <= dist; m++) v += 0.5 * cf[m] * (w[t + H + m] + w[t + H - m]);
+                        r[t] = v;
+                    }
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < K; t++) r[t] = 0.0;
+    }
+
+    // ---- forward recurrence on the chunk, zero carry-in ----
+    double prev = 0.0;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        const int j = s + t;
+        if (j < N) { prev = nu[j] * r[t] + mu[j] * prev; r[t] = prev; }
+    }
+    if (cvalid) gend[c * TL + lloc] = prev;
+    __syncthreads();
+    double G = 0.0;
+    for (int cc = 0; cc < c; cc++) G = gend[cc * TL + lloc] + Mf[cc * K + K - 1] * G;
+    double yN1 = 0.0;
+    if (OP == OP_FILTER && a.cyclic) {
+        double Ga = 0.0;
+        for (int cc = 0; cc < P; cc++) Ga = gend[cc * TL + lloc] + Mf[min(cc * K + K, N) - 1] * Ga;
+        yN1 = Ga;
+    }
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        const int j = s + t;
+        if (j < N) r[t] += Mf[j] * G;
+    }
+
+    // ---- backward recurrence, zero carry-in ----
+    double nxt = 0.0;
+#pragma unroll
+    for (int t = K - 1; t >= 0; t--) {
+        const int j = s + t;
+        if (j < N) { nxt = r[t] + rho[j] * nxt; r[t] = nxt; }
+    }
+    if (cvalid) xbeg[c * TL + lloc] = nxt;
+    __syncthreads();
+    double X = 0.0;
+    for (int cc = P - 1; cc > c; cc--) X = xbeg[cc * TL + lloc] + Rb[cc * K] * X;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        const int j = s + t;
+        if (j < N) r[t] += Rb[j] * X;
+    }
+    if (OP == OP_FILTER && a.cyclic) {
+        double X0 = 0.0;
+        for (int cc = P - 1; cc >= 0; cc--) X0 = xbeg[cc * TL + lloc] + Rb[cc * K] * X0;
+        const double* z = tab + 5 * N;
+        const double beta = (X0 + tab[6 * N + 1] * yN1) * tab[6 * N];
+#pragma unroll
+        for (int t = 0; t < K; t++) {
+            const int j = s + t;
+            if (j < N) r[t] -= beta * z[j];
+        }
+    }
+
+    // ---- boundary values for the characteristic BCs ----
+    if (a.drva && active && cvalid) {
+        const long fs = a.nlines;
+        if (c == 0) a.drva[(long)a.var * fs + line] = r[0];
+        if (c == P - 1) {
+#pragma unroll
+            for (int t = 0; t < K; t++)
+                if (s + t == N - 1) a.drva[(long)kNV * fs + (long)a.var * fs + line] = r[t];
+        }
+    }
+
+    // ---- store ----
+    if (AX == 0) {
+        // every thread finished reading the tile before the second barrier
+        if (cvalid) {
+#pragma unroll
+            for (int t = 0; t < K; t++)
+                if (s + t < N) tile[(s + t) * TP + lloc] = r[t];
+        }
+        __syncthreads();
+        const int nl = (int)min((long)TL, (long)a.nlines - line0);
+        double* dst = a.out + line0 * N;
+        for (int l = 0; l < nl; l++)
+            for (int i = tid; i < N; i += blockDim.x) dst[(long)l * N + i] = tile[i * TP + l];
+    } else if (active && cvalid) {
+#pragma unroll
+        for (int t = 0; t < K; t++)
+            if (s + t < N) a.out[base + (long)(s + t) * a.st] = r[t];
+    }
+}
+
+template <int K, int OP, int AX>
+static int launch_one(const LineArgs& a, hipStream_t s) {
+    constexpr int TL = lines_per_block(K, AX);
+    dim3 block(TL * a.P);
+    dim3 grid((unsigned)((a.nlines + TL - 1) / TL));
+    size_t lds = sizeof(double) * 2 * kMaxP * TL;
+    if (AX == 0) lds += sizeof(double) * (size_t)a.N * (TL + 1);
+    if (lds > 64 * 1024)
+        CS_HIP(hipFuncSetAttribute((const void*)line_kernel<K, OP, AX>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((line_kernel<K, OP, AX>), grid, block, lds, s, a);
+    CS_HIP(hipGetLastError());
+    return CS_OK;
+}
+
+template <int K, int OP>
+static int launch_axis(int axis, const LineArgs& a, hipStream_t s) {
+    if (axis == 0) return launch_one<K, OP, 0>(a, s);
+    if (axis == 1) return launch_one<K, OP, 1>(a, s);
+    return launch_one<K, OP, 2>(a, s);
+}
+
+int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s) {
+    if (a.P < 1 || a.P > kMaxP) return fail(CS_ERR_SHAPE, "chunk count out of range");
+    if ((long)a.P * K < a.N) return fail(CS_ERR_SHAPE, "chunks do not cover the line");
+    if (axis == 0 && (size_t)a.N * (kXLines + 1) * 8 + 2 * kMaxP * kXLines * 8 > 160 * 1024)
+        return fail(CS_ERR_SHAPE, "x line too long for the LDS tile");
+    if (op == OP_DERIV) {
+        if (K == 8) return launch_axis<8, OP_DERIV>(axis, a, s);
+        if (K == 16) return launch_axis<16, OP_DERIV>(axis, a, s);
+        if (K == 32) return launch_axis<32, OP_DERIV>(axis, a, s);
+    } else {
+        if (K == 8) return launch_axis<8, OP_FILTER>(axis, a, s);
+        if (K == 16) return launch_axis<16, OP_FILTER>(axis, a, s);
+        if (K == 32) return launch_axis<32, OP_FILTER>(axis, a, s);
+    }
+    return fail(CS_ERR_SHAPE, "unsupported chunk length");
+}
+
+// ---------------------------------------------------------------------------
+// context helpers
+// ---------------------------------------------------------------------------
+int get_line_op(cs_numerics* num, int op, int axis, int start, int end, LineOpTable** out) {
+    const int n = num->g.n(axis);
+    LineOpKey key{op, n, start, end};
+    auto it = num->ops.find(key);
+    if (it != num->ops.end()) { *out = &it->second; return CS_OK; }
+    int minlen = (op == OP_FILTER && start == 1) ? 11 : 8;
+    if (n < minlen) return fail(CS_ERR_SHAPE, "line length " + std::to_string(n) + " below the stencil size");
+    LineOpTable t;
+    t.n = n;
+    int rc = choose_chunk(n, &t.K, &t.P);
+    if (rc) return rc;
+    std::vector<double> h;
+    rc = build_line_table(op, n, start, end, num->alpha_f, t.K, h);
+    if (rc) return rc;
+    CS_HIP(hipMalloc(&t.dev, h.size() * sizeof(double)));
+    CS_HIP(hipMemcpy(t.dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    t.count = h.size();
+    auto res = num->ops.emplace(key, t);
+    *out = &res.first->second;
+    return CS_OK;
+}
+
+int line_apply(cs_numerics* num, int op, int axis, const double* in, double* out, int start, int end,
+               const double* recv, int var, bool write_drva, hipStream_t s) {
+    if (axis < 0 || axis > 2) return fail(CS_ERR_ARG, "axis must be 0, 1 or 2");
+    if (!in || !out) return fail(CS_ERR_ARG, "null field pointer");
+    if (op == OP_DERIV && (start == 1 || end == 1) && !num->table_set)
+        return fail(CS_ERR_STATE, "seam rows need deriv_setup (or an interface table) first");
+    LineOpTable* t = nullptr;
+    int rc = get_line_op(num, op, axis, start, end, &t);
+    if (rc) return rc;
+    LineArgs a{};
+    a.in = in;
+    a.out = out;
+    a.tab = t->dev;
+    a.pb = num->pb;
+    a.recv = recv;
+    a.drva = write_drva ? num->drva[axis] : nullptr;
+    a.var = var;
+    a.N = t->n;
+    a.nlines = (int)num->g.face_size(axis);
+    a.lxi = num->g.lxi;
+    a.plane = (long)num->g.lxi * num->g.let;
+    a.st = num->g.stride(axis);
+    a.P = t->P;
+    a.seam0 = (op == OP_DERIV) ? start : 0;
+    a.seam1 = (op == OP_DERIV) ? end : 0;
+    a.cyclic = (op == OP_FILTER) ? (start == 1 && end == 1) : 0;
+    return launch_line(op, axis, t->K, a, s);
+}
+
+}  // namespace cs

@@ -1,0 +1,111 @@
+"""Seeded synthetic inputs of the benchmark and test workloads (numpy, FP64).
+
+Fields use the reference layout: SoA, variable-major, x fastest,
+idx = (k*let + j)*lxi + i.  Grid spacing is 1 (unit Cartesian metrics).
+"""
+import numpy as np
+
+GAM = 1.4
+
+NBC_PERIODIC = [[45, 45, 45], [45, 45, 45]]
+NBC_NON_REFLECTIVE = [[10, 10, 10], [10, 10, 10]]
+
+
+def grid_index(dims):
+    lxi, let, lze = dims
+    k, j, i = np.meshgrid(np.arange(lze), np.arange(let), np.arange(lxi), indexing="ij")
+    return i.astype(np.float64), j.astype(np.float64), k.astype(np.float64)
+
+
+def periodic_field(dims, seed=1, modes=8, kmax=20, noise=1e-3):
+    """Sum of `modes` plane waves sin(2 pi (kx x + ky y + kz z)/L + phi) with
+    wavenumbers in 1..kmax and phases in [0, 2 pi), plus `noise` N(0,1).
+    Returns (f, df_dx, df_dy, df_dz) of the noise-free part as flat arrays."""
+    rng = np.random.default_rng(seed)
+    lxi, let, lze = dims
+    x, y, z = grid_index(dims)
+    f = np.zeros_like(x)
+    g = [np.zeros_like(x) for _ in range(3)]
+    for _ in range(modes):
+        kx, ky, kz = rng.integers(1, kmax + 1, size=3)
+        phi = rng.uniform(0.0, 2.0 * np.pi)
+        wx, wy, wz = 2 * np.pi * kx / lxi, 2 * np.pi * ky / let, 2 * np.pi * kz / lze
+        arg = wx * x + wy * y + wz * z + phi
+        f += np.sin(arg)
+        c = np.cos(arg)
+        g[0] += wx * c
+        g[1] += wy * c
+        g[2] += wz * c
+    f_noisy = f + noise * rng.standard_normal(f.shape)
+    return f_noisy.reshape(-1), [gi.reshape(-1) for gi in g]
+
+
+def metrics(dims):
+    """Unit Cartesian metrics: xim = (1,0,0), etm = (0,1,0), zem = (0,0,1), yaco = 1."""
+    lmx = dims[0] * dims[1] * dims[2]
+    xim = np.zeros(3 * lmx)
+    etm = np.zeros(3 * lmx)
+    zem = np.zeros(3 * lmx)
+    xim[0:lmx] = 1.0
+    etm[lmx:2 * lmx] = 1.0
+    zem[2 * lmx:3 * lmx] = 1.0
+    return xim, etm, zem, np.ones(lmx)
+
+
+def face_normals(dims):
+    """cm[a] = [2 face][3][face_size]: unit normal e_a on both faces of axis a."""
+    lxi, let, lze = dims
+    fs = (let * lze, lxi * lze, lxi * let)
+    cm = []
+    for a in range(3):
+        c = np.zeros((2, 3, fs[a]))
+        c[:, a, :] = 1.0
+        cm.append(c.reshape(-1))
+    return cm
+
+
+def boundary_weights(dims):
+    """sbcc: 1 on every boundary point of the subdomain, 0 inside."""
+    x, y, z = grid_index(dims)
+    lxi, let, lze = dims
+    on = (x == 0) | (x == lxi - 1) | (y == 0) | (y == let - 1) | (z == 0) | (z == lze - 1)
+    return on.astype(np.float64).reshape(-1)
+
+
+def flow_state(dims, seed=2, mach=0.5, amp=1e-3, width=8.0, mu=1e-4, noise=0.0):
+    """Uniform flow at Mach `mach` in x (carried by the frame velocity umf,
+    stored velocity = perturbation), rho0 = 1, p0 = 1/gam (c0 = 1), with a
+    Gaussian pressure pulse p' = amp exp(-ln2 r^2/width^2) at a seeded centre
+    in the middle half of the box and rho' = p'/c0^2.  `noise` adds seeded
+    N(0,1)*noise perturbations to rho, u, v, w, p (exercises every term).
+    Returns a dict of flat float64 arrays and scalars."""
+    rng = np.random.default_rng(seed)
+    lxi, let, lze = dims
+    lmx = lxi * let * lze
+    x, y, z = grid_index(dims)
+    cen = [rng.uniform(0.25 * n, 0.75 * n) for n in dims]
+    r2 = (x - cen[0]) ** 2 + (y - cen[1]) ** 2 + (z - cen[2]) ** 2
+    pp = amp * np.exp(-np.log(2.0) * r2 / width ** 2)
+    p = 1.0 / GAM + pp
+    rho = 1.0 + pp
+    u = np.zeros_like(rho)
+    v = np.zeros_like(rho)
+    w = np.zeros_like(rho)
+    if noise > 0:
+        rho = rho + noise * rng.standard_normal(rho.shape)
+        u = u + noise * rng.standard_normal(rho.shape)
+        v = v + noise * rng.standard_normal(rho.shape)
+        w = w + noise * rng.standard_normal(rho.shape)
+        p = p + noise * rng.standard_normal(rho.shape) / GAM
+    E = p / (GAM - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    qa = np.concatenate([a.reshape(-1) for a in (rho, rho * u, rho * v, rho * w, E)])
+    xim, etm, zem, yaco = metrics(dims)
+    return {
+        "qa": qa,
+        "xim": xim, "etm": etm, "zem": zem, "yaco": yaco,
+        "ssk": np.full(lmx, mu),
+        "cm": face_normals(dims),
+        "sbcc": boundary_weights(dims),
+        "umf": (mach, 0.0, 0.0),
+        "center": cen,
+    }

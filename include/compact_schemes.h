@@ -1,0 +1,157 @@
+/*
+ * compact_schemes.h -- C-ABI of the MI355X compact-scheme hot path.
+ *
+ * Every entry point replaces one operator of the reference's L5 API
+ * (EnricoDeg/compact_schemes, paths relative to that repository); the cited
+ * file:line is the interface it stands in for.  All field pointers are device
+ * pointers to FP64 data in the reference layout: SoA, variable-major, x
+ * fastest, idx = (k*let + j)*lxi + i, lmx = lxi*let*lze.
+ *
+ * Conventions
+ *   - every call returns CS_OK (0) or a CS_ERR_* code; cs_last_error() gives
+ *     the message of the last failing call on the calling thread;
+ *   - launches are asynchronous on the given stream (a hipStream_t, NULL for
+ *     the null stream); only cs_calc_time_step synchronises (it returns a
+ *     host value, as physics::calc_time_step does);
+ *   - contexts are thread-compatible, not thread-safe;
+ *   - halo exchange does not exist on one GPU: faces with a seam code
+ *     (BC_INTER_STRAIGHT 35, BC_INTER_SUBDOMAINS 40, BC_PERIODIC 45) exchange
+ *     with the opposite face of the same subdomain, exactly what the reference
+ *     does when the neighbour rank is the rank itself.
+ */
+#ifndef COMPACT_SCHEMES_H
+#define COMPACT_SCHEMES_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CS_OK              0
+#define CS_ERR_ARG         1   /* null pointer, bad axis/variable/BC code   */
+#define CS_ERR_SHAPE       2   /* line length or grid shape not supported   */
+#define CS_ERR_HIP         3   /* a HIP runtime call failed                 */
+#define CS_ERR_UNSUPPORTED 4   /* needs something one GPU does not have     */
+#define CS_ERR_STATE       5   /* call order (e.g. no deriv_setup yet)      */
+
+/* boundary-condition codes, src/common/parameters.hpp:49-55 */
+#define CS_BC_NON_REFLECTIVE   10
+#define CS_BC_WALL_INVISCID    20
+#define CS_BC_WALL_VISCOUS     25
+#define CS_BC_INTER_CURV       30
+#define CS_BC_INTER_STRAIGHT   35
+#define CS_BC_INTER_SUBDOMAINS 40
+#define CS_BC_PERIODIC         45
+
+typedef void* cs_stream;                  /* hipStream_t                        */
+typedef struct { int lxi, let, lze; } cs_dims;   /* t_dcomp, data_types.hpp:33-39 */
+typedef struct { double x, y, z; } cs_point;     /* t_point, data_types.hpp:41-47 */
+
+typedef struct cs_numerics cs_numerics;   /* numerics_base / numerics_pc state  */
+typedef struct cs_physics  cs_physics;    /* physics_base / physics state       */
+
+const char* cs_last_error(void);
+const char* cs_version(void);
+
+/* ---- numerics ---------------------------------------------------------- */
+
+/* numerics_base(t_dcomp, int nbc[2][3]), src/numerics_base.hpp:43-94.
+ * nbc is [2 faces][3 axes] row-major.  Allocates drva, send, recv, pbco, pbci
+ * on the current device. */
+int cs_numerics_create(const cs_dims* dims, const int* nbc, cs_numerics** out);
+/* ~numerics_base, src/numerics_base.hpp:96-111 */
+int cs_numerics_destroy(cs_numerics* num);
+/* numerics_base::deriv_setup, src/numerics_base.hpp:182-297: the as-shipped
+ * quasi-disjoint interface table (computed in FP64). */
+int cs_deriv_setup(cs_numerics* num);
+/* replaces the pbco/pbci table (e.g. the D1-corrected one); no reference
+ * counterpart -- the reference only has deriv_setup. */
+int cs_set_interface_table(cs_numerics* num, const double* pbco16, const double* pbci16);
+int cs_get_interface_table(const cs_numerics* num, double* pbco16, double* pbci16);
+/* numerics_base members drva_buffer/send_buffer/recv_buffer,
+ * src/numerics_base.hpp:299-305: device pointers of axis `axis`.
+ * drva = [2 face][5 var][face], send/recv = [5 var][2 face][2 row][face]. */
+int cs_numerics_buffers(cs_numerics* num, int axis, double** drva, double** send, double** recv);
+/* numerics_base::ndf, src/numerics_base.hpp:73-93: ndf[face*3+axis]. */
+int cs_numerics_ndf(const cs_numerics* num, int* ndf6);
+
+/* numerics_pc::fill_buffers<lo,hi>, src/numerics_pc.hpp:231-287: fills send
+ * for variables lo..hi-1 of a [5][3][lmx] field; component nzk=(1-nrt)*axis. */
+int cs_fill_buffers(cs_numerics* num, const double* infield, int var_lo, int var_hi, int nrt,
+                    cs_stream stream);
+/* numerics_base::mpigo (2D), src/numerics_base.hpp:114-180, one subdomain:
+ * recv[var][face] = send[var][1-face] on every seam face. */
+int cs_mpigo(cs_numerics* num, int var, cs_stream stream);
+/* numerics_pc::deriv1d<Axis>, src/numerics_pc.hpp:74-119: out = d(in)/d(xi_axis),
+ * closure rows where nstart/nend == 0, seam rows (reading recv) where 1. */
+int cs_deriv1d(cs_numerics* num, int axis, const double* in, double* out, int nstart, int nend,
+               int var, cs_stream stream);
+/* numerics_pc::deriv2d<Axis>, src/numerics_pc.hpp:121-168: reads component
+ * `comp` of in ([3][lmx] per variable), writes component `axis` of out. */
+int cs_deriv2d(cs_numerics* num, int axis, const double* in, double* out, int nstart, int nend,
+               int var, int comp, cs_stream stream);
+/* fill + mpigo + deriv1d in one launch: the seam sums are formed from the
+ * opposite end of each line inside the kernel (no send/recv traffic).  Same
+ * result as cs_fill_buffers + cs_mpigo + cs_deriv1d with the context's ndf. */
+int cs_deriv(cs_numerics* num, int axis, const double* in, double* out, int var, cs_stream stream);
+/* compact filter along `axis` (absent upstream; 10th-order tridiagonal
+ * filter, alpha_f from cs_set_filter_alpha, default 0.49).  Cyclic when both
+ * faces of the axis are seam codes, reduced-order centred closures otherwise.
+ * in == out is allowed. */
+int cs_filter(cs_numerics* num, int axis, const double* in, double* out, cs_stream stream);
+int cs_set_filter_alpha(cs_numerics* num, double alpha_f);
+
+/* ---- physics ----------------------------------------------------------- */
+
+/* physics_base<Visc,T>(t_dcomp), src/physics_base.hpp:45-130, with umf set as
+ * read_config does (src/physics_base.hpp:135-145) and dudtmf = 0. */
+int cs_physics_create(const cs_dims* dims, int viscous, cs_point umf, cs_physics** out);
+int cs_physics_destroy(cs_physics* phys);
+int cs_physics_set_frame(cs_physics* phys, cs_point umf, cs_point dudtmf);
+/* stress = [6][lmx] (xx,yy,zz,xy,yz,zx), heat = [3][lmx] (xx,yy,zz):
+ * the device storage behind physics_base::stress_tensor / heat_fluxes,
+ * src/physics_base.hpp:57-86. */
+int cs_physics_buffers(cs_physics* phys, double** stress, double** heat, double** buffer,
+                       double** buffer_deriv);
+
+/* primitive variables and pressure (absent upstream; the layout the
+ * reference kernels read, src/cuda/kernels/physics.hpp:106-108,395):
+ * de0 = 1/rho, de1..3 = u,v,w, de4 = c^2 = gam*p/rho, p = (gam-1)(E-rho|u|^2/2). */
+int cs_primitives(const cs_dims* dims, const double* qa, double* de, double* pressure,
+                  cs_stream stream);
+/* physics::calc_fluxes, src/physics_pc.hpp:83-149: de = div of the fluxes. */
+int cs_calc_fluxes(cs_physics* phys, cs_numerics* num, const double* qa, const double* pressure,
+                   double* de, const double* xim, const double* etm, const double* zem,
+                   cs_stream stream);
+/* physics::calc_viscous_shear_stress, src/physics_pc.hpp:151-227 (D6-D8
+ * corrected: seam fill from component 0, axis-matched BC rows, chain rule on
+ * the derivatives). */
+int cs_calc_viscous_shear_stress(cs_physics* phys, cs_numerics* num, double* de, const double* ssk,
+                                 const double* xim, const double* etm, const double* zem,
+                                 const double* yaco, cs_stream stream);
+/* physics::calc_time_step, src/physics_pc.hpp:229-256 (D9 corrected: FP64
+ * max, zeroed accumulators).  max2 (host, may be NULL) receives the two
+ * maxima.  Synchronises `stream`. */
+int cs_calc_time_step(cs_physics* phys, const double* xim, const double* etm, const double* zem,
+                      const double* de, const double* yaco, const double* ssk, double cfl,
+                      double* dte, double* max2, cs_stream stream);
+/* gcbc_go, src/gcbc.hpp:86-186 (D10-D13 corrected).  cm[a] = [2][3][face]
+ * unit face normals; drvb[a] = neighbour characteristics, only read on
+ * BC_INTER_CURV faces (the exchange itself is the caller's). */
+int cs_gcbc_go(cs_numerics* num, const double* const* cm, const double* const* drvb,
+               const double* qa, double* de, const double* pressure, const double* yaco,
+               const double* sbcc, cs_point umf, cs_point dudtmf, double dtwi, cs_stream stream);
+/* wall_condition_update, src/gcbc.hpp:188-228. */
+int cs_wall_condition_update(cs_numerics* num, double* qa, const int* npex, cs_point umf,
+                             cs_stream stream);
+/* RK stage update (absent upstream; canard.cu:196-200 names the weights):
+ * q = q0 - dtk * yaco * de for the 5 conserved variables. */
+int cs_rk_stage(const cs_dims* dims, double* q, const double* q0, const double* de,
+                const double* yaco, double dtk, cs_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COMPACT_SCHEMES_H */

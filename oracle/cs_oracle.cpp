@@ -1,0 +1,956 @@
+// cs_oracle.cpp -- CPU restatement of the compact-scheme hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+// compact_schemes_amd/: it is loaded by tests/, by __graft_entry__.smoke() and
+// by bench.py's cpu_baseline leg, never by the product path.
+//
+// Each function restates the reference algorithm named in its comment
+// (paths are relative to the reference repository EnricoDeg/compact_schemes).
+// Where the reference is undefined or wrong (defect ids D1..D21 as listed in
+// the design notes) the "intended" behaviour is restated and the comment says
+// so.  Components that the reference does not contain (compact filter, RK
+// stage update, primitive variables) are defined here from the published
+// method and are marked "build-defined".
+//
+// Field layout: SoA, variable-major, x fastest: idx = (k*let + j)*lxi + i.
+// Face arrays for axis a: face_idx = k*let+j (a=0), k*lxi+i (a=1), j*lxi+i (a=2).
+//
+// Parity status of each function (what pins it) is listed in tests/ and in the
+// design notes; functions with no independent check say "parity unpinned".
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <omp.h>
+
+namespace {
+
+// ---- constants: src/common/parameters.hpp:33-74 ---------------------------
+constexpr double alpha   = 1.0 / 3.0;
+constexpr double alpha01 = 2.0;
+constexpr double alpha10 = 1.0 / 4.0;
+constexpr double aa      = 7.0 / 9.0;
+constexpr double bb      = 1.0 / 36.0;
+constexpr double ab00    = -5.0 / 2.0;
+constexpr double ab01    = 2.0;
+constexpr double ab02    = 1.0 / 2.0;
+constexpr double ab10    = 3.0 / 4.0;
+constexpr int lmd  = 8;
+constexpr int mbci = 4;
+constexpr int NV   = 5;
+constexpr int BC_NON_REFLECTIVE   = 10;
+constexpr int BC_WALL_INVISCID    = 20;
+constexpr int BC_WALL_VISCOUS     = 25;
+constexpr int BC_INTER_CURV       = 30;
+constexpr int BC_INTER_STRAIGHT   = 35;
+constexpr int BC_INTER_SUBDOMAINS = 40;
+constexpr int BC_PERIODIC         = 45;
+constexpr double gam      = 1.4;
+constexpr double gamm1    = gam - 1.0;
+constexpr double ham      = 1.0 / gam;
+constexpr double hamm1    = 1.0 / gamm1;
+constexpr double hamhamm1 = ham * hamm1;
+constexpr double prndtl   = 0.71;
+constexpr double gamm1prndtli = 1.0 / (gamm1 * prndtl);
+
+struct Geo {
+    long lxi, let, lze, lmx;
+    long n[3];
+    Geo(int x, int y, int z) : lxi(x), let(y), lze(z), lmx((long)x * y * z) {
+        n[0] = x; n[1] = y; n[2] = z;
+    }
+    long face_size(int a) const { return a == 0 ? let * lze : (a == 1 ? lxi * lze : lxi * let); }
+    long stride(int a) const { return a == 0 ? 1 : (a == 1 ? lxi : lxi * let); }
+    // first point of the line with face index f (transforms.hpp:108-142 face ordering)
+    long base(int a, long f) const {
+        if (a == 0) return f * lxi;                       // f = k*let + j
+        if (a == 1) return (f / lxi) * lxi * let + (f % lxi); // f = k*lxi + i
+        return f;                                         // f = j*lxi + i
+    }
+};
+
+// ndf mapping: numerics_base.hpp:73-93
+int ndf_of(int code) {
+    if (code == BC_NON_REFLECTIVE || code == BC_WALL_INVISCID ||
+        code == BC_WALL_VISCOUS || code == BC_INTER_CURV) return 0;
+    if (code == BC_INTER_STRAIGHT || code == BC_INTER_SUBDOMAINS || code == BC_PERIODIC) return 1;
+    return -1;
+}
+
+// ---- A4: interface coefficients, numerics_base.hpp:182-297 ----------------
+int maxloc(const double* p, int size) {            // numerics_base.hpp:309-322
+    int imax = 0; double dmax = p[0];
+    for (int i = 1; i < size; i++) if (p[i] > dmax) { dmax = p[i]; imax = i; }
+    return imax;
+}
+
+// Gauss-Jordan inverse with row swaps, numerics_base.hpp:324-386 (flat indexing kept)
+void mtrxi(const double* mtrx, double* imtrx, int n) {
+    const int size = n * n;
+    std::vector<int> ipvt(n);
+    std::vector<double> rx(size), arx(size), temp(n);
+    for (int i = 0; i < size; i++) rx[i] = mtrx[i];
+    for (int i = 0; i < n; i++) ipvt[i] = i;
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < size; k++) arx[k] = std::fabs(rx[k]);
+        int loc = i * n + i;
+        int imax = maxloc(arx.data() + loc, n - i - 1);
+        int mk = i + imax;
+        if (mk != i) {
+            std::swap(ipvt[mk], ipvt[i]);
+            for (int l = 0; l < n; l++) std::swap(rx[mk + l * n], rx[i + l * n]);
+        }
+        double ra0 = 1.0 / rx[i + i * n];
+        for (int k = 0; k < n; k++) temp[k] = rx[k + i * n];
+        for (int k = 0; k < n; k++) {
+            double ra1 = ra0 * rx[i + k * n];
+            for (int l = 0; l < n; l++) rx[l + k * n] = rx[l + k * n] - ra1 * temp[l];
+            rx[i + k * n] = ra1;
+        }
+        for (int k = 0; k < n; k++) rx[k + i * n] = -ra0 * temp[k];
+        rx[i + i * n] = ra0;
+    }
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) imtrx[i + ipvt[j] * n] = rx[i + j * n];
+}
+
+// numerics_base.hpp:388-398
+void matmul_square(const double* m1, const double* m2, double* r, int n) {
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            double s = 0.0;
+            for (int k = 0; k < n; k++) s += m1[k + j * n] * m2[i + k * n];
+            r[i + j * n] = s;
+        }
+}
+
+// ---- A7: tridiagonal rows of one line, numerics_device_pc.hpp:176-293 -----
+// recv4 = {face0 row0, face0 row1, face1 row0, face1 row1} for this line.
+void build_deriv_rows(int N, const double* sx, int nstart, int nend, const double* recv4,
+                      const double* pbci, double* sa, double* sb, double* sc, double* srhs) {
+    for (int t = 0; t < N; t++) {
+        if (t == 0) {
+            if (nstart == 0) {
+                srhs[0] = (ab00 * sx[0] + ab01 * sx[1] + ab02 * sx[2]);
+                sa[0] = 0.0; sb[0] = 1.0; sc[0] = alpha01;
+            } else {
+                double s = 0;
+                for (int i = 0; i < lmd; i++) s += pbci[i] * sx[i];
+                s += recv4[0];
+                srhs[0] = s; sa[0] = 0.0; sb[0] = 1.0; sc[0] = alpha;
+            }
+        } else if (t == 1) {
+            if (nstart == 0) {
+                srhs[1] = ab10 * (sx[2] - sx[0]);
+                sa[1] = alpha10; sb[1] = 1.0; sc[1] = alpha10;
+            } else {
+                double s = 0;
+                for (int i = 0; i < lmd; i++) s += pbci[lmd + i] * sx[i];
+                s += recv4[1];
+                srhs[1] = s; sa[1] = alpha; sb[1] = 1.0; sc[1] = alpha;
+            }
+        } else if (t == N - 1) {
+            if (nend == 0) {
+                srhs[N - 1] = -(ab00 * sx[N - 1] + ab01 * sx[N - 2] + ab02 * sx[N - 3]);
+                sa[N - 1] = alpha01; sb[N - 1] = 1.0; sc[N - 1] = 0.0;
+            } else {
+                double s = 0;
+                for (int i = 0; i < lmd; i++) s += pbci[i] * sx[N - 1 - i];
+                s += recv4[2];
+                srhs[N - 1] = -s; sa[N - 1] = alpha; sb[N - 1] = 1.0; sc[N - 1] = 0.0;
+            }
+        } else if (t == N - 2) {
+            if (nend == 0) {
+                srhs[N - 2] = -ab10 * (sx[N - 3] - sx[N - 1]);
+                sa[N - 2] = alpha10; sb[N - 2] = 1.0; sc[N - 2] = alpha10;
+            } else {
+                double s = 0;
+                for (int i = 0; i < lmd; i++) s += pbci[lmd + i] * sx[N - 1 - i];
+                s += recv4[3];
+                srhs[N - 2] = -s; sa[N - 2] = alpha; sb[N - 2] = 1.0; sc[N - 2] = alpha;
+            }
+        } else {
+            srhs[t] = aa * (sx[t + 1] - sx[t - 1]) + bb * (sx[t + 2] - sx[t - 2]);
+            sa[t] = alpha; sb[t] = 1.0; sc[t] = alpha;
+        }
+    }
+}
+
+// ---- A8: PCR, lock-step emulation of numerics_device_pc.hpp:121-174 --------
+// Valid for N a power of two (D5); arrays are modified in place.
+void pcr_solve(int N, double* sa, double* sb, double* sc, double* srhs, double* sx) {
+    int iter = (int)std::log2((double)(N / 2));
+    int stride = 1;
+    std::vector<double> ta(N), tb(N), tc(N), tr(N);
+    for (int j = 0; j < iter; j++) {
+        for (int t = 0; t < N; t++) {
+            int right = t + stride; if (right >= N) right = N - 1;
+            int left = t - stride;  if (left < 0) left = 0;
+            double k1 = sa[t] / sb[left];
+            double k2 = sc[t] / sb[right];
+            tb[t] = sb[t] - sc[left] * k1 - sa[right] * k2;
+            tr[t] = srhs[t] - srhs[left] * k1 - srhs[right] * k2;
+            ta[t] = -sa[left] * k1;
+            tc[t] = -sc[right] * k2;
+        }
+        for (int t = 0; t < N; t++) { sb[t] = tb[t]; srhs[t] = tr[t]; sa[t] = ta[t]; sc[t] = tc[t]; }
+        stride <<= 1;
+    }
+    for (int t = 0; t < N / 2; t++) {
+        int i = t, j = t + stride;
+        double det = sb[j] * sb[i] - sc[i] * sa[j];
+        det = 1.0 / det;
+        sx[i] = (sb[j] * srhs[i] - sc[i] * srhs[j]) * det;
+        sx[j] = (srhs[j] * sb[i] - srhs[i] * sa[j]) * det;
+    }
+}
+
+// Thomas algorithm (textbook), used where the reference PCR is undefined (D5).
+void thomas_solve(int N, const double* a, const double* b, const double* c, const double* r, double* x) {
+    std::vector<double> cp(N), dp(N);
+    double w = 1.0 / b[0];
+    cp[0] = c[0] * w; dp[0] = r[0] * w;
+    for (int i = 1; i < N; i++) {
+        w = 1.0 / (b[i] - a[i] * cp[i - 1]);
+        cp[i] = c[i] * w;
+        dp[i] = (r[i] - a[i] * dp[i - 1]) * w;
+    }
+    x[N - 1] = dp[N - 1];
+    for (int i = N - 2; i >= 0; i--) x[i] = dp[i] - cp[i] * x[i + 1];
+}
+
+// Dense Gaussian elimination with partial pivoting (textbook; small N only).
+// A is row-major N x N and is overwritten.
+void dense_solve(int N, double* A, double* r, double* x) {
+    for (int k = 0; k < N; k++) {
+        int p = k; double m = std::fabs(A[k * N + k]);
+        for (int i = k + 1; i < N; i++) if (std::fabs(A[i * N + k]) > m) { m = std::fabs(A[i * N + k]); p = i; }
+        if (p != k) { for (int j = 0; j < N; j++) std::swap(A[k * N + j], A[p * N + j]); std::swap(r[k], r[p]); }
+        for (int i = k + 1; i < N; i++) {
+            double f = A[i * N + k] / A[k * N + k];
+            if (f == 0.0) continue;
+            for (int j = k; j < N; j++) A[i * N + j] -= f * A[k * N + j];
+            r[i] -= f * r[k];
+        }
+    }
+    for (int i = N - 1; i >= 0; i--) {
+        double s = r[i];
+        for (int j = i + 1; j < N; j++) s -= A[i * N + j] * x[j];
+        x[i] = s / A[i * N + i];
+    }
+}
+
+bool is_pow2(int n) { return n >= 2 && (n & (n - 1)) == 0; }
+
+// solver: 0 = reference PCR where valid (N = 2^k) else Thomas; 1 = Thomas; 2 = dense
+void solve_tridiag(int N, int solver, double* a, double* b, double* c, double* r, double* x) {
+    if (solver == 2) {
+        std::vector<double> A((size_t)N * N, 0.0), rr(r, r + N);
+        for (int i = 0; i < N; i++) {
+            if (i > 0) A[(size_t)i * N + i - 1] = a[i];
+            A[(size_t)i * N + i] = b[i];
+            if (i < N - 1) A[(size_t)i * N + i + 1] = c[i];
+        }
+        dense_solve(N, A.data(), rr.data(), x);
+    } else if (solver == 0 && is_pow2(N)) {
+        pcr_solve(N, a, b, c, r, x);
+    } else {
+        thomas_solve(N, a, b, c, r, x);
+    }
+}
+
+// ---- compact filter (build-defined; absent upstream, A17) -----------------
+// Centred tridiagonal filter of order 2M (Gaitonde & Visbal family):
+//   af*fh[i-1] + fh[i] + af*fh[i+1] = sum_{n=0..M} a_n/2 (f[i+n] + f[i-n]).
+// The a_n solve: sum a_n = 1+2af; sum_{n>=1} a_n n^{2m} = 2af (m=1..M-1);
+// sum (-1)^n a_n = 0 (transfer function zero at the grid Nyquist).
+void filter_coeffs(int M, double af, double* an) {
+    int n = M + 1;
+    std::vector<double> A((size_t)n * n, 0.0), r(n, 0.0), x(n);
+    for (int j = 0; j <= M; j++) A[j] = 1.0;           // m = 0 row
+    r[0] = 1.0 + 2.0 * af;
+    for (int m = 1; m < M; m++) {
+        for (int j = 1; j <= M; j++) A[(size_t)m * n + j] = std::pow((double)j, 2.0 * m);
+        r[m] = 2.0 * af;
+    }
+    for (int j = 0; j <= M; j++) A[(size_t)M * n + j] = (j % 2 == 0) ? 1.0 : -1.0;
+    r[M] = 0.0;
+    dense_solve(n, A.data(), r.data(), x.data());
+    for (int j = 0; j <= M; j++) an[j] = x[j];
+}
+
+// Rows of the non-periodic filter: identity at the end points, reduced-order
+// centred stencils (orders 2,4,6,8) on the next four rows, order 10 inside.
+void build_filter_rows(int N, const double* f, double af, const double (*an)[6],
+                       double* a, double* b, double* c, double* r) {
+    for (int t = 0; t < N; t++) {
+        int dist = std::min(t, N - 1 - t);
+        if (dist == 0) { a[t] = 0.0; b[t] = 1.0; c[t] = 0.0; r[t] = f[t]; continue; }
+        int M = std::min(dist, 5);
+        const double* cf = an[M];
+        double s = cf[0] * f[t];
+        for (int m = 1; m <= M; m++) s += 0.5 * cf[m] * (f[t + m] + f[t - m]);
+        a[t] = af; b[t] = 1.0; c[t] = af; r[t] = s;
+    }
+}
+
+// Periodic (cyclic) 10th-order filter rows; the wrap is exact.
+void build_filter_rows_cyclic(int N, const double* f, double af, const double* an10, double* r) {
+    for (int t = 0; t < N; t++) {
+        double s = an10[0] * f[t];
+        for (int m = 1; m <= 5; m++) s += 0.5 * an10[m] * (f[(t + m) % N] + f[(t - m + N) % N]);
+        r[t] = s;
+    }
+}
+
+// Cyclic tridiagonal (af, 1, af) solve by Sherman-Morrison (textbook).
+void cyclic_solve(int N, double af, const double* r, double* x) {
+    std::vector<double> a(N, af), b(N, 1.0), c(N, af), u(N, 0.0), y(N), z(N);
+    double gamma = -b[0];
+    double cornerA = af, cornerC = af;   // A[0][N-1] and A[N-1][0]
+    b[0] = b[0] - gamma;
+    b[N - 1] = b[N - 1] - cornerA * cornerC / gamma;
+    a[0] = 0.0; c[N - 1] = 0.0;
+    u[0] = gamma; u[N - 1] = cornerC;
+    thomas_solve(N, a.data(), b.data(), c.data(), r, y.data());
+    thomas_solve(N, a.data(), b.data(), c.data(), u.data(), z.data());
+    double fact = (y[0] + cornerA * y[N - 1] / gamma) / (1.0 + z[0] + cornerA * z[N - 1] / gamma);
+    for (int i = 0; i < N; i++) x[i] = y[i] - fact * z[i];
+}
+
+// ---- gcbc eigenvector matrices, src/cuda/kernels/gcbc.hpp:38-158 ------------
+struct Eig {
+    double ao, aoi, hv2, vn, vs, ve[3];
+    // gcbc.hpp:44-61
+    void eleme(const double* cm, double qa1, const double* qa24, double pp, const double* umf) {
+        double rhoi = 1.0 / qa1;
+        ao = std::sqrt(gam * rhoi * pp);
+        aoi = 1.0 / ao;
+        ve[0] = rhoi * qa24[0]; ve[1] = rhoi * qa24[1]; ve[2] = rhoi * qa24[2];
+        hv2 = 0.5 * (ve[0] * ve[0] + ve[1] * ve[1] + ve[2] * ve[2]);
+        vn = cm[0] * ve[0] + cm[1] * ve[1] + cm[2] * ve[2];
+        vs = cm[0] * umf[0] + cm[1] * umf[1] + cm[2] * umf[2];
+    }
+    // right eigenvectors (characteristic -> conservative), gcbc.hpp:63-105
+    void xtr2q(const double* cm, double xt[5][5]) const {
+        double bo = hv2 + hamm1 * ao * ao, co = ao * vn, dm[3] = {ao * cm[0], ao * cm[1], ao * cm[2]};
+        xt[0][0] = cm[0]; xt[0][1] = cm[1]; xt[0][2] = cm[2]; xt[0][3] = 0.5; xt[0][4] = 0.5;
+        xt[1][0] = cm[0] * ve[0]; xt[1][1] = cm[1] * ve[0] - dm[2]; xt[1][2] = cm[2] * ve[0] + dm[1];
+        xt[1][3] = 0.5 * (ve[0] + dm[0]); xt[1][4] = xt[1][3] - dm[0];
+        xt[2][0] = cm[0] * ve[1] + dm[2]; xt[2][1] = cm[1] * ve[1]; xt[2][2] = cm[2] * ve[1] - dm[0];
+        xt[2][3] = 0.5 * (ve[1] + dm[1]); xt[2][4] = xt[2][3] - dm[1];
+        xt[3][0] = cm[0] * ve[2] - dm[1]; xt[3][1] = cm[1] * ve[2] + dm[0]; xt[3][2] = cm[2] * ve[2];
+        xt[3][3] = 0.5 * (ve[2] + dm[2]); xt[3][4] = xt[3][3] - dm[2];
+        xt[4][0] = hv2 * cm[0] + dm[2] * ve[1] - dm[1] * ve[2];
+        xt[4][1] = hv2 * cm[1] + dm[0] * ve[2] - dm[2] * ve[0];
+        xt[4][2] = hv2 * cm[2] + dm[1] * ve[0] - dm[0] * ve[1];
+        xt[4][3] = 0.5 * (bo + co); xt[4][4] = xt[4][3] - co;
+    }
+    // left eigenvectors (conservative -> characteristic), gcbc.hpp:107-150.
+    // fixed = 1 applies the three D10 corrections (dm[2] uses cm[2];
+    // xt[0][4] uses cm[0]; xt[4][0] = 1 - bo + co); fixed = 0 is as shipped.
+    void xtq2r(const double* cm, double xt[5][5], int fixed) const {
+        double ho = gamm1 * aoi * aoi, bo = 1.0 - ho * hv2, co = aoi * vn;
+        double dm[3] = {aoi * cm[0], aoi * cm[1], aoi * (fixed ? cm[2] : cm[1])};
+        double rv[3] = {ho * ve[0], ho * ve[1], ho * ve[2]};
+        xt[0][0] = bo * cm[0] + dm[1] * ve[2] - dm[2] * ve[1];
+        xt[0][1] = cm[0] * rv[0]; xt[0][2] = cm[0] * rv[1] + dm[2]; xt[0][3] = cm[0] * rv[2] - dm[1];
+        xt[0][4] = -ho * (fixed ? cm[0] : cm[1]);
+        xt[1][0] = bo * cm[1] + dm[2] * ve[0] - dm[0] * ve[2];
+        xt[1][1] = cm[1] * rv[0] - dm[2]; xt[1][2] = cm[1] * rv[1]; xt[1][3] = cm[1] * rv[2] + dm[0];
+        xt[1][4] = -ho * cm[1];
+        xt[2][0] = bo * cm[2] + dm[0] * ve[1] - dm[1] * ve[0];
+        xt[2][1] = cm[2] * rv[0] + dm[1]; xt[2][2] = cm[2] * rv[1] - dm[0]; xt[2][3] = cm[2] * rv[2];
+        xt[2][4] = -ho * cm[2];
+        xt[3][0] = 1.0 - bo - co; xt[3][1] = dm[0] - rv[0]; xt[3][2] = dm[1] - rv[1];
+        xt[3][3] = dm[2] - rv[2]; xt[3][4] = ho;
+        xt[4][0] = fixed ? (1.0 - bo + co) : (1.0 - bo - co);
+        xt[4][1] = -dm[0] - rv[0]; xt[4][2] = -dm[1] - rv[1]; xt[4][3] = -dm[2] - rv[2]; xt[4][4] = ho;
+    }
+};
+
+void matvec5(const double m[5][5], const double* v, double* r) {   // kernels/common.hpp:36-48
+    for (int i = 0; i < 5; i++) {
+        double s = 0;
+        for (int j = 0; j < 5; j++) s += m[i][j] * v[j];
+        r[i] = s;
+    }
+}
+
+} // namespace
+
+// ============================================================================
+// Context (mirrors numerics_base + physics_base state on one subdomain)
+// ============================================================================
+struct orc_ctx {
+    Geo g;
+    int nbc[2][3];
+    int ndf[2][3];
+    int solver;
+    double pbco[16], pbci[16];
+    double af;                 // filter alpha
+    double an[6][6];           // filter coefficients by half-width M (1..5)
+    std::vector<double> drva[3], send[3], recv[3];
+    std::vector<double> buffer, buffer_deriv, ss, stress, heat;
+    orc_ctx(int x, int y, int z) : g(x, y, z) {}
+};
+
+extern "C" {
+
+// A4: numerics_base::deriv_setup (numerics_base.hpp:182-297).  corrected = 1
+// uses the D1-corrected right-end closure row of Q; 0 is the as-shipped table.
+void orc_interface_table(int corrected, double* pbco, double* pbci) {
+    const int n = 2 * lmd;
+    std::vector<double> Q(n * n, 0.0), P(n * n, 0.0), P1(n * n, 0.0), tmp(n * n), rslt(n * n);
+    auto q = [&](int i, int j) -> double& { return Q[i * n + j]; };
+    auto p = [&](int i, int j) -> double& { return P[i * n + j]; };
+    q(0, 0) = ab00; q(0, 1) = ab01; q(0, 2) = ab02;
+    q(1, 0) = -ab10; q(1, 1) = 0; q(1, 2) = ab10;
+    if (corrected) { q(n - 1, n - 1) = -ab00; q(n - 1, n - 2) = -ab01; q(n - 1, n - 3) = -ab02; }
+    else           { q(n - 1, n - 1) = -ab02; q(n - 1, n - 2) = -ab01; q(n - 1, n - 3) = -ab00; }
+    q(n - 2, n - 1) = ab10; q(n - 2, n - 2) = 0.0; q(n - 2, n - 3) = -ab10;
+    for (int i = 2; i < n - 2; i++) { q(i, i) = 0; q(i, i - 1) = -aa; q(i, i + 1) = aa; q(i, i - 2) = -bb; q(i, i + 2) = bb; }
+    p(0, 0) = 1.0; p(0, 1) = alpha01;
+    p(n - 1, n - 1) = 1.0; p(n - 1, n - 2) = alpha01;
+    p(1, 0) = alpha10; p(1, 1) = 1.0; p(1, 2) = alpha10;
+    p(n - 2, n - 1) = alpha10; p(n - 2, n - 2) = 1.0; p(n - 2, n - 3) = alpha10;
+    for (int i = 2; i < n - 2; i++) { p(i, i) = 1.0; p(i, i - 1) = alpha; p(i, i + 1) = alpha; }
+    mtrxi(P.data(), P1.data(), n);
+    int i = n / 2 - 2;                // R = P decoupled between rows 7 and 8
+    p(i, i + 2) = 0.0; p(i + 1, i + 2) = 0.0; p(i + 1, i + 3) = 0.0;
+    i = n / 2 + 1;
+    p(i, i - 2) = 0.0; p(i - 1, i - 2) = 0.0; p(i - 1, i - 3) = 0.0;
+    matmul_square(P1.data(), Q.data(), tmp.data(), n);
+    matmul_square(P.data(), tmp.data(), rslt.data(), n);
+    for (int t = 0; t < lmd; t++) {
+        int j = n / 2;
+        pbco[lmd - 1 - t] = rslt[j * n + t];
+        pbci[t] = rslt[j * n + lmd + t];
+    }
+    for (int t = 0; t < lmd; t++) {
+        int j = n / 2 + 1;
+        pbco[lmd + lmd - 1 - t] = rslt[j * n + t];
+        pbci[lmd + t] = rslt[j * n + lmd + t];
+    }
+}
+
+// Line-level entry points (tests use them on small cases).
+void orc_build_deriv_rows(int N, const double* f, int nstart, int nend, const double* recv4,
+                          const double* pbci, double* a, double* b, double* c, double* r) {
+    build_deriv_rows(N, f, nstart, nend, recv4, pbci, a, b, c, r);
+}
+
+// solver: 0 = reference PCR (N = 2^k), 1 = Thomas, 2 = dense.  Returns -1 if
+// PCR is asked for a length it cannot solve (D5).
+int orc_solve_tridiag(int N, int solver, double* a, double* b, double* c, double* r, double* x) {
+    if (solver == 0 && !is_pow2(N)) return -1;
+    solve_tridiag(N, solver, a, b, c, r, x);
+    return 0;
+}
+
+void orc_filter_coeffs(int M, double af, double* an) { filter_coeffs(M, af, an); }
+
+void orc_cyclic_solve(int N, double af, const double* r, double* x) { cyclic_solve(N, af, r, x); }
+
+// gcbc_vgpr::eleme + xtr2q + xtq2r on one point (gcbc.hpp:38-158).
+void orc_gcbc_matrices(const double* cm, double rho, const double* mom, double p,
+                       const double* umf, int fixed, double* R, double* L, double* info) {
+    Eig e; e.eleme(cm, rho, mom, p, umf);
+    double r[5][5], l[5][5];
+    e.xtr2q(cm, r); e.xtq2r(cm, l, fixed);
+    std::memcpy(R, r, sizeof r); std::memcpy(L, l, sizeof l);
+    if (info) { info[0] = e.ao; info[1] = e.vn; info[2] = e.vs; }
+}
+
+// ---- context -----------------------------------------------------------
+// nbc is [2][3] row-major (face, axis).  table: 0 = as shipped, 1 = D1-corrected.
+orc_ctx* orc_create(int lxi, int let, int lze, const int* nbc, int table, int solver, double af) {
+    orc_ctx* c = new orc_ctx(lxi, let, lze);
+    for (int ip = 0; ip < 2; ip++)
+        for (int nn = 0; nn < 3; nn++) {
+            c->nbc[ip][nn] = nbc[ip * 3 + nn];
+            c->ndf[ip][nn] = ndf_of(c->nbc[ip][nn]);
+        }
+    c->solver = solver;
+    orc_interface_table(table, c->pbco, c->pbci);
+    c->af = af;
+    std::memset(c->an, 0, sizeof c->an);
+    for (int M = 1; M <= 5; M++) filter_coeffs(M, af, c->an[M]);
+    for (int a = 0; a < 3; a++) {
+        long fs = c->g.face_size(a);
+        c->drva[a].assign(2 * NV * fs, 0.0);
+        c->send[a].assign(NV * 4 * fs, 0.0);
+        c->recv[a].assign(NV * 4 * fs, 0.0);
+    }
+    return c;
+}
+
+void orc_destroy(orc_ctx* c) { delete c; }
+
+void orc_set_interface_table(orc_ctx* c, const double* pbco, const double* pbci) {
+    std::memcpy(c->pbco, pbco, sizeof c->pbco); std::memcpy(c->pbci, pbci, sizeof c->pbci);
+}
+
+double* orc_drva(orc_ctx* c, int axis) { return c->drva[axis].data(); }
+double* orc_send(orc_ctx* c, int axis) { return c->send[axis].data(); }
+double* orc_recv(orc_ctx* c, int axis) { return c->recv[axis].data(); }
+double* orc_stress(orc_ctx* c) { return c->stress.empty() ? nullptr : c->stress.data(); }
+double* orc_heat(orc_ctx* c) { return c->heat.empty() ? nullptr : c->heat.data(); }
+double* orc_buffer(orc_ctx* c) { return c->buffer.empty() ? nullptr : c->buffer.data(); }
+double* orc_buffer_deriv(orc_ctx* c) { return c->buffer_deriv.empty() ? nullptr : c->buffer_deriv.data(); }
+
+// A5: fill_gpu_buffer_kernel for one face and one variable
+// (kernels/numerics_pc.hpp:42-122; istart/increment from numerics_pc.hpp:275-280).
+void orc_fill(orc_ctx* c, int axis, const double* in, int var) {
+    const Geo& g = c->g;
+    long fs = g.face_size(axis), st = g.stride(axis);
+    int dim = (int)g.n[axis];
+    double* send = c->send[axis].data() + (long)var * 4 * fs;
+    for (int ip = 0; ip < 2; ip++) {
+        if (c->ndf[ip][axis] != 1) continue;
+        int istart = ip * (dim - 1), inc = 1 - 2 * ip;
+        double* out = send + ip * 2 * fs;
+        #pragma omp parallel for schedule(static)
+        for (long f = 0; f < fs; f++) {
+            long b = g.base(axis, f);
+            double v0 = 0, v1 = 0;
+            for (int i = 0; i < lmd; i++) {
+                double x = in[b + (long)(istart + inc * i) * st];
+                v0 += c->pbco[i] * x;
+                v1 += c->pbco[lmd + i] * x;
+            }
+            out[f] = v0; out[fs + f] = v1;
+        }
+    }
+}
+
+// A6: mpigo with the neighbour equal to this rank (single subdomain):
+// recv[var][ip] = send[var][1-ip] (tags itag+iq -> itag+ip, numerics_base.hpp:157-175).
+void orc_mpigo(orc_ctx* c, int var) {
+    for (int a = 0; a < 3; a++) {
+        long fs = c->g.face_size(a);
+        double* s = c->send[a].data() + (long)var * 4 * fs;
+        double* r = c->recv[a].data() + (long)var * 4 * fs;
+        for (int ip = 0; ip < 2; ip++)
+            if (c->ndf[ip][a] == 1)
+                std::memcpy(r + ip * 2 * fs, s + (1 - ip) * 2 * fs, sizeof(double) * 2 * fs);
+    }
+}
+
+// A9: deriv_kernel_2d<Axis> (kernels/numerics_pc.hpp:174-249) on all lines,
+// x-lines indexed as in numerics_device_rtc.hpp:266-268 (D2 intended).
+// in/out point at the component to read / write.  nstart/nend select closure
+// (0) or seam (1) rows; the seam rows read recv[axis] of variable var.
+void orc_deriv(orc_ctx* c, int axis, const double* in, double* out, int nstart, int nend, int var) {
+    const Geo& g = c->g;
+    int N = (int)g.n[axis];
+    long fs = g.face_size(axis), st = g.stride(axis);
+    const double* recv = c->recv[axis].data() + (long)var * 4 * fs;
+    double* drva = c->drva[axis].data();
+    #pragma omp parallel
+    {
+        std::vector<double> sx(N), a(N), b(N), cc(N), r(N), x(N);
+        #pragma omp for schedule(static)
+        for (long f = 0; f < fs; f++) {
+            long base = g.base(axis, f);
+            for (int t = 0; t < N; t++) sx[t] = in[base + t * st];
+            double recv4[4] = {recv[f], recv[fs + f], recv[2 * fs + f], recv[3 * fs + f]};
+            build_deriv_rows(N, sx.data(), nstart, nend, recv4, c->pbci, a.data(), b.data(), cc.data(), r.data());
+            solve_tridiag(N, c->solver, a.data(), b.data(), cc.data(), r.data(), x.data());
+            for (int t = 0; t < N; t++) out[base + t * st] = x[t];
+            if (var >= 0 && var < NV) {
+                drva[(long)var * fs + f] = x[0];
+                drva[(long)NV * fs + (long)var * fs + f] = x[N - 1];
+            }
+        }
+    }
+}
+
+// Convenience: numerics_pc::deriv1d on a scalar field with this context's BCs
+// (fill + self mpigo + deriv), as the reference test drives it.
+void orc_deriv_field(orc_ctx* c, int axis, const double* in, double* out, int var) {
+    orc_fill(c, axis, in, var);
+    orc_mpigo(c, var);
+    orc_deriv(c, axis, in, out, c->ndf[0][axis], c->ndf[1][axis], var);
+}
+
+// Compact filter along one axis (build-defined, absent upstream).  Periodic
+// when both faces of the axis are seam codes (single subdomain wraps), else
+// reduced-order centred closures.
+void orc_filter(orc_ctx* c, int axis, const double* in, double* out) {
+    const Geo& g = c->g;
+    int N = (int)g.n[axis];
+    long fs = g.face_size(axis), st = g.stride(axis);
+    bool cyc = c->ndf[0][axis] == 1 && c->ndf[1][axis] == 1;
+    #pragma omp parallel
+    {
+        std::vector<double> sx(N), a(N), b(N), cc(N), r(N), x(N);
+        #pragma omp for schedule(static)
+        for (long f = 0; f < fs; f++) {
+            long base = g.base(axis, f);
+            for (int t = 0; t < N; t++) sx[t] = in[base + t * st];
+            if (cyc) {
+                build_filter_rows_cyclic(N, sx.data(), c->af, c->an[5], r.data());
+                cyclic_solve(N, c->af, r.data(), x.data());
+            } else {
+                build_filter_rows(N, sx.data(), c->af, c->an, a.data(), b.data(), cc.data(), r.data());
+                thomas_solve(N, a.data(), b.data(), cc.data(), r.data(), x.data());
+            }
+            for (int t = 0; t < N; t++) out[base + t * st] = x[t];
+        }
+    }
+}
+
+// ---- physics pointwise kernels ------------------------------------------
+// Primitive variables (build-defined, A19; layout the reference kernels
+// expect, kernels/physics.hpp:106-108,395): de0 = 1/rho, de1..3 = u,v,w,
+// p = (gam-1)(E - 0.5 rho |u|^2), de4 = c^2 = gam p / rho.
+void orc_primitives(long lmx, const double* qa, double* de, double* p) {
+    #pragma omp parallel for schedule(static)
+    for (long i = 0; i < lmx; i++) {
+        double rhoi = 1.0 / qa[i];
+        double u = qa[i + lmx] * rhoi, v = qa[i + 2 * lmx] * rhoi, w = qa[i + 3 * lmx] * rhoi;
+        double pp = gamm1 * (qa[i + 4 * lmx] - 0.5 * (qa[i + lmx] * u + qa[i + 2 * lmx] * v + qa[i + 3 * lmx] * w));
+        de[i] = rhoi; de[i + lmx] = u; de[i + 2 * lmx] = v; de[i + 3 * lmx] = w;
+        de[i + 4 * lmx] = gam * pp * rhoi;
+        p[i] = pp;
+    }
+}
+
+// calc_fluxes_pre_compute_kernel (kernels/physics.hpp:42-193).
+// stress = [6][lmx] (xx,yy,zz,xy,yz,zx), heat = [3][lmx] (xx,yy,zz); both may be
+// null when visc = 0 (D21 intended: the inviscid path does not read them).
+void orc_fluxes_pre(int visc, long lmx, double* buffer, const double* qa, const double* p, double* de,
+                    const double* xim, const double* etm, const double* zem,
+                    const double* stress, const double* heat, const double* umf) {
+    #pragma omp parallel for schedule(static)
+    for (long t = 0; t < lmx; t++) {
+        double dv[5], xm[3], em[3], zm[3], q[5];
+        for (int v = 0; v < 5; v++) { dv[v] = de[t + v * lmx]; q[v] = qa[t + v * lmx]; }
+        for (int d = 0; d < 3; d++) { xm[d] = xim[t + d * lmx]; em[d] = etm[t + d * lmx]; zm[d] = zem[t + d * lmx]; }
+        double txx = 0, tyy = 0, tzz = 0, txy = 0, tyz = 0, tzx = 0, hxx = 0, hyy = 0, hzz = 0;
+        if (visc) {
+            txx = stress[t]; tyy = stress[t + lmx]; tzz = stress[t + 2 * lmx];
+            txy = stress[t + 3 * lmx]; tyz = stress[t + 4 * lmx]; tzx = stress[t + 5 * lmx];
+            hxx = heat[t]; hyy = heat[t + lmx]; hzz = heat[t + 2 * lmx];
+        }
+        double pv = p[t], rr[3], ss[3];
+        rr[0] = dv[1] + umf[0]; rr[1] = dv[2] + umf[1]; rr[2] = dv[3] + umf[2];
+        ss[0] = xm[0] * rr[0] + xm[1] * rr[1] + xm[2] * rr[2];
+        ss[1] = em[0] * rr[0] + em[1] * rr[1] + em[2] * rr[2];
+        ss[2] = zm[0] * rr[0] + zm[1] * rr[1] + zm[2] * rr[2];
+        const double* mt[3] = {xm, em, zm};
+        for (int j = 0; j < 3; j++) buffer[t + j * lmx + 0 * 3 * lmx] = q[0] * ss[j];
+        for (int j = 0; j < 3; j++) {
+            double r = q[1] * ss[j] + mt[j][0] * pv;
+            if (visc) r -= (mt[j][0] * txx + mt[j][1] * txy + mt[j][2] * tzx);
+            buffer[t + j * lmx + 1 * 3 * lmx] = r;
+        }
+        for (int j = 0; j < 3; j++) {
+            double r = q[2] * ss[j] + mt[j][1] * pv;
+            if (visc) r -= (mt[j][0] * txy + mt[j][1] * tyy + mt[j][2] * tyz);
+            buffer[t + j * lmx + 2 * 3 * lmx] = r;
+        }
+        for (int j = 0; j < 3; j++) {
+            double r = q[3] * ss[j] + mt[j][2] * pv;
+            if (visc) r -= (mt[j][0] * tzx + mt[j][1] * tyz + mt[j][2] * tzz);
+            buffer[t + j * lmx + 3 * 3 * lmx] = r;
+        }
+        double ep = q[4] + pv;
+        for (int j = 0; j < 3; j++) {
+            double r = ep * ss[j] - pv * (umf[0] * mt[j][0] + umf[1] * mt[j][1] + umf[2] * mt[j][2]);
+            if (visc) r -= (mt[j][0] * hxx + mt[j][1] * hyy + mt[j][2] * hzz);
+            buffer[t + j * lmx + 4 * 3 * lmx] = r;
+        }
+        de[t + 4 * lmx] = ep;
+    }
+}
+
+// calc_time_step (kernels/physics.hpp:335-427, tests/test_physics.cu:57-117),
+// D9 intended: FP64 max with accumulators starting at 0.  out[0..1] = the two
+// maxima; returns dte = min(cfl/out0, 0.5/out1).
+double orc_time_step(long lmx, const double* xim, const double* etm, const double* zem,
+                     const double* de, const double* yaco, const double* ssk, const double* umf,
+                     double cfl, double* out) {
+    double m0 = 0.0, m1 = 0.0;
+    #pragma omp parallel for reduction(max:m0, m1) schedule(static)
+    for (long t = 0; t < lmx; t++) {
+        double rr1 = 0.0;
+        for (int d = 0; d < 3; d++) rr1 += xim[t + d * lmx] * xim[t + d * lmx];
+        for (int d = 0; d < 3; d++) rr1 += etm[t + d * lmx] * etm[t + d * lmx];
+        for (int d = 0; d < 3; d++) rr1 += zem[t + d * lmx] * zem[t + d * lmx];
+        double u = de[t + lmx] + umf[0], v = de[t + 2 * lmx] + umf[1], w = de[t + 3 * lmx] + umf[2];
+        double rr2 = std::fabs(xim[t] * u + xim[t + lmx] * v + xim[t + 2 * lmx] * w) +
+                     std::fabs(etm[t] * u + etm[t + lmx] * v + etm[t + 2 * lmx] * w) +
+                     std::fabs(zem[t] * u + zem[t + lmx] * v + zem[t + 2 * lmx] * w);
+        double ssi = std::fabs(yaco[t]);
+        double r0 = (std::sqrt(de[t + 4 * lmx] * rr1) + rr2) * ssi;
+        double r1 = de[t] * ssk[t] * rr1 * ssi * ssi;
+        m0 = std::max(m0, r0); m1 = std::max(m1, r1);
+    }
+    if (out) { out[0] = m0; out[1] = m1; }
+    return std::min(cfl / m0, 0.5 / m1);
+}
+
+// ---- physics::calc_viscous_shear_stress (physics_pc.hpp:151-227) -----------
+// Intended semantics: D6 (seam fill from component 0), D7 (axis-matched BC
+// rows), D8 (post kernel reads the derivatives).  Writes the context's
+// stress [6][lmx] and heat [3][lmx]; overwrites de0 = ssk, de4 = 2/3 div u
+// (kernels/physics.hpp:285-333).
+void orc_calc_viscous(orc_ctx* c, double* de, const double* ssk, const double* xim,
+                      const double* etm, const double* zem, const double* yaco) {
+    const Geo& g = c->g; long lmx = g.lmx;
+    c->buffer.resize(15 * lmx); c->buffer_deriv.resize(15 * lmx);
+    c->ss.resize(3 * lmx); c->stress.resize(6 * lmx); c->heat.resize(3 * lmx);
+    double* buf = c->buffer.data(); double* bd = c->buffer_deriv.data();
+    double* S = c->stress.data(); double* H = c->heat.data(); double* SS = c->ss.data();
+    // init kernel: buffer[m][0] = de[m], m = 1..4 (kernels/physics.hpp:211-225)
+    for (int m = 1; m < 5; m++) std::memcpy(buf + (long)m * 3 * lmx, de + (long)m * lmx, sizeof(double) * lmx);
+    for (int m = 1; m < 5; m++) {
+        const double* src = buf + (long)m * 3 * lmx;
+        for (int a = 0; a < 3; a++) orc_fill(c, a, src, m);   // nrt = 1: component 0
+        orc_mpigo(c, m);
+        for (int a = 0; a < 3; a++)
+            orc_deriv(c, a, src, bd + (long)m * 3 * lmx + (long)a * lmx, c->ndf[0][a], c->ndf[1][a], m);
+        // post kernel (kernels/physics.hpp:228-283) with the reference's slot map
+        double *o0, *o1, *o2;
+        if (m == 1)      { o0 = S + 0 * lmx; o1 = H + 2 * lmx; o2 = S + 5 * lmx; }
+        else if (m == 2) { o0 = S + 3 * lmx; o1 = S + 1 * lmx; o2 = H + 0 * lmx; }
+        else if (m == 3) { o0 = H + 1 * lmx; o1 = S + 4 * lmx; o2 = S + 2 * lmx; }
+        else             { o0 = SS + 0 * lmx; o1 = SS + 1 * lmx; o2 = SS + 2 * lmx; }
+        const double* d0 = bd + (long)m * 3 * lmx;
+        #pragma omp parallel for schedule(static)
+        for (long t = 0; t < lmx; t++) {
+            double f0 = d0[t], f1 = d0[t + lmx], f2 = d0[t + 2 * lmx];
+            o0[t] = xim[t] * f0 + etm[t] * f1 + zem[t] * f2;
+            o1[t] = xim[t + lmx] * f0 + etm[t + lmx] * f1 + zem[t + lmx] * f2;
+            o2[t] = xim[t + 2 * lmx] * f0 + etm[t + 2 * lmx] * f1 + zem[t + 2 * lmx] * f2;
+        }
+    }
+    // final kernel (kernels/physics.hpp:285-333)
+    #pragma omp parallel for schedule(static)
+    for (long t = 0; t < lmx; t++) {
+        double mu = ssk[t];
+        de[t] = mu;
+        double b0 = de[t] * yaco[t];
+        double b1 = gamm1prndtli * b0;
+        double d4 = 2.0 / 3.0 * (S[t] + S[t + lmx] + S[t + 2 * lmx]);
+        de[t + 4 * lmx] = d4;
+        double sxx = b0 * (2.0 * S[t] - d4);
+        double syy = b0 * (2.0 * S[t + lmx] - d4);
+        double szz = b0 * (2.0 * S[t + 2 * lmx] - d4);
+        double sxy = b0 * (S[t + 3 * lmx] + H[t + 2 * lmx]);
+        double syz = b0 * (S[t + 4 * lmx] + H[t]);
+        double szx = b0 * (S[t + 5 * lmx] + H[t + lmx]);
+        double u = de[t + lmx], v = de[t + 2 * lmx], w = de[t + 3 * lmx];
+        double hx = b1 * SS[t] + u * sxx + v * sxy + w * szx;
+        double hy = b1 * SS[t + lmx] + u * sxy + v * syy + w * syz;
+        double hz = b1 * SS[t + 2 * lmx] + u * szx + v * syz + w * szz;
+        S[t] = sxx; S[t + lmx] = syy; S[t + 2 * lmx] = szz;
+        S[t + 3 * lmx] = sxy; S[t + 4 * lmx] = syz; S[t + 5 * lmx] = szx;
+        H[t] = hx; H[t + lmx] = hy; H[t + 2 * lmx] = hz;
+    }
+}
+
+// ---- physics::calc_fluxes (physics_pc.hpp:83-149) ------------------------
+void orc_calc_fluxes(orc_ctx* c, int visc, const double* qa, const double* p, double* de,
+                     const double* xim, const double* etm, const double* zem, const double* umf) {
+    const Geo& g = c->g; long lmx = g.lmx;
+    c->buffer.resize(15 * lmx); c->buffer_deriv.resize(15 * lmx);
+    double* buf = c->buffer.data(); double* bd = c->buffer_deriv.data();
+    orc_fluxes_pre(visc, lmx, buf, qa, p, de, xim, etm, zem,
+                   visc ? c->stress.data() : nullptr, visc ? c->heat.data() : nullptr, umf);
+    for (int m = 0; m < 5; m++) {
+        for (int a = 0; a < 3; a++) orc_fill(c, a, buf + (long)m * 3 * lmx + (long)a * lmx, m);  // nzk = axis
+        orc_mpigo(c, m);
+        for (int a = 0; a < 3; a++)
+            orc_deriv(c, a, buf + (long)m * 3 * lmx + (long)a * lmx, bd + (long)m * 3 * lmx + (long)a * lmx,
+                      c->ndf[0][a], c->ndf[1][a], m);
+        // post kernel (kernels/physics.hpp:195-209)
+        const double* d0 = bd + (long)m * 3 * lmx;
+        double* o = de + (long)m * lmx;
+        #pragma omp parallel for schedule(static)
+        for (long t = 0; t < lmx; t++) o[t] = d0[t] + d0[t + lmx] + d0[t + 2 * lmx];
+    }
+}
+
+// ---- gcbc_go (gcbc.hpp:86-186; kernels/gcbc.hpp:251-695) ---------------------
+// Intended semantics: D10 (fixed left eigenvectors), D11 (out index = base +
+// v*lmx, sbcc at the point), D12 (one face entry per thread), D13 (face
+// offset dim-1 for the far face).  cm[a] = [2 face][3][face_size].
+// drvb[a] (same layout as drva) is only read for BC_INTER_CURV faces.
+// Returns -1 if an inter-curv face has no drvb.
+int orc_gcbc(orc_ctx* c, const double* cm0, const double* cm1, const double* cm2,
+             const double* drvb0, const double* drvb1, const double* drvb2,
+             const double* qa, double* de, const double* p, const double* yaco,
+             const double* sbcc, const double* umf, const double* dudtmf, double dtwi) {
+    const Geo& g = c->g; long lmx = g.lmx;
+    const double* cmv[3] = {cm0, cm1, cm2};
+    const double* drvb[3] = {drvb0, drvb1, drvb2};
+    // setup on every characteristic face first (gcbc.hpp:101-131)
+    for (int nn = 0; nn < 3; nn++) {
+        long fs = g.face_size(nn), st = g.stride(nn); int dim = (int)g.n[nn];
+        for (int ip = 0; ip < 2; ip++) {
+            int np = c->nbc[ip][nn];
+            if (!(np == BC_NON_REFLECTIVE || np == BC_WALL_INVISCID || np == BC_WALL_VISCOUS || np == BC_INTER_CURV)) continue;
+            int flag = (BC_WALL_INVISCID - np) * (BC_WALL_VISCOUS - np) * (BC_INTER_CURV - np) / 3000;
+            long off = (long)ip * (dim - 1);
+            double* drva = c->drva[nn].data();
+            #pragma omp parallel for schedule(static)
+            for (long f = 0; f < fs; f++) {
+                long idx = g.base(nn, f) + off * st;
+                double cmv_[3], dv[5], cha[5], res[5];
+                for (int d = 0; d < 3; d++) cmv_[d] = cmv[nn][(long)ip * 3 * fs + d * fs + f];
+                for (int v = 0; v < 5; v++) dv[v] = drva[(long)ip * 5 * fs + v * fs + f];
+                double mom[3] = {qa[idx + lmx], qa[idx + 2 * lmx], qa[idx + 3 * lmx]};
+                Eig e; e.eleme(cmv_, qa[idx], mom, p[idx], umf);
+                double L[5][5]; e.xtq2r(cmv_, L, 1);
+                for (int v = 0; v < 5; v++) cha[v] = yaco[idx] * (flag * dv[v] + (1 - flag) * de[idx + v * lmx]);
+                matvec5(L, cha, res);
+                for (int v = 0; v < 5; v++) drva[(long)ip * 5 * fs + v * fs + f] = res[v];
+            }
+        }
+    }
+    for (int nn = 0; nn < 3; nn++)
+        for (int ip = 0; ip < 2; ip++)
+            if (c->nbc[ip][nn] == BC_INTER_CURV && drvb[nn] == nullptr) return -1;
+    // updates (gcbc.hpp:160-185), faces in axis then face order
+    for (int nn = 0; nn < 3; nn++) {
+        long fs = g.face_size(nn), st = g.stride(nn); int dim = (int)g.n[nn];
+        for (int ip = 0; ip < 2; ip++) {
+            int np = c->nbc[ip][nn];
+            if (!(np == BC_NON_REFLECTIVE || np == BC_WALL_INVISCID || np == BC_WALL_VISCOUS || np == BC_INTER_CURV)) continue;
+            long off = (long)ip * (dim - 1);
+            const double* drva = c->drva[nn].data();
+            // face points are disjoint, but the mbci-deep updates of neighbouring
+            // lines never overlap within one face either: lines are independent.
+            #pragma omp parallel for schedule(static)
+            for (long f = 0; f < fs; f++) {
+                long idx = g.base(nn, f) + off * st;
+                double cmv_[3], dv[5], cha[5], res[5];
+                for (int d = 0; d < 3; d++) cmv_[d] = cmv[nn][(long)ip * 3 * fs + d * fs + f];
+                for (int v = 0; v < 5; v++) dv[v] = drva[(long)ip * 5 * fs + v * fs + f];
+                double mom[3] = {qa[idx + lmx], qa[idx + 2 * lmx], qa[idx + 3 * lmx]};
+                Eig e; e.eleme(cmv_, qa[idx], mom, p[idx], umf);
+                double ra0 = 1 - 2 * ip;
+                for (int v = 0; v < 5; v++) cha[v] = dv[v];
+                if (np == BC_NON_REFLECTIVE) {           // kernels/gcbc.hpp:419-436
+                    if (ra0 * (e.vn + e.vs + e.ao) > 0) cha[3] = 0;
+                    if (ra0 * (e.vn + e.vs - e.ao) > 0) cha[4] = 0;
+                    for (int v = 0; v < 5; v++) cha[v] -= dv[v];
+                } else if (np == BC_INTER_CURV) {         // kernels/gcbc.hpp:651-677
+                    double dh[5];
+                    for (int v = 0; v < 5; v++) dh[v] = drvb[nn][(long)ip * 5 * fs + v * fs + f];
+                    if (ra0 * (e.vn + e.vs) > 0) { cha[0] = dh[0]; cha[1] = dh[1]; cha[2] = dh[2]; }
+                    if (ra0 * (e.vn + e.vs + e.ao) > 0) cha[3] = dh[3];
+                    if (ra0 * (e.vn + e.vs - e.ao) > 0) cha[4] = dh[4];
+                    for (int v = 0; v < 5; v++) cha[v] -= dv[v];
+                } else {                                  // walls, kernels/gcbc.hpp:533-540
+                    double accum = cmv_[0] * dudtmf[0] + cmv_[1] * dudtmf[1] + cmv_[2] * dudtmf[2];
+                    cha[3 + ip] = cha[4 - ip] + 2 * ra0 * e.aoi * qa[idx] * (accum + dtwi * (e.vn + e.vs));
+                }
+                double R[5][5]; e.xtr2q(cmv_, R);
+                matvec5(R, cha, res);
+                int iq = 1 - 2 * ip;
+                for (int ii = 0; ii < mbci; ii++) {
+                    long o = idx + (long)ii * iq * st;
+                    for (int v = 0; v < 5; v++) de[o + v * lmx] += sbcc[o] * res[v];
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+// wall_condition_update (gcbc.hpp:188-228; kernels/gcbc.hpp:160-249), D13
+// intended for the far face.
+void orc_wall_condition_update(orc_ctx* c, double* qa, const int* npex, const double* umf) {
+    const Geo& g = c->g; long lmx = g.lmx;
+    for (int nn = 0; nn < 3; nn++) {
+        long fs = g.face_size(nn), st = g.stride(nn); int dim = (int)g.n[nn];
+        for (int ip = 0; ip < 2; ip++) {
+            int np = c->nbc[ip][nn];
+            if (np != BC_WALL_INVISCID && np != BC_WALL_VISCOUS) continue;
+            long off = (long)ip * (dim - 1);
+            for (long f = 0; f < fs; f++) {
+                long idx = g.base(nn, f) + off * st;
+                double q0 = qa[idx], q1 = qa[idx + lmx], q2 = qa[idx + 2 * lmx], q3 = qa[idx + 3 * lmx], q4 = qa[idx + 4 * lmx];
+                double v2 = q1 * q1 + q2 * q2 + q3 * q3;
+                double n = npex[idx];
+                if (np == BC_WALL_INVISCID) {
+                    qa[idx + 4 * lmx] = n * q4 + (1.0 - n) * (hamhamm1 * std::pow(q0, gam) + 0.5 * v2 / q0);
+                } else {
+                    double fctr = (1 - n) * q0;
+                    qa[idx + lmx] = n * q1 - fctr * umf[0];
+                    qa[idx + 2 * lmx] = n * q2 - fctr * umf[1];
+                    qa[idx + 3 * lmx] = n * q3 - fctr * umf[2];
+                    qa[idx + 4 * lmx] = n * q4 + (1 - n) * (hamhamm1 * q0 + 0.5 * v2 / q0);
+                }
+            }
+        }
+    }
+}
+
+// RK stage update (build-defined, A18): q = q0 - dtk * yaco * de for the 5
+// conserved variables (CANARD low-storage RK, weights dt/4, dt/3, dt/2, dt).
+void orc_rk_stage(long lmx, double* q, const double* q0, const double* de, const double* yaco, double dtk) {
+    #pragma omp parallel for schedule(static)
+    for (long t = 0; t < lmx; t++)
+        for (int v = 0; v < 5; v++) q[t + v * lmx] = q0[t + v * lmx] - dtk * yaco[t] * de[t + v * lmx];
+}
+
+// Full right-hand side in the driver's order (canard.cu:142-193 with gcbc on):
+// primitives -> viscous stress -> fluxes -> gcbc.  de receives the RHS
+// (dq/dt = -yaco * de); p receives the pressure.
+int orc_rhs(orc_ctx* c, int visc, const double* qa, double* de, double* p, const double* ssk,
+            const double* xim, const double* etm, const double* zem, const double* yaco,
+            const double* cm0, const double* cm1, const double* cm2, const double* sbcc,
+            const double* umf, const double* dudtmf, double dtwi) {
+    long lmx = c->g.lmx;
+    orc_primitives(lmx, qa, de, p);
+    if (visc) orc_calc_viscous(c, de, ssk, xim, etm, zem, yaco);
+    orc_calc_fluxes(c, visc, qa, p, de, xim, etm, zem, umf);
+    bool any = false;
+    for (int nn = 0; nn < 3; nn++) for (int ip = 0; ip < 2; ip++) if (ndf_of(c->nbc[ip][nn]) == 0) any = true;
+    if (any) return orc_gcbc(c, cm0, cm1, cm2, nullptr, nullptr, nullptr, qa, de, p, yaco, sbcc, umf, dudtmf, dtwi);
+    return 0;
+}
+
+// One RK4 step: time step from the primitives of q (canard.cu:146-161),
+// four stages with dtk = dt/(nkrk-nk+1), nk = 1..4 (D15 intended), then the
+// compact filter of all five conserved variables along x, y, z
+// (build-defined).  work = [5][lmx] scratch, q0 = [5][lmx] scratch.
+// If dt_in > 0 it is used instead of the computed time step.
+double orc_rk4_step(orc_ctx* c, int visc, double* q, double* q0, double* de, double* p, double* work,
+                    const double* ssk, const double* xim, const double* etm, const double* zem,
+                    const double* yaco, const double* cm0, const double* cm1, const double* cm2,
+                    const double* sbcc, const double* umf, const double* dudtmf, double cfl,
+                    double dt_in, int filter) {
+    long lmx = c->g.lmx;
+    std::memcpy(q0, q, sizeof(double) * 5 * lmx);
+    double dt = dt_in;
+    if (!(dt > 0)) {
+        orc_primitives(lmx, q, de, p);
+        dt = orc_time_step(lmx, xim, etm, zem, de, yaco, ssk, umf, cfl, nullptr);
+    }
+    for (int nk = 1; nk <= 4; nk++) {
+        double dtk = dt / (4 - nk + 1);
+        orc_rhs(c, visc, q, de, p, ssk, xim, etm, zem, yaco, cm0, cm1, cm2, sbcc, umf, dudtmf, 1.0 / dt);
+        orc_rk_stage(lmx, q, q0, de, yaco, dtk);
+    }
+    if (filter) {
+        for (int v = 0; v < 5; v++) {
+            double* f = q + (long)v * lmx;
+            orc_filter(c, 0, f, work); orc_filter(c, 1, work, f); orc_filter(c, 2, f, work);
+            std::memcpy(f, work, sizeof(double) * lmx);
+        }
+    }
+    return dt;
+}
+
+int orc_num_threads(void) { return omp_get_max_threads(); }
+void orc_set_num_threads(int n) { omp_set_num_threads(n); }
+
+} // extern "C"
