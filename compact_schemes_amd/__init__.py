@@ -34,6 +34,8 @@ EXPORTS = [
     "cs_physics_create", "cs_physics_destroy", "cs_physics_set_frame", "cs_physics_buffers",
     "cs_primitives", "cs_calc_fluxes", "cs_calc_viscous_shear_stress", "cs_calc_time_step",
     "cs_gcbc_go", "cs_wall_condition_update", "cs_rk_stage",
+    "cs_solver_create", "cs_solver_destroy", "cs_solver_rhs", "cs_solver_sweep", "cs_solver_time_step",
+    "cs_solver_rk4_step",
 ]
 
 
@@ -49,6 +51,12 @@ class Dims(ctypes.Structure):
 
 class Point(ctypes.Structure):
     _fields_ = [("x", ctypes.c_double), ("y", ctypes.c_double), ("z", ctypes.c_double)]
+
+
+class SolverConfig(ctypes.Structure):
+    _fields_ = [("dims", Dims), ("nbc", ctypes.c_int * 6), ("viscous", ctypes.c_int), ("umf", Point),
+                ("mu", ctypes.c_double), ("metric", ctypes.c_double * 3), ("yaco", ctypes.c_double),
+                ("cfl", ctypes.c_double), ("alpha_f", ctypes.c_double), ("filter", ctypes.c_int)]
 
 
 _lib = None
@@ -103,6 +111,12 @@ def lib():
             "cs_gcbc_go": (I, [P, PP, PP, P, P, P, P, P, Point, Point, D, P]),
             "cs_wall_condition_update": (I, [P, P, P, Point, P]),
             "cs_rk_stage": (I, [ctypes.POINTER(Dims), P, P, P, P, D, P]),
+            "cs_solver_create": (I, [ctypes.POINTER(SolverConfig), PP]),
+            "cs_solver_destroy": (I, [P]),
+            "cs_solver_rhs": (I, [P, P, P, P, P]),
+            "cs_solver_sweep": (I, [P, I, P, P, P, P]),
+            "cs_solver_time_step": (I, [P, P, DP, P]),
+            "cs_solver_rk4_step": (I, [P, P, P, D, DP, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -1,0 +1,647 @@
+// fused.hip -- the right-hand side as six fused line sweeps (uniform
+// Cartesian metrics), with the characteristic BCs and the RK stage update
+// folded into the sweeps.
+//
+// The reference evaluates one RHS as calc_viscous_shear_stress + calc_fluxes
+// (src/physics_pc.hpp:83-227) -- 27 line solves and 12 pointwise kernels, each
+// reading and writing whole fields -- followed by gcbc_go (src/gcbc.hpp:86-186).
+// Here each sweep reads its inputs once, differentiates several fields along
+// one axis and writes only what the next sweep needs:
+//   S1x, S1y: q -> (u, v, w, c^2) -> d/dx, d/dy                      (4 fields out)
+//   S1z:      q, x/y gradients -> d/dz -> stress (6) + heat flux (3)
+//   S2x:      q, stress row x, heat x -> F^x (5) -> dF/dx            (partial RHS)
+//   S2y, S2z: ... + previous partial RHS; S2z adds the RK stage update
+// and every S2 sweep applies the non-reflective characteristic update of its
+// two faces from the boundary values of its own solve (the reference's drva).
+//
+// Line solve: chunked LU recurrences as in line_solver.hip, chunks balanced
+// (lengths differ by at most one, every chunk >= 4 rows).  Thread layouts:
+//   x lines: lanes of a wave are the chunks of 64/GS lines (GS = pow2 >= P);
+//            carries by segmented Kogge-Stone scans with wave shuffles;
+//   y/z lines: 16 consecutive lines x P chunks per workgroup (128-byte
+//            coalesced rows), carries through LDS.
+#include "fused.hpp"
+
+namespace cs {
+namespace fused {
+
+// ---------------------------------------------------------------------------
+// sweep functors
+// ---------------------------------------------------------------------------
+struct Phys {
+    cs_point umf;
+    double mu, J;
+    double m[3];   // metric scale per axis: xi_x, eta_y, zeta_z
+};
+
+// S1: velocity and c^2 gradients along AX
+template <int AX>
+struct S1 {
+    static constexpr int NF = 4;
+    static constexpr bool kGcbc = false;   // gradients only; BC updates act on the flux sweeps
+    const double* q;
+    long lmx;
+    double* gout;          // AX < 2: [4][lmx] index-space derivatives
+    const double* gx;      // AX == 2: x and y derivatives from S1x, S1y
+    const double* gy;
+    double* tau;           // [6][lmx]  xx, yy, zz, xy, yz, zx
+    double* hf;            // [3][lmx]
+    Phys ph;
+
+    __device__ __forceinline__ void point(long idx, double* v) const {
+        double qq[5];
+        load_q(q, idx, lmx, qq);
+        Prim p = primitive(qq);
+        v[0] = p.u;
+        v[1] = p.v;
+        v[2] = p.w;
+        v[3] = p.c2;
+    }
+    __device__ __forceinline__ void store(long idx, const double* d, const double* pv) const {
+        if (AX < 2) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) gout[(long)f * lmx + idx] = d[f];
+        } else {
+            double a[4], b[4];
+#pragma unroll
+            for (int f = 0; f < 4; f++) { a[f] = gx[(long)f * lmx + idx]; b[f] = gy[(long)f * lmx + idx]; }
+            // chain rule of calc_viscous_shear_stress_post_compute_kernel with
+            // diagonal metrics (src/cuda/kernels/physics.hpp:267-281)
+            double g[9];
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                g[3 * f + 0] = ph.m[0] * a[f];
+                g[3 * f + 1] = ph.m[1] * b[f];
+                g[3 * f + 2] = ph.m[2] * d[f];
+            }
+            double c2g[3] = {ph.m[0] * a[3], ph.m[1] * b[3], ph.m[2] * d[3]};
+            double t6[6], h3[3], d4;
+            viscous_closure(ph.mu, ph.J, g, c2g, pv, t6, h3, &d4);
+#pragma unroll
+            for (int k = 0; k < 6; k++) tau[(long)k * lmx + idx] = t6[k];
+#pragma unroll
+            for (int k = 0; k < 3; k++) hf[(long)k * lmx + idx] = h3[k];
+        }
+    }
+};
+
+// S2: flux divergence along AX.  MODE 0: write the partial RHS; 1: add to it;
+// 2: add and write the RHS; 3: add and write q0 - dtk*J*RHS (RK stage).
+template <int AX, int MODE, bool VISC>
+struct S2 {
+    static constexpr int NF = 5;
+    static constexpr bool kGcbc = true;
+    const double* q;
+    long lmx;
+    const double* tau;
+    const double* hf;
+    const double* din;
+    double* dout;
+    const double* q0;
+    const double* dtp;
+    int div;
+    Phys ph;
+
+    // the AX row of calc_fluxes_pre_compute_kernel with metric row m*e_AX
+    // (src/cuda/kernels/physics.hpp:98-191); zero metric terms dropped
+    __device__ __forceinline__ void point(long idx, double* F) const {
+        double qq[5];
+        load_q(q, idx, lmx, qq);
+        Prim p = primitive(qq);
+        const double m = ph.m[AX];
+        const double vel = (AX == 0) ? p.u : (AX == 1 ? p.v : p.w);
+        const double uf = (AX == 0) ? ph.umf.x : (AX == 1 ? ph.umf.y : ph.umf.z);
+        const double ss = m * (vel + uf);
+        F[0] = qq[0] * ss;
+        F[1] = qq[1] * ss;
+        F[2] = qq[2] * ss;
+        F[3] = qq[3] * ss;
+        F[1 + AX] += m * p.p;
+        F[4] = (qq[4] + p.p) * ss - p.p * (uf * m);
+        if (VISC) {
+            constexpr int t0 = AX == 0 ? 0 : (AX == 1 ? 3 : 5);   // row AX of the stress tensor
+            constexpr int t1 = AX == 0 ? 3 : (AX == 1 ? 1 : 4);
+            constexpr int t2 = AX == 0 ? 5 : (AX == 1 ? 4 : 2);
+            F[1] -= m * tau[(long)t0 * lmx + idx];
+            F[2] -= m * tau[(long)t1 * lmx + idx];
+            F[3] -= m * tau[(long)t2 * lmx + idx];
+            F[4] -= m * hf[(long)AX * lmx + idx];
+        }
+    }
+    __device__ __forceinline__ void store(long idx, const double* d, const double*) const {
+        if (MODE == 0) {
+#pragma unroll
+            for (int f = 0; f < 5; f++) dout[(long)f * lmx + idx] = d[f];
+        } else if (MODE == 1 || MODE == 2) {
+#pragma unroll
+            for (int f = 0; f < 5; f++) dout[(long)f * lmx + idx] = din[(long)f * lmx + idx] + d[f];
+        } else {
+            const double dtk = dtp[0] / (double)div;
+            const double fj = dtk * ph.J;
+#pragma unroll
+            for (int f = 0; f < 5; f++) {
+                const double de = din[(long)f * lmx + idx] + d[f];
+                dout[(long)f * lmx + idx] = q0[(long)f * lmx + idx] - fj * de;
+            }
+        }
+    }
+    // non-reflective characteristic correction at a face point from the
+    // boundary derivatives of this sweep (gcbc_setup_kernel +
+    // gcbc_update_non_reflective_kernel, src/cuda/kernels/gcbc.hpp:251-454)
+    __device__ __forceinline__ void gcbc(long idx, int face, const double* drv, double* corr) const {
+        double qq[5];
+        load_q(q, idx, lmx, qq);
+        Prim p = primitive(qq);
+        const double cm[3] = {AX == 0 ? 1.0 : 0.0, AX == 1 ? 1.0 : 0.0, AX == 2 ? 1.0 : 0.0};
+        Eig e;
+        e.eleme(cm, qq[0], qq + 1, p.p, ph.umf);
+        double cha[5], ch[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) cha[v] = ph.J * drv[v];
+        e.l_apply(cm, cha, ch);
+        const double ra0 = 1 - 2 * face;
+        double dl[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (ra0 * (e.vn + e.vs + e.ao) > 0) dl[3] = -ch[3];
+        if (ra0 * (e.vn + e.vs - e.ao) > 0) dl[4] = -ch[4];
+        e.r_apply(cm, dl, corr);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// generic sweep kernel
+// ---------------------------------------------------------------------------
+template <int K, int AX, class SW, bool GCBC>
+__global__ void __launch_bounds__(AX == 0 ? 256 : 512) sweep_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
+    constexpr int NF = SW::NF;
+    constexpr int W = K + 4;
+    extern __shared__ double lds[];
+
+    const int N = ax.N, P = ax.P;
+    int c, lloc = 0;
+    long line;
+    if (AX == 0) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int lpw = 64 / ax.GS;
+        c = lane % ax.GS;
+        line = (long)blockIdx.x * (4 * lpw) + wave * lpw + lane / ax.GS;
+    } else {
+        lloc = threadIdx.x % kYLines;
+        c = threadIdx.x / kYLines;
+        line = (long)blockIdx.x * kYLines + lloc;
+    }
+    const bool act = line < ax.nlines;
+    const bool cv = c < P;
+    const bool work = act && cv;
+    const int s = cv ? (int)(((long)c * N) / P) : N;
+    const int L = cv ? (int)(((long)(c + 1) * N) / P) - s : 0;
+    const long base = line_base(AX, act ? line : 0, cm.lxi, cm.plane);
+    const long st = ax.st;
+    const double* tab = ax.tab;
+
+    // ---- rows s-2 .. s+L+1 through the pointwise functor, right-hand sides ----
+    double r[NF][K];
+    double pv[NF][K];      // field values at the chunk rows (for the epilogue)
+    {
+        double w[NF][W];
+#pragma unroll
+        for (int u = 0; u < W; u++) {
+            const int row = s - 2 + u;
+            double v[NF];
+            if (work && u < L + 4 && row >= 0 && row < N) {
+                sw.point(base + (long)row * st, v);
+            } else {
+#pragma unroll
+                for (int f = 0; f < NF; f++) v[f] = 0.0;
+            }
+#pragma unroll
+            for (int f = 0; f < NF; f++) w[f][u] = v[f];
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+            for (int t = 0; t < K; t++) {
+                r[f][t] = kAa * (w[f][t + 3] - w[f][t + 1]) + kBb * (w[f][t + 4] - w[f][t]);
+                pv[f][t] = w[f][t + 2];
+            }
+        if (work && c == 0 && !ax.seam) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                r[f][0] = kAb00 * w[f][2] + kAb01 * w[f][3] + kAb02 * w[f][4];
+                r[f][1] = kAb10 * (w[f][4] - w[f][2]);
+            }
+        }
+        if (work && c == P - 1 && !ax.seam) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                double fN1 = 0.0, fN2 = 0.0, fN3 = 0.0;
+#pragma unroll
+                for (int u = 0; u < W; u++) {
+                    if (u == L + 1) fN1 = w[f][u];
+                    if (u == L) fN2 = w[f][u];
+                    if (u == L - 1) fN3 = w[f][u];
+                }
+                const double r1 = -(kAb00 * fN1 + kAb01 * fN2 + kAb02 * fN3);
+                const double r2 = -kAb10 * (fN3 - fN1);
+#pragma unroll
+                for (int t = 0; t < K; t++) {
+                    if (t == L - 1) r[f][t] = r1;
+                    if (t == L - 2) r[f][t] = r2;
+                }
+            }
+        }
+    }
+    // quasi-disjoint seam rows, wrapped onto the opposite end of the line
+    // (build_system seam branch, numerics_device_pc.hpp:199-282, with recv
+    // formed as fill_gpu_buffer_kernel does, kernels/numerics_pc.hpp:86-121)
+    if (work && ax.seam && (c == 0 || c == P - 1)) {
+        const double* pbco = cm.pb;
+        const double* pbci = cm.pb + 16;
+#pragma unroll
+        for (int end = 0; end < 2; end++) {
+            if ((end == 0 && c != 0) || (end == 1 && c != P - 1)) continue;
+            double s0[NF], s1[NF], o0[NF], o1[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) { s0[f] = 0.0; s1[f] = 0.0; o0[f] = 0.0; o1[f] = 0.0; }
+            for (int i = 0; i < kLmd; i++) {
+                double v[NF];
+                sw.point(base + (long)(end == 0 ? i : N - 1 - i) * st, v);
+#pragma unroll
+                for (int f = 0; f < NF; f++) { s0[f] += pbci[i] * v[f]; s1[f] += pbci[kLmd + i] * v[f]; }
+            }
+            for (int i = 0; i < kLmd; i++) {
+                double v[NF];
+                sw.point(base + (long)(end == 0 ? N - 1 - i : i) * st, v);
+#pragma unroll
+                for (int f = 0; f < NF; f++) { o0[f] += pbco[i] * v[f]; o1[f] += pbco[kLmd + i] * v[f]; }
+            }
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                if (end == 0) {
+                    r[f][0] = s0[f] + o0[f];
+                    r[f][1] = s1[f] + o1[f];
+                } else {
+                    const double r1 = -(s0[f] + o0[f]);
+                    const double r2 = -(s1[f] + o1[f]);
+#pragma unroll
+                    for (int t = 0; t < K; t++) {
+                        if (t == L - 1) r[f][t] = r1;
+                        if (t == L - 2) r[f][t] = r2;
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- forward recurrence, carries, backward recurrence, carries ----
+    double carry[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) carry[f] = 0.0;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        if (t < L) {
+            const double nu = tab[s + t], mu = tab[N + s + t];
+#pragma unroll
+            for (int f = 0; f < NF; f++) { carry[f] = nu * r[f][t] + mu * carry[f]; r[f][t] = carry[f]; }
+        }
+    }
+    double G[NF];
+    if (AX == 0) {
+        const int GS = ax.GS;
+        double A = cv ? tab[3 * N + s + L - 1] : 1.0;
+        double B[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) B[f] = cv ? carry[f] : 0.0;
+        for (int d = 1; d < GS; d <<= 1) {
+            const double Au = __shfl_up(A, d, GS);
+            double Bu[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) Bu[f] = __shfl_up(B[f], d, GS);
+            if (c >= d) {
+#pragma unroll
+                for (int f = 0; f < NF; f++) B[f] += A * Bu[f];
+                A *= Au;
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const double g = __shfl_up(B[f], 1, GS);
+            G[f] = (c == 0) ? 0.0 : g;
+        }
+    } else {
+        double* gend = lds;
+        if (cv) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) gend[(f * P + c) * kYLines + lloc] = carry[f];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < NF; f++) G[f] = 0.0;
+        for (int cc = 0; cc < c && cc < P; cc++) {
+            const int e = (int)(((long)(cc + 1) * N) / P) - 1;
+            const double M = tab[3 * N + e];
+#pragma unroll
+            for (int f = 0; f < NF; f++) G[f] = gend[(f * P + cc) * kYLines + lloc] + M * G[f];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        if (t < L) {
+            const double M = tab[3 * N + s + t];
+#pragma unroll
+            for (int f = 0; f < NF; f++) r[f][t] += M * G[f];
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; f++) carry[f] = 0.0;
+#pragma unroll
+    for (int t = K - 1; t >= 0; t--) {
+        if (t < L) {
+            const double rh = tab[2 * N + s + t];
+#pragma unroll
+            for (int f = 0; f < NF; f++) { carry[f] = r[f][t] + rh * carry[f]; r[f][t] = carry[f]; }
+        }
+    }
+    double X[NF];
+    if (AX == 0) {
+        const int GS = ax.GS;
+        double A = cv ? tab[4 * N + s] : 0.0;
+        double B[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) B[f] = cv ? carry[f] : 0.0;
+        for (int d = 1; d < GS; d <<= 1) {
+            const double Ad = __shfl_down(A, d, GS);
+            double Bd[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) Bd[f] = __shfl_down(B[f], d, GS);
+            if (c + d < GS) {
+#pragma unroll
+                for (int f = 0; f < NF; f++) B[f] += A * Bd[f];
+                A *= Ad;
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const double x = __shfl_down(B[f], 1, GS);
+            X[f] = (c + 1 < P) ? x : 0.0;
+        }
+    } else {
+        double* xbeg = lds + NF * P * kYLines;
+        if (cv) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) xbeg[(f * P + c) * kYLines + lloc] = carry[f];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < NF; f++) X[f] = 0.0;
+        for (int cc = P - 1; cc > c; cc--) {
+            const int sc = (int)(((long)cc * N) / P);
+            const double R = tab[4 * N + sc];
+#pragma unroll
+            for (int f = 0; f < NF; f++) X[f] = xbeg[(f * P + cc) * kYLines + lloc] + R * X[f];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        if (t < L) {
+            const double R = tab[4 * N + s + t];
+#pragma unroll
+            for (int f = 0; f < NF; f++) r[f][t] += R * X[f];
+        }
+    }
+
+    if (!work) return;
+
+    // ---- characteristic BCs of the two faces of this axis ----
+    double corr0[NF], corr1[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) { corr0[f] = 0.0; corr1[f] = 0.0; }
+    if constexpr (GCBC) {
+        if (c == 0) {
+            double drv[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) drv[f] = r[f][0];
+            sw.gcbc(base, 0, drv, corr0);
+        }
+        if (c == P - 1) {
+            double drv[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                drv[f] = 0.0;
+#pragma unroll
+                for (int t = 0; t < K; t++)
+                    if (t == L - 1) drv[f] = r[f][t];
+            }
+            sw.gcbc(base + (long)(N - 1) * st, 1, drv, corr1);
+        }
+    }
+
+    // ---- epilogue ----
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        if (t < L) {
+            const int row = s + t;
+            const long idx = base + (long)row * st;
+            double d[NF], p[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) { d[f] = r[f][t]; p[f] = pv[f][t]; }
+            if constexpr (GCBC) {
+                if (row < kMbci || row >= N - kMbci) {
+                    const double wgt = cm.sbcc[idx];
+#pragma unroll
+                    for (int f = 0; f < NF; f++) d[f] += wgt * (row < kMbci ? corr0[f] : corr1[f]);
+                }
+            }
+            sw.store(idx, d, p);
+        }
+    }
+}
+
+template <int K, int AX, class SW, bool GCBC>
+static int launch_sweep(const AxisDesc& ax, const SweepCommon& cm, const SW& sw, hipStream_t s) {
+    constexpr int NF = SW::NF;
+    dim3 block, grid;
+    size_t lds = 0;
+    if (AX == 0) {
+        const int lpb = 4 * (64 / ax.GS);
+        block = dim3(256);
+        grid = dim3((unsigned)((ax.nlines + lpb - 1) / lpb));
+    } else {
+        block = dim3(kYLines * ax.P);
+        grid = dim3((unsigned)((ax.nlines + kYLines - 1) / kYLines));
+        lds = sizeof(double) * 2 * NF * ax.P * kYLines;
+    }
+    hipLaunchKernelGGL((sweep_kernel<K, AX, SW, GCBC>), grid, block, lds, s, ax, cm, sw);
+    CS_HIP(hipGetLastError());
+    return CS_OK;
+}
+
+template <int AX, class SW>
+static int dispatch(const AxisDesc& ax, const SweepCommon& cm, const SW& sw, hipStream_t s) {
+    if constexpr (SW::kGcbc) {
+        if (ax.K == 8) {
+            if (ax.gcbc) return launch_sweep<8, AX, SW, true>(ax, cm, sw, s);
+            return launch_sweep<8, AX, SW, false>(ax, cm, sw, s);
+        }
+        if (ax.gcbc) return launch_sweep<16, AX, SW, true>(ax, cm, sw, s);
+        return launch_sweep<16, AX, SW, false>(ax, cm, sw, s);
+    } else {
+        if (ax.K == 8) return launch_sweep<8, AX, SW, false>(ax, cm, sw, s);
+        return launch_sweep<16, AX, SW, false>(ax, cm, sw, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// time step with uniform Cartesian metrics (calc_time_step_kernel,
+// src/cuda/kernels/physics.hpp:335-427, D9 corrected), primitives inline
+// ---------------------------------------------------------------------------
+constexpr int kTB = 256;
+
+__global__ void __launch_bounds__(kTB) dt_partial_kernel(const double* q, long lmx, Phys ph, double* part) {
+    __shared__ double s0[kTB], s1[kTB];
+    const double rr1 = ph.m[0] * ph.m[0] + ph.m[1] * ph.m[1] + ph.m[2] * ph.m[2];
+    const double ssi = fabs(ph.J);
+    double m0 = 0.0, m1 = 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < lmx; t += (long)gridDim.x * blockDim.x) {
+        double qq[5];
+        load_q(q, t, lmx, qq);
+        Prim p = primitive(qq);
+        const double rr2 = fabs(ph.m[0] * (p.u + ph.umf.x)) + fabs(ph.m[1] * (p.v + ph.umf.y)) +
+                           fabs(ph.m[2] * (p.w + ph.umf.z));
+        m0 = fmax(m0, (sqrt(p.c2 * rr1) + rr2) * ssi);
+        m1 = fmax(m1, p.rhoi * ph.mu * rr1 * ssi * ssi);
+    }
+    s0[threadIdx.x] = m0;
+    s1[threadIdx.x] = m1;
+    __syncthreads();
+    for (int w = kTB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s0[threadIdx.x] = fmax(s0[threadIdx.x], s0[threadIdx.x + w]);
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = s0[0]; part[2 * blockIdx.x + 1] = s1[0]; }
+}
+
+// out[0] = dt = min(cfl/max0, 0.5/max1) (or dt_fixed when > 0), out[1..2] = maxima
+__global__ void __launch_bounds__(kTB) dt_final_kernel(const double* part, int n, double cfl, double dt_fixed,
+                                                       double* out) {
+    __shared__ double s0[kTB], s1[kTB];
+    double m0 = 0.0, m1 = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) { m0 = fmax(m0, part[2 * i]); m1 = fmax(m1, part[2 * i + 1]); }
+    s0[threadIdx.x] = m0;
+    s1[threadIdx.x] = m1;
+    __syncthreads();
+    for (int w = kTB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s0[threadIdx.x] = fmax(s0[threadIdx.x], s0[threadIdx.x + w]);
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[1] = s0[0];
+        out[2] = s1[0];
+        out[0] = dt_fixed > 0.0 ? dt_fixed : fmin(cfl / s0[0], 0.5 / s1[0]);
+    }
+}
+
+__global__ void set_dt_kernel(double dt, double* out) {
+    if (threadIdx.x == 0) { out[0] = dt; out[1] = 0.0; out[2] = 0.0; }
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+static Phys phys_of(const cs_solver* sv) {
+    Phys ph;
+    ph.umf = sv->cfg.umf;
+    ph.mu = sv->cfg.mu;
+    ph.J = sv->cfg.yaco;
+    for (int a = 0; a < 3; a++) ph.m[a] = sv->cfg.metric[a];
+    return ph;
+}
+
+int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, const double* q0, int div,
+               hipStream_t s) {
+    const Phys ph = phys_of(sv);
+    const long lmx = sv->g.lmx;
+    SweepCommon cm;
+    cm.lxi = sv->g.lxi;
+    cm.plane = (long)sv->g.lxi * sv->g.let;
+    cm.pb = sv->num->pb;
+    cm.sbcc = sbcc;
+    int rc;
+    const bool visc = sv->cfg.viscous != 0;
+    if (visc) {
+        S1<0> a{q, lmx, sv->gx, nullptr, nullptr, nullptr, nullptr, ph};
+        if ((rc = dispatch<0>(sv->ax[0], cm, a, s))) return rc;
+        S1<1> b{q, lmx, sv->gy, nullptr, nullptr, nullptr, nullptr, ph};
+        if ((rc = dispatch<1>(sv->ax[1], cm, b, s))) return rc;
+        S1<2> c{q, lmx, nullptr, sv->gx, sv->gy, sv->tau, sv->hf, ph};
+        if ((rc = dispatch<2>(sv->ax[2], cm, c, s))) return rc;
+    }
+    auto run = [&](auto vtag) -> int {
+        constexpr bool V = decltype(vtag)::value;
+        S2<0, 0, V> x{q, lmx, sv->tau, sv->hf, nullptr, sv->dpart, nullptr, nullptr, 0, ph};
+        int r2 = dispatch<0>(sv->ax[0], cm, x, s);
+        if (r2) return r2;
+        S2<1, 1, V> y{q, lmx, sv->tau, sv->hf, sv->dpart, sv->dpart, nullptr, nullptr, 0, ph};
+        if ((r2 = dispatch<1>(sv->ax[1], cm, y, s))) return r2;
+        if (q0) {
+            S2<2, 3, V> z{q, lmx, sv->tau, sv->hf, sv->dpart, out, q0, sv->dt, div, ph};
+            return dispatch<2>(sv->ax[2], cm, z, s);
+        }
+        S2<2, 2, V> z{q, lmx, sv->tau, sv->hf, sv->dpart, out, nullptr, nullptr, 0, ph};
+        return dispatch<2>(sv->ax[2], cm, z, s);
+    };
+    if (visc) return run(std::true_type{});
+    return run(std::false_type{});
+}
+
+// One sweep of the right-hand side in RHS mode on the solver's scratch buffers
+// (0..5 = S1x, S1y, S1z, S2x, S2y, S2z); for timing single kernels.
+int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, double* out, hipStream_t s) {
+    const Phys ph = phys_of(sv);
+    const long lmx = sv->g.lmx;
+    SweepCommon cm;
+    cm.lxi = sv->g.lxi;
+    cm.plane = (long)sv->g.lxi * sv->g.let;
+    cm.pb = sv->num->pb;
+    cm.sbcc = sbcc;
+    const bool visc = sv->cfg.viscous != 0;
+    if (which < 3 && !visc) return fail(CS_ERR_STATE, "gradient sweeps exist only for the viscous RHS");
+    switch (which) {
+        case 0: { S1<0> a{q, lmx, sv->gx, nullptr, nullptr, nullptr, nullptr, ph}; return dispatch<0>(sv->ax[0], cm, a, s); }
+        case 1: { S1<1> a{q, lmx, sv->gy, nullptr, nullptr, nullptr, nullptr, ph}; return dispatch<1>(sv->ax[1], cm, a, s); }
+        case 2: { S1<2> a{q, lmx, nullptr, sv->gx, sv->gy, sv->tau, sv->hf, ph}; return dispatch<2>(sv->ax[2], cm, a, s); }
+        default: break;
+    }
+    auto run = [&](auto vtag) -> int {
+        constexpr bool V = decltype(vtag)::value;
+        if (which == 3) { S2<0, 0, V> a{q, lmx, sv->tau, sv->hf, nullptr, sv->dpart, nullptr, nullptr, 0, ph}; return dispatch<0>(sv->ax[0], cm, a, s); }
+        if (which == 4) { S2<1, 1, V> a{q, lmx, sv->tau, sv->hf, sv->dpart, sv->dpart, nullptr, nullptr, 0, ph}; return dispatch<1>(sv->ax[1], cm, a, s); }
+        S2<2, 2, V> a{q, lmx, sv->tau, sv->hf, sv->dpart, out, nullptr, nullptr, 0, ph};
+        return dispatch<2>(sv->ax[2], cm, a, s);
+    };
+    if (which > 5 || which < 0) return fail(CS_ERR_ARG, "sweep index out of 0..5");
+    if (visc) return run(std::true_type{});
+    return run(std::false_type{});
+}
+
+int time_step(cs_solver* sv, const double* q, double dt_fixed, hipStream_t s) {
+    const long lmx = sv->g.lmx;
+    if (dt_fixed > 0.0) {
+        hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(64), 0, s, dt_fixed, sv->dt);
+        CS_HIP(hipGetLastError());
+        return CS_OK;
+    }
+    hipLaunchKernelGGL(dt_partial_kernel, dim3(sv->red_blocks), dim3(kTB), 0, s, q, lmx, phys_of(sv), sv->red);
+    CS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(kTB), 0, s, sv->red, sv->red_blocks, sv->cfg.cfl, 0.0, sv->dt);
+    CS_HIP(hipGetLastError());
+    return CS_OK;
+}
+
+}  // namespace fused
+}  // namespace cs

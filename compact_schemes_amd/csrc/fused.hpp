@@ -1,0 +1,61 @@
+// fused.hpp -- state of the fused right-hand-side path (cs_solver).
+#pragma once
+
+#include <type_traits>
+
+#include "physics_kernels.hpp"
+
+namespace cs {
+namespace fused {
+
+constexpr int kYLines = 16;   // lines per workgroup of a y/z sweep
+constexpr int kMaxP = 32;     // chunks per line
+
+// one axis: derivative operator with balanced chunks
+struct AxisDesc {
+    const double* tab;   // [5][N]: nu, mu, rho, Mf, Rb
+    int N, P, K, GS;     // GS: lanes per line in the x layout (pow2 >= P)
+    long nlines;
+    long st;
+    int seam;            // both faces seam (periodic wrap on one subdomain)
+    int gcbc;            // both faces non-reflective characteristic BCs
+};
+
+struct SweepCommon {
+    int lxi;
+    long plane;
+    const double* pb;    // pbco[16] | pbci[16]
+    const double* sbcc;
+};
+
+}  // namespace fused
+}  // namespace cs
+
+struct cs_solver {
+    cs_solver_config cfg;
+    cs::Geo g;
+    cs_numerics* num = nullptr;         // interface table, filter operators
+    cs::fused::AxisDesc ax[3];
+    double* tabs[3] = {nullptr, nullptr, nullptr};
+    double* gx = nullptr;      // [4][lmx]
+    double* gy = nullptr;      // [4][lmx]
+    double* tau = nullptr;     // [6][lmx]
+    double* hf = nullptr;      // [3][lmx]
+    double* dpart = nullptr;   // [5][lmx]
+    double* qb = nullptr;      // [5][lmx] stage buffers
+    double* qc = nullptr;
+    double* red = nullptr;     // time-step partial maxima
+    int red_blocks = 0;
+    double* dt = nullptr;      // [0] dt, [1..2] maxima
+    double* dt_host = nullptr; // pinned
+};
+
+namespace cs {
+namespace fused {
+int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, const double* q0, int div,
+               hipStream_t s);
+int time_step(cs_solver* sv, const double* q, double dt_fixed, hipStream_t s);
+int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, double* out, hipStream_t s);
+int build_axis_table(int n, int seam, int K, int P, std::vector<double>& h);
+}  // namespace fused
+}  // namespace cs

@@ -12,52 +12,6 @@ namespace cs {
 
 constexpr int kGBlock = 256;
 
-struct Eig {
-    double ao, aoi, hv2, vn, vs, ve[3];
-    // gcbc_vgpr::eleme, src/cuda/kernels/gcbc.hpp:44-61
-    __device__ void eleme(const double* cm, double qa1, const double* qa24, double pp, cs_point umf) {
-        double rhoi = 1.0 / qa1;
-        ao = sqrt(kGam * rhoi * pp);
-        aoi = 1.0 / ao;
-        ve[0] = rhoi * qa24[0];
-        ve[1] = rhoi * qa24[1];
-        ve[2] = rhoi * qa24[2];
-        hv2 = 0.5 * (ve[0] * ve[0] + ve[1] * ve[1] + ve[2] * ve[2]);
-        vn = cm[0] * ve[0] + cm[1] * ve[1] + cm[2] * ve[2];
-        vs = cm[0] * umf.x + cm[1] * umf.y + cm[2] * umf.z;
-    }
-    // characteristic -> conservative, gcbc_vgpr::xtr2q, src/cuda/kernels/gcbc.hpp:63-105
-    __device__ void r_apply(const double* cm, const double* x, double* y) const {
-        const double bo = hv2 + kHamm1 * ao * ao, co = ao * vn;
-        const double dm0 = ao * cm[0], dm1 = ao * cm[1], dm2 = ao * cm[2];
-        const double a13 = 0.5 * (ve[0] + dm0), a23 = 0.5 * (ve[1] + dm1), a33 = 0.5 * (ve[2] + dm2);
-        const double a43 = 0.5 * (bo + co);
-        y[0] = cm[0] * x[0] + cm[1] * x[1] + cm[2] * x[2] + 0.5 * x[3] + 0.5 * x[4];
-        y[1] = cm[0] * ve[0] * x[0] + (cm[1] * ve[0] - dm2) * x[1] + (cm[2] * ve[0] + dm1) * x[2] + a13 * x[3] +
-               (a13 - dm0) * x[4];
-        y[2] = (cm[0] * ve[1] + dm2) * x[0] + cm[1] * ve[1] * x[1] + (cm[2] * ve[1] - dm0) * x[2] + a23 * x[3] +
-               (a23 - dm1) * x[4];
-        y[3] = (cm[0] * ve[2] - dm1) * x[0] + (cm[1] * ve[2] + dm0) * x[1] + cm[2] * ve[2] * x[2] + a33 * x[3] +
-               (a33 - dm2) * x[4];
-        y[4] = (hv2 * cm[0] + dm2 * ve[1] - dm1 * ve[2]) * x[0] + (hv2 * cm[1] + dm0 * ve[2] - dm2 * ve[0]) * x[1] +
-               (hv2 * cm[2] + dm1 * ve[0] - dm0 * ve[1]) * x[2] + a43 * x[3] + (a43 - co) * x[4];
-    }
-    // conservative -> characteristic, gcbc_vgpr::xtq2r, src/cuda/kernels/gcbc.hpp:107-150 (D10 corrected)
-    __device__ void l_apply(const double* cm, const double* x, double* y) const {
-        const double ho = kGamm1 * aoi * aoi, bo = 1.0 - ho * hv2, co = aoi * vn;
-        const double dm0 = aoi * cm[0], dm1 = aoi * cm[1], dm2 = aoi * cm[2];
-        const double rv0 = ho * ve[0], rv1 = ho * ve[1], rv2 = ho * ve[2];
-        y[0] = (bo * cm[0] + dm1 * ve[2] - dm2 * ve[1]) * x[0] + cm[0] * rv0 * x[1] + (cm[0] * rv1 + dm2) * x[2] +
-               (cm[0] * rv2 - dm1) * x[3] - ho * cm[0] * x[4];
-        y[1] = (bo * cm[1] + dm2 * ve[0] - dm0 * ve[2]) * x[0] + (cm[1] * rv0 - dm2) * x[1] + cm[1] * rv1 * x[2] +
-               (cm[1] * rv2 + dm0) * x[3] - ho * cm[1] * x[4];
-        y[2] = (bo * cm[2] + dm0 * ve[1] - dm1 * ve[0]) * x[0] + (cm[2] * rv0 + dm1) * x[1] +
-               (cm[2] * rv1 - dm0) * x[2] + cm[2] * rv2 * x[3] - ho * cm[2] * x[4];
-        y[3] = (1.0 - bo - co) * x[0] + (dm0 - rv0) * x[1] + (dm1 - rv1) * x[2] + (dm2 - rv2) * x[3] + ho * x[4];
-        y[4] = (1.0 - bo + co) * x[0] + (-dm0 - rv0) * x[1] + (-dm1 - rv1) * x[2] + (-dm2 - rv2) * x[3] + ho * x[4];
-    }
-};
-
 struct FaceGeo {
     long fs, lmx, plane, st;
     int lxi, axis, face, N;

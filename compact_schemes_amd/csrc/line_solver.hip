@@ -186,8 +186,13 @@ __global__ void __launch_bounds__(max_threads(K)) line_kernel(LineArgs a) {
     if (AX == 0) {
         const int nl = (int)min((long)TL, (long)a.nlines - line0);
         const double* src = a.in + line0 * N;
-        for (int l = 0; l < nl; l++)
-            for (int i = tid; i < N; i += blockDim.x) tile[i * TP + l] = src[(long)l * N + i];
+        for (int i = tid; i < N; i += blockDim.x) {
+            double v[TL];
+#pragma unroll
+            for (int l = 0; l < TL; l++) v[l] = (l < nl) ? src[(long)l * N + i] : 0.0;
+#pragma unroll
+            for (int l = 0; l < TL; l++) tile[i * TP + l] = v[l];
+        }
         __syncthreads();
     }
 
@@ -377,8 +382,11 @@ __global__ void __launch_bounds__(max_threads(K)) line_kernel(LineArgs a) {
         __syncthreads();
         const int nl = (int)min((long)TL, (long)a.nlines - line0);
         double* dst = a.out + line0 * N;
-        for (int l = 0; l < nl; l++)
-            for (int i = tid; i < N; i += blockDim.x) dst[(long)l * N + i] = tile[i * TP + l];
+        for (int i = tid; i < N; i += blockDim.x) {
+#pragma unroll
+            for (int l = 0; l < TL; l++)
+                if (l < nl) dst[(long)l * N + i] = tile[i * TP + l];
+        }
     } else if (active && cvalid) {
 #pragma unroll
         for (int t = 0; t < K; t++)

@@ -1,0 +1,197 @@
+// solver.hip -- cs_solver: the fused right-hand side and the RK4 step.
+//
+// The RK4 step is the CANARD low-storage scheme the reference driver loop
+// sketches (driver/canard.cu:132-205, weights dtk = dt/(nkrk-nk+1) for
+// 1-based nk): q_k = q0 - dtk*yaco*de(q_{k-1}).  Stage buffers ping-pong
+// between the caller's q and two scratch fields so that q0 is never copied:
+// A -> B, (B, A) -> C, (C, A) -> B, (B, A) -> A.
+#include "fused.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace cs {
+namespace fused {
+
+// Derivative operator of one axis with balanced chunks (the table layout of
+// the line solver; closure rows for 0, seam rows for 1).
+int build_axis_table(int n, int seam, int K, int P, std::vector<double>& h) {
+    (void)K;
+    h.assign(5 * (size_t)n, 0.0);
+    std::vector<double> a(n, kAlpha), b(n, 1.0), c(n, kAlpha);
+    if (!seam) { a[0] = 0; c[0] = kAlpha01; a[1] = kAlpha10; c[1] = kAlpha10; }
+    else       { a[0] = 0; c[0] = kAlpha; }
+    if (!seam) { a[n - 1] = kAlpha01; c[n - 1] = 0; a[n - 2] = kAlpha10; c[n - 2] = kAlpha10; }
+    else       { a[n - 1] = kAlpha; c[n - 1] = 0; }
+    double* nu = &h[0];
+    double* mu = &h[n];
+    double* rho = &h[2 * (size_t)n];
+    double* Mf = &h[3 * (size_t)n];
+    double* Rb = &h[4 * (size_t)n];
+    double cp_prev = 0.0;
+    for (int i = 0; i < n; i++) {
+        double w = 1.0 / (b[i] - (i > 0 ? a[i] * cp_prev : 0.0));
+        nu[i] = w;
+        mu[i] = (i > 0) ? -a[i] * w : 0.0;
+        double cp = c[i] * w;
+        rho[i] = -cp;
+        cp_prev = cp;
+    }
+    rho[n - 1] = 0.0;
+    for (int ch = 0; ch < P; ch++) {
+        int s = (int)(((long)ch * n) / P), e = (int)(((long)(ch + 1) * n) / P) - 1;
+        double p = 1.0;
+        for (int j = s; j <= e; j++) { p *= mu[j]; Mf[j] = p; }
+        p = 1.0;
+        for (int j = e; j >= s; j--) { p *= rho[j]; Rb[j] = p; }
+    }
+    return CS_OK;
+}
+
+}  // namespace fused
+}  // namespace cs
+
+using namespace cs;
+using namespace cs::fused;
+
+static int axis_mode(const int* nbc, int axis, int* seam, int* gcbc) {
+    int f0 = nbc[axis], f1 = nbc[3 + axis];
+    auto is_seam = [](int v) {
+        return v == CS_BC_INTER_STRAIGHT || v == CS_BC_INTER_SUBDOMAINS || v == CS_BC_PERIODIC;
+    };
+    if (is_seam(f0) && is_seam(f1)) { *seam = 1; *gcbc = 0; return CS_OK; }
+    if (f0 == CS_BC_NON_REFLECTIVE && f1 == CS_BC_NON_REFLECTIVE) { *seam = 0; *gcbc = 1; return CS_OK; }
+    return fail(CS_ERR_UNSUPPORTED, "fused path: each axis needs two non-reflective faces or two seam faces "
+                                    "(walls and curved interfaces use the per-operator API)");
+}
+
+extern "C" {
+
+int cs_solver_destroy(cs_solver* sv) {
+    if (!sv) return CS_OK;
+    double* bufs[] = {sv->gx, sv->gy, sv->tau, sv->hf, sv->dpart, sv->qb, sv->qc, sv->red, sv->dt,
+                      sv->tabs[0], sv->tabs[1], sv->tabs[2]};
+    for (double* p : bufs)
+        if (p) (void)hipFree(p);
+    if (sv->dt_host) (void)hipHostFree(sv->dt_host);
+    if (sv->num) cs_numerics_destroy(sv->num);
+    delete sv;
+    return CS_OK;
+}
+
+int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
+    if (!cfg || !out) return fail(CS_ERR_ARG, "null argument");
+    *out = nullptr;
+    Geo g;
+    int rc = check_dims(&cfg->dims, &g);
+    if (rc) return rc;
+    int seam[3], gc[3];
+    for (int a = 0; a < 3; a++) {
+        if ((rc = axis_mode(cfg->nbc, a, &seam[a], &gc[a]))) return rc;
+        if (g.n(a) < 16) return fail(CS_ERR_SHAPE, "fused path needs at least 16 points per axis");
+        if (g.n(a) > 512) return fail(CS_ERR_SHAPE, "fused path supports at most 512 points per axis");
+    }
+    cs_solver* sv = new cs_solver();
+    sv->cfg = *cfg;
+    sv->g = g;
+    if ((rc = cs_numerics_create(&cfg->dims, cfg->nbc, &sv->num)) || (rc = cs_deriv_setup(sv->num)) ||
+        (rc = cs_set_filter_alpha(sv->num, cfg->alpha_f))) {
+        std::string msg = cs_last_error();
+        cs_solver_destroy(sv);
+        return fail(rc, msg);
+    }
+    const long lmx = g.lmx;
+    for (int a = 0; a < 3; a++) {
+        AxisDesc& ax = sv->ax[a];
+        ax.N = g.n(a);
+        ax.K = ax.N <= 256 ? 8 : 16;
+        ax.P = (ax.N + ax.K - 1) / ax.K;
+        ax.GS = 1;
+        while (ax.GS < ax.P) ax.GS <<= 1;
+        ax.nlines = g.face_size(a);
+        ax.st = g.stride(a);
+        ax.seam = seam[a];
+        ax.gcbc = gc[a];
+        std::vector<double> h;
+        build_axis_table(ax.N, ax.seam, ax.K, ax.P, h);
+        if (hipMalloc(&sv->tabs[a], h.size() * sizeof(double)) != hipSuccess ||
+            hipMemcpy(sv->tabs[a], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            cs_solver_destroy(sv);
+            return fail(CS_ERR_HIP, "operator table upload failed");
+        }
+        ax.tab = sv->tabs[a];
+    }
+    sv->red_blocks = (int)std::min<long>(2048, (lmx + 255) / 256);
+    bool ok = hipMalloc(&sv->dpart, sizeof(double) * 5 * lmx) == hipSuccess &&
+              hipMalloc(&sv->qb, sizeof(double) * 5 * lmx) == hipSuccess &&
+              hipMalloc(&sv->qc, sizeof(double) * 5 * lmx) == hipSuccess &&
+              hipMalloc(&sv->red, sizeof(double) * 2 * sv->red_blocks) == hipSuccess &&
+              hipMalloc(&sv->dt, sizeof(double) * 4) == hipSuccess &&
+              hipHostMalloc(&sv->dt_host, sizeof(double) * 4) == hipSuccess;
+    if (ok && cfg->viscous)
+        ok = hipMalloc(&sv->gx, sizeof(double) * 4 * lmx) == hipSuccess &&
+             hipMalloc(&sv->gy, sizeof(double) * 4 * lmx) == hipSuccess &&
+             hipMalloc(&sv->tau, sizeof(double) * 6 * lmx) == hipSuccess &&
+             hipMalloc(&sv->hf, sizeof(double) * 3 * lmx) == hipSuccess;
+    if (!ok) {
+        cs_solver_destroy(sv);
+        return fail(CS_ERR_HIP, "allocation of solver buffers failed");
+    }
+    *out = sv;
+    return CS_OK;
+}
+
+int cs_solver_rhs(cs_solver* sv, const double* qa, double* de, const double* sbcc, cs_stream stream) {
+    if (!sv || !qa || !de) return fail(CS_ERR_ARG, "null argument");
+    if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
+        return fail(CS_ERR_ARG, "non-reflective faces need sbcc");
+    return rhs_sweeps(sv, qa, sbcc, de, nullptr, 0, (hipStream_t)stream);
+}
+
+int cs_solver_sweep(cs_solver* sv, int which, const double* qa, double* de, const double* sbcc, cs_stream stream) {
+    if (!sv || !qa || !de) return fail(CS_ERR_ARG, "null argument");
+    if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
+        return fail(CS_ERR_ARG, "non-reflective faces need sbcc");
+    return one_sweep(sv, which, qa, sbcc, de, (hipStream_t)stream);
+}
+
+int cs_solver_time_step(cs_solver* sv, const double* qa, double* dte, cs_stream stream) {
+    if (!sv || !qa) return fail(CS_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = time_step(sv, qa, 0.0, s);
+    if (rc || !dte) return rc;
+    CS_HIP(hipMemcpyAsync(sv->dt_host, sv->dt, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    CS_HIP(hipStreamSynchronize(s));
+    *dte = sv->dt_host[0];
+    return CS_OK;
+}
+
+int cs_solver_rk4_step(cs_solver* sv, double* qa, const double* sbcc, double dt, double* dt_out, cs_stream stream) {
+    if (!sv || !qa) return fail(CS_ERR_ARG, "null argument");
+    if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
+        return fail(CS_ERR_ARG, "non-reflective faces need sbcc");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = time_step(sv, qa, dt, s);
+    if (rc) return rc;
+    if ((rc = rhs_sweeps(sv, qa, sbcc, sv->qb, qa, 4, s))) return rc;
+    if ((rc = rhs_sweeps(sv, sv->qb, sbcc, sv->qc, qa, 3, s))) return rc;
+    if ((rc = rhs_sweeps(sv, sv->qc, sbcc, sv->qb, qa, 2, s))) return rc;
+    if ((rc = rhs_sweeps(sv, sv->qb, sbcc, qa, qa, 1, s))) return rc;
+    if (sv->cfg.filter) {
+        for (int v = 0; v < 5; v++) {
+            double* f = qa + (long)v * sv->g.lmx;
+            for (int a = 0; a < 3; a++) {
+                const int sp = sv->ax[a].seam;
+                if ((rc = line_apply(sv->num, OP_FILTER, a, f, f, sp, sp, nullptr, 0, false, s))) return rc;
+            }
+        }
+    }
+    if (dt_out) {
+        CS_HIP(hipMemcpyAsync(sv->dt_host, sv->dt, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+        CS_HIP(hipStreamSynchronize(s));
+        *dt_out = sv->dt_host[0];
+    }
+    return CS_OK;
+}
+
+}  // extern "C"

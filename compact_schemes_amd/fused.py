@@ -1,0 +1,121 @@
+"""The fused right-hand side and RK4 step of the HIP library (cs_solver).
+
+Six line sweeps per RHS on a uniform Cartesian grid with the characteristic
+BCs and the RK stage update inside the sweeps; see include/compact_schemes.h.
+"""
+import ctypes
+
+import numpy as np
+
+from . import Dims, Point, SolverConfig, _p, _stream, check, lib
+
+# algorithmic HBM bytes per grid point of each sweep (FP64, viscous RHS):
+# fields read + fields written once each
+SWEEP_NAMES = ["S1x grad", "S1y grad", "S1z grad+stress", "S2x flux", "S2y flux", "S2z flux"]
+SWEEP_BYTES_VISC = [8 * (5 + 4), 8 * (5 + 4), 8 * (5 + 8 + 9), 8 * (5 + 4 + 5), 8 * (5 + 4 + 5 + 5), 8 * (5 + 4 + 5 + 5)]
+SWEEP_BYTES_EULER = [0, 0, 0, 8 * (5 + 5), 8 * (5 + 5 + 5), 8 * (5 + 5 + 5)]
+
+
+def _uniform(a, name):
+    v = float(a.reshape(-1)[0])
+    if not np.all(a == v):
+        raise ValueError(f"fused path needs a uniform {name}")
+    return v
+
+
+class FusedSolver:
+    """cs_solver for one subdomain.  state: dict of numpy arrays as produced by
+    compact_schemes_amd.workloads.flow_state (metrics, yaco, ssk must be uniform)."""
+
+    def __init__(self, dims, nbc, state, viscous=True, cfl=0.95, alpha_f=0.49, filt=True):
+        import torch
+        self.dims = tuple(int(d) for d in dims)
+        lmx = self.dims[0] * self.dims[1] * self.dims[2]
+        self.lmx = lmx
+        xim, etm, zem = (np.asarray(state[k]) for k in ("xim", "etm", "zem"))
+        metric = (_uniform(xim[:lmx], "xi_x"), _uniform(etm[lmx:2 * lmx], "eta_y"), _uniform(zem[2 * lmx:], "zeta_z"))
+        for arr, zero in ((xim, (1, 2)), (etm, (0, 2)), (zem, (0, 1))):
+            for z in zero:
+                if np.any(arr[z * lmx:(z + 1) * lmx] != 0):
+                    raise ValueError("fused path needs diagonal metrics")
+        cfg = SolverConfig()
+        cfg.dims = Dims(*self.dims)
+        flat = [int(v) for row in nbc for v in row]
+        cfg.nbc = (ctypes.c_int * 6)(*flat)
+        cfg.viscous = int(viscous)
+        cfg.umf = Point(*state["umf"])
+        cfg.mu = _uniform(np.asarray(state["ssk"]), "ssk")
+        cfg.metric = (ctypes.c_double * 3)(*metric)
+        cfg.yaco = _uniform(np.asarray(state["yaco"]), "yaco")
+        cfg.cfl = float(cfl)
+        cfg.alpha_f = float(alpha_f)
+        cfg.filter = int(filt)
+        self.cfg = cfg
+        self.viscous = bool(viscous)
+        h = ctypes.c_void_p()
+        check(lib().cs_solver_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.sbcc = torch.as_tensor(np.asarray(state["sbcc"], dtype=np.float64)).cuda().contiguous()
+        self.de = torch.zeros(5 * lmx, dtype=torch.float64, device="cuda")
+        self.num = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cs_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rhs(self, q, de=None, stream=None):
+        de = self.de if de is None else de
+        check(lib().cs_solver_rhs(self.h, _p(q), _p(de), _p(self.sbcc), _stream(stream)))
+        return de
+
+    def sweep(self, which, q, de=None, stream=None):
+        de = self.de if de is None else de
+        check(lib().cs_solver_sweep(self.h, int(which), _p(q), _p(de), _p(self.sbcc), _stream(stream)))
+
+    def time_step(self, q, stream=None):
+        dt = ctypes.c_double()
+        check(lib().cs_solver_time_step(self.h, _p(q), ctypes.byref(dt), _stream(stream)))
+        return dt.value
+
+    def rk4_step(self, q, dt=0.0, report_dt=False, stream=None):
+        out = ctypes.c_double()
+        check(lib().cs_solver_rk4_step(self.h, _p(q), _p(self.sbcc), float(dt),
+                                       ctypes.byref(out) if report_dt else None, _stream(stream)))
+        return out.value if report_dt else None
+
+    def time_sweeps(self, q, stream, reps=10):
+        """Average duration (ms) of each sweep, HIP events on the launch stream."""
+        import torch
+        self.rhs(q, stream=stream)
+        res = []
+        nbytes = SWEEP_BYTES_VISC if self.viscous else SWEEP_BYTES_EULER
+        for w in range(6):
+            if nbytes[w] == 0:
+                continue
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            self.sweep(w, q, stream=stream)
+            e0.record(stream)
+            for _ in range(reps):
+                self.sweep(w, q, stream=stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            res.append({"name": SWEEP_NAMES[w], "ms": e0.elapsed_time(e1) / reps,
+                        "bytes": float(nbytes[w]) * self.lmx})
+        return res
+
+    def time_dominant_kernel(self, q, stream, reps=10):
+        sw = self.time_sweeps(q, stream, reps)
+        dom = max(sw, key=lambda r: r["ms"])
+        out = dict(dom)
+        out["name"] = f"cs::fused::sweep_kernel ({dom['name']})"
+        out["all"] = [{"sweep": r["name"], "ms": round(r["ms"], 5),
+                       "GBs": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 1)} for r in sw]
+        return out

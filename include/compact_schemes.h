@@ -150,6 +150,53 @@ int cs_wall_condition_update(cs_numerics* num, double* qa, const int* npex, cs_p
 int cs_rk_stage(const cs_dims* dims, double* q, const double* q0, const double* de,
                 const double* yaco, double dtk, cs_stream stream);
 
+/* ---- fused right-hand side and RK4 step (one call per step) -------------
+ * The reference evaluates the RHS as calc_viscous_shear_stress + calc_fluxes
+ * + gcbc_go (src/physics_pc.hpp:83-227, src/gcbc.hpp:86-186) and leaves the
+ * stage update as a stub (driver/canard.cu:132-205, physics_base::movef).
+ * cs_solver computes the same quantities with six fused line sweeps per RHS
+ * on a uniform Cartesian grid: the metrics are the constant diagonals
+ * xim = (metric[0],0,0), etm = (0,metric[1],0), zem = (0,0,metric[2]), yaco
+ * and ssk (= mu) are constants, cm = +e_axis.  Per axis both faces must be
+ * non-reflective (10) or both seam codes (35/40/45); walls and curved
+ * interfaces use the per-operator API above. */
+typedef struct {
+    cs_dims  dims;
+    int      nbc[6];      /* [2 faces][3 axes] */
+    int      viscous;
+    cs_point umf;         /* frame velocity (amach1..3, physics_base.hpp:135-145) */
+    double   mu;          /* ssk: constant dynamic viscosity, 1/Re */
+    double   metric[3];   /* xi_x, eta_y, zeta_z */
+    double   yaco;        /* Jacobian */
+    double   cfl;         /* canard.cu:120 uses 0.95 */
+    double   alpha_f;     /* compact filter parameter (0.49) */
+    int      filter;      /* filter the conserved variables after each RK4 step */
+} cs_solver_config;
+
+typedef struct cs_solver cs_solver;
+
+int cs_solver_create(const cs_solver_config* cfg, cs_solver** out);
+int cs_solver_destroy(cs_solver* sv);
+/* de = the reference's de after calc_fluxes + gcbc_go (dq/dt = -yaco*de).
+ * sbcc: [lmx] characteristic-update weights (read on the 4 rows next to each
+ * non-reflective face). */
+int cs_solver_rhs(cs_solver* sv, const double* qa, double* de, const double* sbcc, cs_stream stream);
+/* one of the six sweeps of cs_solver_rhs (0..5 = gradients x, y, z, flux
+ * divergence x, y, z) on the solver's scratch state; no reference
+ * counterpart -- it exists to time and profile single kernels.  Sweeps
+ * depend on the outputs of the earlier ones of the same RHS. */
+int cs_solver_sweep(cs_solver* sv, int which, const double* qa, double* de, const double* sbcc,
+                    cs_stream stream);
+/* time step of physics::calc_time_step on qa (D9 corrected); dte may be NULL
+ * (then nothing synchronises and the value stays on the device). */
+int cs_solver_time_step(cs_solver* sv, const double* qa, double* dte, cs_stream stream);
+/* one RK4 step in place on qa: dt from cs_solver_time_step when dt <= 0,
+ * stages q_k = q0 - dt/(5-k) * yaco * de(q_{k-1}), k = 1..4, then the compact
+ * filter along x, y, z when cfg.filter.  dt_out (host, may be NULL)
+ * receives the step used; passing it synchronises the stream. */
+int cs_solver_rk4_step(cs_solver* sv, double* qa, const double* sbcc, double dt, double* dt_out,
+                       cs_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,120 @@
+"""GPU parity of the fused RHS / RK4 path (cs_solver) against the CPU oracle.
+
+Bar: FP64, norm-wise relative difference <= 1e-12 on the RHS; <= 1e-10 on the
+change of the state over one RK4 step (the change is ~dt * RHS, so this is
+the RHS bar carried through four stages and the filter).
+"""
+import numpy as np
+import pytest
+
+from gpu_util import case, dev, host, nrel
+
+CL = [[10, 10, 10], [10, 10, 10]]
+PER = [[45, 45, 45], [45, 45, 45]]
+MIX = [[45, 10, 10], [45, 10, 10]]
+
+
+def _state(dims, seed, noise=1e-2, width=3.0):
+    from compact_schemes_amd import workloads as wl
+    return wl.flow_state(dims, seed=seed, noise=noise, width=width)
+
+
+def _oracle_rhs(oracle, dims, nbc, st, visc):
+    lmx = dims[0] * dims[1] * dims[2]
+    ctx = oracle.Context(dims, nbc)
+    de, p = np.zeros(5 * lmx), np.zeros(lmx)
+    ctx.rhs(int(visc), st["qa"], de, p, st["ssk"], st["xim"], st["etm"], st["zem"], st["yaco"], st["cm"],
+            st["sbcc"], st["umf"])
+    return de
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims", [(32, 28, 24), (48, 40, 36), (100, 33, 17), (16, 64, 20)])
+@pytest.mark.parametrize("nbc", [CL, PER, MIX], ids=["closure", "periodic", "mixed"])
+@pytest.mark.parametrize("visc", [True, False], ids=["viscous", "euler"])
+def test_fused_rhs_matches_oracle(oracle, dims, nbc, visc):
+    from compact_schemes_amd.fused import FusedSolver
+    st = _state(dims, seed=3)
+    lmx = dims[0] * dims[1] * dims[2]
+    sv = FusedSolver(dims, nbc, st, viscous=visc)
+    q = dev(st["qa"])
+    case(f"fused rhs dims={dims} nbc={nbc[0]} visc={visc}")
+    de = host(sv.rhs(q))
+    ref = _oracle_rhs(oracle, dims, nbc, st, visc)
+    for m in range(5):
+        assert nrel(de[m * lmx:(m + 1) * lmx], ref[m * lmx:(m + 1) * lmx]) < 1e-12, f"variable {m}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims,nbc", [((32, 28, 24), CL), ((40, 24, 20), MIX), ((24, 24, 24), PER)])
+def test_fused_rk4_step_matches_oracle(oracle, dims, nbc):
+    from compact_schemes_amd.fused import FusedSolver
+    st = _state(dims, seed=6)
+    lmx = dims[0] * dims[1] * dims[2]
+    sv = FusedSolver(dims, nbc, st)
+    q = dev(st["qa"])
+    case(f"fused rk4 dims={dims} nbc={nbc[0]}")
+    dt = sv.rk4_step(q, report_dt=True)
+    ctx = oracle.Context(dims, nbc)
+    qo = st["qa"].copy()
+    dto = ctx.rk4_step(1, qo, st["ssk"], st["xim"], st["etm"], st["zem"], st["yaco"], st["cm"], st["sbcc"],
+                       st["umf"])
+    assert abs(dt - dto) / dto < 1e-14
+    qh = host(q)
+    for v in range(5):
+        sl = slice(v * lmx, (v + 1) * lmx)
+        assert nrel(qh[sl] - st["qa"][sl], qo[sl] - st["qa"][sl]) < 1e-10, f"variable {v}"
+
+
+@pytest.mark.gpu
+def test_fused_fixed_dt_and_time_step(oracle):
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (24, 20, 18)
+    st = _state(dims, seed=7)
+    sv = FusedSolver(dims, CL, st)
+    q = dev(st["qa"])
+    case("fused time step")
+    dt = sv.time_step(q)
+    lmx = dims[0] * dims[1] * dims[2]
+    de, _ = oracle.primitives(st["qa"], lmx)
+    ref, _ = oracle.time_step(lmx, st["xim"], st["etm"], st["zem"], de, st["yaco"], st["ssk"], st["umf"], 0.95)
+    assert abs(dt - ref) / ref < 1e-14
+    used = sv.rk4_step(q, dt=0.01, report_dt=True)
+    assert used == 0.01
+
+
+@pytest.mark.gpu
+def test_fused_rhs_256_matches_oracle(oracle):
+    """The benchmark workload itself: 256^3, Mach 0.5 flow with the acoustic
+    pulse, non-reflective characteristic BCs on all faces, viscous."""
+    from compact_schemes_amd import workloads as wl
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (256, 256, 256)
+    lmx = 256 ** 3
+    st = wl.flow_state(dims, seed=2)
+    sv = FusedSolver(dims, CL, st)
+    q = dev(st["qa"])
+    case("fused rhs 256^3")
+    de = host(sv.rhs(q))
+    ref = _oracle_rhs(oracle, dims, CL, st, True)
+    for m in range(5):
+        assert nrel(de[m * lmx:(m + 1) * lmx], ref[m * lmx:(m + 1) * lmx]) < 1e-12, f"variable {m}"
+    # uniform flow: every derivative vanishes, the RHS is zero to rounding
+    st0 = wl.flow_state(dims, seed=2, amp=0.0)
+    case("fused rhs 256^3 uniform")
+    de0 = host(sv.rhs(dev(st0["qa"])))
+    assert np.max(np.abs(de0)) < 1e-13
+
+
+@pytest.mark.gpu
+def test_fused_errors():
+    import compact_schemes_amd as cs
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (24, 20, 18)
+    st = _state(dims, seed=1)
+    with pytest.raises(cs.CsError) as e:
+        FusedSolver(dims, [[20, 10, 10], [20, 10, 10]], st)          # walls: per-operator API
+    assert e.value.code == 4
+    with pytest.raises(cs.CsError) as e:
+        FusedSolver((24, 20, 8), CL, _state((24, 20, 8), seed=1))
+    assert e.value.code == 2
