@@ -14,10 +14,10 @@
 // (Mf, Rb).  Any line length 8..512 works.
 //
 // Layouts (one workgroup = TL lines x P chunks, threads line-fastest):
-//   y/z lines (AX = 1, 2): TL = 64, lanes of a wave are 64 consecutive
-//     lines, so every load and store of a wave touches 64 consecutive
-//     doubles of one row; the chunk index is wave-uniform and the tables are
-//     read with scalar loads;
+//   y/z lines (AX = 1, 2): TL = 64 (32 for 32-point chunks), lanes of a
+//     wave are consecutive lines, so every load and store of a wave touches
+//     consecutive doubles of one row; with TL = 64 the chunk index is
+//     wave-uniform and the tables are read with scalar loads;
 //   x lines   (AX = 0): TL = 16, the 16 lines (contiguous in memory) are
 //     staged through LDS with coalesced loads and stores, transposed so
 //     that a thread reads its chunk conflict-free.
@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(max_threads(K)) line_kernel(LineArgs a) {
     const int tid = threadIdx.x;
     const int lloc = tid % TL;
     int c = tid / TL;
-    if (AX != 0) c = __builtin_amdgcn_readfirstlane(c);
+    if (AX != 0 && TL == 64) c = __builtin_amdgcn_readfirstlane(c);   // one chunk per wave
     const int N = a.N;
     const int P = a.P;
     const long line0 = (long)blockIdx.x * TL;

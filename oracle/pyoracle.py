@@ -154,9 +154,12 @@ class Context:
                                    int(table), int(solver), float(af))
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().orc_destroy(self._h)
-            self._h = None
+        try:
+            if getattr(self, "_h", None):
+                lib().orc_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
 
     def _view(self, p, n):
         return np.ctypeslib.as_array(p, shape=(n,))
