@@ -34,8 +34,8 @@ EXPORTS = [
     "cs_physics_create", "cs_physics_destroy", "cs_physics_set_frame", "cs_physics_buffers",
     "cs_primitives", "cs_calc_fluxes", "cs_calc_viscous_shear_stress", "cs_calc_time_step",
     "cs_gcbc_go", "cs_wall_condition_update", "cs_rk_stage",
-    "cs_solver_create", "cs_solver_destroy", "cs_solver_rhs", "cs_solver_sweep", "cs_solver_time_step",
-    "cs_solver_rk4_step",
+    "cs_solver_create", "cs_solver_destroy", "cs_solver_rhs", "cs_solver_sweep", "cs_solver_tune",
+    "cs_solver_time_step", "cs_solver_rk4_step",
 ]
 
 
@@ -115,6 +115,7 @@ def lib():
             "cs_solver_destroy": (I, [P]),
             "cs_solver_rhs": (I, [P, P, P, P, P]),
             "cs_solver_sweep": (I, [P, I, P, P, P, P]),
+            "cs_solver_tune": (I, [P, I, I, I, I, I]),
             "cs_solver_time_step": (I, [P, P, DP, P]),
             "cs_solver_rk4_step": (I, [P, P, P, D, DP, P]),
         }

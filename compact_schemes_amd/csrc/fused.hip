@@ -39,6 +39,7 @@ template <int AX>
 struct S1 {
     static constexpr int NF = 4;
     static constexpr bool kGcbc = false;   // gradients only; BC updates act on the flux sweeps
+    static constexpr bool kPv = AX == 2;   // the z sweep needs u, v, w at its rows
     const double* q;
     long lmx;
     double* gout;          // AX < 2: [4][lmx] index-space derivatives
@@ -91,6 +92,7 @@ template <int AX, int MODE, bool VISC>
 struct S2 {
     static constexpr int NF = 5;
     static constexpr bool kGcbc = true;
+    static constexpr bool kPv = false;
     const double* q;
     long lmx;
     const double* tau;
@@ -168,7 +170,255 @@ struct S2 {
 };
 
 // ---------------------------------------------------------------------------
-// generic sweep kernel
+// pieces shared by the sweep kernels
+// ---------------------------------------------------------------------------
+
+// interior rows and closure rows from a window w[f][u] holding rows s-2+u
+// (build_system, numerics_device_pc.hpp:176-293)
+template <int K, int NF, int W>
+__device__ __forceinline__ void window_rhs(const double (&w)[NF][W], double (&r)[NF][K], int L, bool first,
+                                           bool last) {
+#pragma unroll
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+        for (int t = 0; t < K; t++)
+            r[f][t] = kAa * (w[f][t + 3] - w[f][t + 1]) + kBb * (w[f][t + 4] - w[f][t]);
+    if (first) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            r[f][0] = kAb00 * w[f][2] + kAb01 * w[f][3] + kAb02 * w[f][4];
+            r[f][1] = kAb10 * (w[f][4] - w[f][2]);
+        }
+    }
+    if (last) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            double fN1 = 0.0, fN2 = 0.0, fN3 = 0.0;
+#pragma unroll
+            for (int u = 0; u < W; u++) {
+                if (u == L + 1) fN1 = w[f][u];
+                if (u == L) fN2 = w[f][u];
+                if (u == L - 1) fN3 = w[f][u];
+            }
+            const double r1 = -(kAb00 * fN1 + kAb01 * fN2 + kAb02 * fN3);
+            const double r2 = -kAb10 * (fN3 - fN1);
+#pragma unroll
+            for (int t = 0; t < K; t++) {
+                if (t == L - 1) r[f][t] = r1;
+                if (t == L - 2) r[f][t] = r2;
+            }
+        }
+    }
+}
+
+// the same rows as window_rhs, streamed through a 5-row register window so
+// that only the rows a right-hand side needs are live (rows from an LDS tile)
+template <int K, int NF, class ROW>
+__device__ __forceinline__ void stream_rhs(ROW&& row_of, double (&r)[NF][K], int L, bool first, bool last) {
+    constexpr int W = K + 4;
+    double w0[NF], w1[NF], w2[NF], w3[NF], w4[NF];
+    double f0[NF], f1[NF], f2[NF], fN1[NF], fN2[NF], fN3[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        w0[f] = w1[f] = w2[f] = w3[f] = w4[f] = 0.0;
+        f0[f] = f1[f] = f2[f] = fN1[f] = fN2[f] = fN3[f] = 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < W; u++) {
+        double v[NF];
+        row_of(u, v);
+#pragma unroll
+        for (int f = 0; f < NF; f++) { w0[f] = w1[f]; w1[f] = w2[f]; w2[f] = w3[f]; w3[f] = w4[f]; w4[f] = v[f]; }
+        if (u >= 4) {
+            const int t = u - 4;
+#pragma unroll
+            for (int f = 0; f < NF; f++) r[f][t] = kAa * (w3[f] - w1[f]) + kBb * (w4[f] - w0[f]);
+        }
+        if (u == 4) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) { f0[f] = w2[f]; f1[f] = w3[f]; f2[f] = w4[f]; }
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            if (u == L + 1) fN1[f] = w4[f];
+            if (u == L) fN2[f] = w4[f];
+            if (u == L - 1) fN3[f] = w4[f];
+        }
+    }
+    if (first) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            r[f][0] = kAb00 * f0[f] + kAb01 * f1[f] + kAb02 * f2[f];
+            r[f][1] = kAb10 * (f2[f] - f0[f]);
+        }
+    }
+    if (last) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const double r1 = -(kAb00 * fN1[f] + kAb01 * fN2[f] + kAb02 * fN3[f]);
+            const double r2 = -kAb10 * (fN3[f] - fN1[f]);
+#pragma unroll
+            for (int t = 0; t < K; t++) {
+                if (t == L - 1) r[f][t] = r1;
+                if (t == L - 2) r[f][t] = r2;
+            }
+        }
+    }
+}
+
+// quasi-disjoint seam rows, wrapped onto the opposite end of the line
+// (build_system seam branch, numerics_device_pc.hpp:199-282, with the
+// neighbour sums formed as fill_gpu_buffer_kernel does, kernels/numerics_pc.hpp:86-121)
+template <int K, int NF, class SW>
+__device__ __forceinline__ void seam_rows(const SW& sw, long base, long st, int N, int L, bool first, bool last,
+                                          const double* pb, double (&r)[NF][K]) {
+    const double* pbco = pb;
+    const double* pbci = pb + 16;
+#pragma unroll
+    for (int end = 0; end < 2; end++) {
+        if ((end == 0 && !first) || (end == 1 && !last)) continue;
+        double s0[NF], s1[NF], o0[NF], o1[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) { s0[f] = 0.0; s1[f] = 0.0; o0[f] = 0.0; o1[f] = 0.0; }
+        for (int i = 0; i < kLmd; i++) {
+            double v[NF];
+            sw.point(base + (long)(end == 0 ? i : N - 1 - i) * st, v);
+#pragma unroll
+            for (int f = 0; f < NF; f++) { s0[f] += pbci[i] * v[f]; s1[f] += pbci[kLmd + i] * v[f]; }
+        }
+        for (int i = 0; i < kLmd; i++) {
+            double v[NF];
+            sw.point(base + (long)(end == 0 ? N - 1 - i : i) * st, v);
+#pragma unroll
+            for (int f = 0; f < NF; f++) { o0[f] += pbco[i] * v[f]; o1[f] += pbco[kLmd + i] * v[f]; }
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            if (end == 0) {
+                r[f][0] = s0[f] + o0[f];
+                r[f][1] = s1[f] + o1[f];
+            } else {
+                const double r1 = -(s0[f] + o0[f]);
+                const double r2 = -(s1[f] + o1[f]);
+#pragma unroll
+                for (int t = 0; t < K; t++) {
+                    if (t == L - 1) r[f][t] = r1;
+                    if (t == L - 2) r[f][t] = r2;
+                }
+            }
+        }
+    }
+}
+
+template <int K, int NF>
+__device__ __forceinline__ void forward_local(const double* tab, int N, int s, int L, double (&r)[NF][K],
+                                              double (&carry)[NF]) {
+#pragma unroll
+    for (int f = 0; f < NF; f++) carry[f] = 0.0;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        if (t < L) {
+            const double nu = tab[s + t], mu = tab[N + s + t];
+#pragma unroll
+            for (int f = 0; f < NF; f++) { carry[f] = nu * r[f][t] + mu * carry[f]; r[f][t] = carry[f]; }
+        }
+    }
+}
+
+template <int K, int NF>
+__device__ __forceinline__ void backward_local(const double* tab, int N, int s, int L, double (&r)[NF][K],
+                                               double (&carry)[NF]) {
+#pragma unroll
+    for (int f = 0; f < NF; f++) carry[f] = 0.0;
+#pragma unroll
+    for (int t = K - 1; t >= 0; t--) {
+        if (t < L) {
+            const double rh = tab[2 * N + s + t];
+#pragma unroll
+            for (int f = 0; f < NF; f++) { carry[f] = r[f][t] + rh * carry[f]; r[f][t] = carry[f]; }
+        }
+    }
+}
+
+template <int K, int NF>
+__device__ __forceinline__ void apply_carry(const double* tab, int off, int s, int L, double (&r)[NF][K],
+                                            const double (&G)[NF]) {
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        if (t < L) {
+            const double M = tab[off + s + t];
+#pragma unroll
+            for (int f = 0; f < NF; f++) r[f][t] += M * G[f];
+        }
+    }
+}
+
+// carries across the chunks of a line through LDS: buf = [NF][P][TL]
+template <int NF>
+__device__ __forceinline__ void fold_forward(const double* buf, const double* tab, int N, int P, int TL, int c,
+                                             int lloc, double (&G)[NF]) {
+#pragma unroll
+    for (int f = 0; f < NF; f++) G[f] = 0.0;
+    for (int cc = 0; cc < c; cc++) {
+        const int e = (int)(((long)(cc + 1) * N) / P) - 1;
+        const double M = tab[3 * N + e];
+#pragma unroll
+        for (int f = 0; f < NF; f++) G[f] = buf[(f * P + cc) * TL + lloc] + M * G[f];
+    }
+}
+
+template <int NF>
+__device__ __forceinline__ void fold_backward(const double* buf, const double* tab, int N, int P, int TL, int c,
+                                              int lloc, double (&X)[NF]) {
+#pragma unroll
+    for (int f = 0; f < NF; f++) X[f] = 0.0;
+    for (int cc = P - 1; cc > c; cc--) {
+        const int sc = (int)(((long)cc * N) / P);
+        const double R = tab[4 * N + sc];
+#pragma unroll
+        for (int f = 0; f < NF; f++) X[f] = buf[(f * P + cc) * TL + lloc] + R * X[f];
+    }
+}
+
+template <int K, int NF, class SW, bool GCBC>
+__device__ __forceinline__ void face_corrections(const SW& sw, long base, long st, int N, int L, bool first,
+                                                 bool last, const double (&r)[NF][K], double (&corr0)[NF],
+                                                 double (&corr1)[NF]) {
+#pragma unroll
+    for (int f = 0; f < NF; f++) { corr0[f] = 0.0; corr1[f] = 0.0; }
+    if constexpr (GCBC) {
+        if (first) {
+            double drv[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) drv[f] = r[f][0];
+            sw.gcbc(base, 0, drv, corr0);
+        }
+        if (last) {
+            double drv[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                drv[f] = 0.0;
+#pragma unroll
+                for (int t = 0; t < K; t++)
+                    if (t == L - 1) drv[f] = r[f][t];
+            }
+            sw.gcbc(base + (long)(N - 1) * st, 1, drv, corr1);
+        }
+    }
+}
+
+// workgroup -> line group, neighbouring groups on the same XCD (workgroups
+// are dispatched round-robin over the 8 XCDs, each with its own L2)
+__device__ __forceinline__ long group_of(int xcd) {
+    const long w = blockIdx.x;
+    if (!xcd) return w;
+    const long G8 = ((long)gridDim.x / 8) * 8;
+    if (w >= G8) return w;
+    return (w % 8) * (G8 / 8) + w / 8;
+}
+
+// ---------------------------------------------------------------------------
+// variant 0: every chunk thread loads its rows (and halo) straight from HBM
 // ---------------------------------------------------------------------------
 template <int K, int AX, class SW, bool GCBC>
 __global__ void __launch_bounds__(AX == 0 ? 256 : 512) sweep_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
@@ -187,7 +437,7 @@ __global__ void __launch_bounds__(AX == 0 ? 256 : 512) sweep_kernel(AxisDesc ax,
     } else {
         lloc = threadIdx.x % kYLines;
         c = threadIdx.x / kYLines;
-        line = (long)blockIdx.x * kYLines + lloc;
+        line = group_of(ax.xcd) * kYLines + lloc;
     }
     const bool act = line < ax.nlines;
     const bool cv = c < P;
@@ -198,7 +448,6 @@ __global__ void __launch_bounds__(AX == 0 ? 256 : 512) sweep_kernel(AxisDesc ax,
     const long st = ax.st;
     const double* tab = ax.tab;
 
-    // ---- rows s-2 .. s+L+1 through the pointwise functor, right-hand sides ----
     double r[NF][K];
     double pv[NF][K];      // field values at the chunk rows (for the epilogue)
     {
@@ -216,95 +465,16 @@ __global__ void __launch_bounds__(AX == 0 ? 256 : 512) sweep_kernel(AxisDesc ax,
 #pragma unroll
             for (int f = 0; f < NF; f++) w[f][u] = v[f];
         }
+        window_rhs<K, NF, W>(w, r, L, work && c == 0 && !ax.seam, work && c == P - 1 && !ax.seam);
 #pragma unroll
         for (int f = 0; f < NF; f++)
 #pragma unroll
-            for (int t = 0; t < K; t++) {
-                r[f][t] = kAa * (w[f][t + 3] - w[f][t + 1]) + kBb * (w[f][t + 4] - w[f][t]);
-                pv[f][t] = w[f][t + 2];
-            }
-        if (work && c == 0 && !ax.seam) {
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                r[f][0] = kAb00 * w[f][2] + kAb01 * w[f][3] + kAb02 * w[f][4];
-                r[f][1] = kAb10 * (w[f][4] - w[f][2]);
-            }
-        }
-        if (work && c == P - 1 && !ax.seam) {
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                double fN1 = 0.0, fN2 = 0.0, fN3 = 0.0;
-#pragma unroll
-                for (int u = 0; u < W; u++) {
-                    if (u == L + 1) fN1 = w[f][u];
-                    if (u == L) fN2 = w[f][u];
-                    if (u == L - 1) fN3 = w[f][u];
-                }
-                const double r1 = -(kAb00 * fN1 + kAb01 * fN2 + kAb02 * fN3);
-                const double r2 = -kAb10 * (fN3 - fN1);
-#pragma unroll
-                for (int t = 0; t < K; t++) {
-                    if (t == L - 1) r[f][t] = r1;
-                    if (t == L - 2) r[f][t] = r2;
-                }
-            }
-        }
+            for (int t = 0; t < K; t++) pv[f][t] = w[f][t + 2];
     }
-    // quasi-disjoint seam rows, wrapped onto the opposite end of the line
-    // (build_system seam branch, numerics_device_pc.hpp:199-282, with recv
-    // formed as fill_gpu_buffer_kernel does, kernels/numerics_pc.hpp:86-121)
-    if (work && ax.seam && (c == 0 || c == P - 1)) {
-        const double* pbco = cm.pb;
-        const double* pbci = cm.pb + 16;
-#pragma unroll
-        for (int end = 0; end < 2; end++) {
-            if ((end == 0 && c != 0) || (end == 1 && c != P - 1)) continue;
-            double s0[NF], s1[NF], o0[NF], o1[NF];
-#pragma unroll
-            for (int f = 0; f < NF; f++) { s0[f] = 0.0; s1[f] = 0.0; o0[f] = 0.0; o1[f] = 0.0; }
-            for (int i = 0; i < kLmd; i++) {
-                double v[NF];
-                sw.point(base + (long)(end == 0 ? i : N - 1 - i) * st, v);
-#pragma unroll
-                for (int f = 0; f < NF; f++) { s0[f] += pbci[i] * v[f]; s1[f] += pbci[kLmd + i] * v[f]; }
-            }
-            for (int i = 0; i < kLmd; i++) {
-                double v[NF];
-                sw.point(base + (long)(end == 0 ? N - 1 - i : i) * st, v);
-#pragma unroll
-                for (int f = 0; f < NF; f++) { o0[f] += pbco[i] * v[f]; o1[f] += pbco[kLmd + i] * v[f]; }
-            }
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                if (end == 0) {
-                    r[f][0] = s0[f] + o0[f];
-                    r[f][1] = s1[f] + o1[f];
-                } else {
-                    const double r1 = -(s0[f] + o0[f]);
-                    const double r2 = -(s1[f] + o1[f]);
-#pragma unroll
-                    for (int t = 0; t < K; t++) {
-                        if (t == L - 1) r[f][t] = r1;
-                        if (t == L - 2) r[f][t] = r2;
-                    }
-                }
-            }
-        }
-    }
+    if (work && ax.seam && (c == 0 || c == P - 1)) seam_rows<K, NF>(sw, base, st, N, L, c == 0, c == P - 1, cm.pb, r);
 
-    // ---- forward recurrence, carries, backward recurrence, carries ----
-    double carry[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) carry[f] = 0.0;
-#pragma unroll
-    for (int t = 0; t < K; t++) {
-        if (t < L) {
-            const double nu = tab[s + t], mu = tab[N + s + t];
-#pragma unroll
-            for (int f = 0; f < NF; f++) { carry[f] = nu * r[f][t] + mu * carry[f]; r[f][t] = carry[f]; }
-        }
-    }
-    double G[NF];
+    double carry[NF], G[NF], X[NF];
+    forward_local<K, NF>(tab, N, s, L, r, carry);
     if (AX == 0) {
         const int GS = ax.GS;
         double A = cv ? tab[3 * N + s + L - 1] : 1.0;
@@ -328,40 +498,15 @@ __global__ void __launch_bounds__(AX == 0 ? 256 : 512) sweep_kernel(AxisDesc ax,
             G[f] = (c == 0) ? 0.0 : g;
         }
     } else {
-        double* gend = lds;
         if (cv) {
 #pragma unroll
-            for (int f = 0; f < NF; f++) gend[(f * P + c) * kYLines + lloc] = carry[f];
+            for (int f = 0; f < NF; f++) lds[(f * P + c) * kYLines + lloc] = carry[f];
         }
         __syncthreads();
-#pragma unroll
-        for (int f = 0; f < NF; f++) G[f] = 0.0;
-        for (int cc = 0; cc < c && cc < P; cc++) {
-            const int e = (int)(((long)(cc + 1) * N) / P) - 1;
-            const double M = tab[3 * N + e];
-#pragma unroll
-            for (int f = 0; f < NF; f++) G[f] = gend[(f * P + cc) * kYLines + lloc] + M * G[f];
-        }
+        fold_forward<NF>(lds, tab, N, P, kYLines, c, lloc, G);
     }
-#pragma unroll
-    for (int t = 0; t < K; t++) {
-        if (t < L) {
-            const double M = tab[3 * N + s + t];
-#pragma unroll
-            for (int f = 0; f < NF; f++) r[f][t] += M * G[f];
-        }
-    }
-#pragma unroll
-    for (int f = 0; f < NF; f++) carry[f] = 0.0;
-#pragma unroll
-    for (int t = K - 1; t >= 0; t--) {
-        if (t < L) {
-            const double rh = tab[2 * N + s + t];
-#pragma unroll
-            for (int f = 0; f < NF; f++) { carry[f] = r[f][t] + rh * carry[f]; r[f][t] = carry[f]; }
-        }
-    }
-    double X[NF];
+    apply_carry<K, NF>(tab, 3 * N, s, L, r, G);
+    backward_local<K, NF>(tab, N, s, L, r, carry);
     if (AX == 0) {
         const int GS = ax.GS;
         double A = cv ? tab[4 * N + s] : 0.0;
@@ -391,51 +536,13 @@ __global__ void __launch_bounds__(AX == 0 ? 256 : 512) sweep_kernel(AxisDesc ax,
             for (int f = 0; f < NF; f++) xbeg[(f * P + c) * kYLines + lloc] = carry[f];
         }
         __syncthreads();
-#pragma unroll
-        for (int f = 0; f < NF; f++) X[f] = 0.0;
-        for (int cc = P - 1; cc > c; cc--) {
-            const int sc = (int)(((long)cc * N) / P);
-            const double R = tab[4 * N + sc];
-#pragma unroll
-            for (int f = 0; f < NF; f++) X[f] = xbeg[(f * P + cc) * kYLines + lloc] + R * X[f];
-        }
+        fold_backward<NF>(xbeg, tab, N, P, kYLines, c, lloc, X);
     }
-#pragma unroll
-    for (int t = 0; t < K; t++) {
-        if (t < L) {
-            const double R = tab[4 * N + s + t];
-#pragma unroll
-            for (int f = 0; f < NF; f++) r[f][t] += R * X[f];
-        }
-    }
+    apply_carry<K, NF>(tab, 4 * N, s, L, r, X);
 
     if (!work) return;
-
-    // ---- characteristic BCs of the two faces of this axis ----
     double corr0[NF], corr1[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) { corr0[f] = 0.0; corr1[f] = 0.0; }
-    if constexpr (GCBC) {
-        if (c == 0) {
-            double drv[NF];
-#pragma unroll
-            for (int f = 0; f < NF; f++) drv[f] = r[f][0];
-            sw.gcbc(base, 0, drv, corr0);
-        }
-        if (c == P - 1) {
-            double drv[NF];
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                drv[f] = 0.0;
-#pragma unroll
-                for (int t = 0; t < K; t++)
-                    if (t == L - 1) drv[f] = r[f][t];
-            }
-            sw.gcbc(base + (long)(N - 1) * st, 1, drv, corr1);
-        }
-    }
-
-    // ---- epilogue ----
+    face_corrections<K, NF, SW, GCBC>(sw, base, st, N, L, c == 0, c == P - 1, r, corr0, corr1);
 #pragma unroll
     for (int t = 0; t < K; t++) {
         if (t < L) {
@@ -456,9 +563,182 @@ __global__ void __launch_bounds__(AX == 0 ? 256 : 512) sweep_kernel(AxisDesc ax,
     }
 }
 
+// ---------------------------------------------------------------------------
+// variant 1: the pointwise inputs of TL lines pass through an LDS tile one
+// row block (kCB chunks + halo) at a time with coalesced loads -- every point
+// is evaluated once -- and chunk threads read their windows from the tile.
+// The tile memory is then reused for the carries and, for x lines, for a
+// transposed coalesced store.
+// ---------------------------------------------------------------------------
+constexpr int kCB = 8;   // chunks per row block
+
+template <int K>
+__host__ __device__ constexpr int tile_rows() { return kCB * K + 4; }
+
+template <int K, int AX, int TL, int OCC, class SW, bool GCBC>
+__global__ void __launch_bounds__(TL * kMaxP) __attribute__((amdgpu_waves_per_eu(OCC)))
+tile_sweep_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
+    constexpr int NF = SW::NF;
+    constexpr int W = K + 4;
+    constexpr int TLP = TL + 1;
+    constexpr int RB = tile_rows<K>();
+    extern __shared__ double lds[];    // tile [NF][RB][TLP]
+
+    const int N = ax.N, P = ax.P;
+    const int nthr = TL * P;
+    const int tid = threadIdx.x;
+    const int lloc = tid % TL, c = tid / TL;
+    const long line0 = group_of(ax.xcd) * TL;
+    const long line = line0 + lloc;
+    const bool act = line < ax.nlines;
+    const int s = (int)(((long)c * N) / P);
+    const int L = (int)(((long)(c + 1) * N) / P) - s;
+    const long base = line_base(AX, act ? line : 0, cm.lxi, cm.plane);
+    const long st = ax.st;
+    const double* tab = ax.tab;
+    const int nb = (P + kCB - 1) / kCB;
+
+    double r[NF][K];
+    double pv[SW::kPv ? NF : 1][K];
+    for (int b = 0; b < nb; b++) {
+        const int c0 = b * kCB, c1 = min(P, c0 + kCB);
+        const int rs = max(0, (int)(((long)c0 * N) / P) - 2);
+        const int re = min(N, (int)(((long)c1 * N) / P) + 2);
+        const int nrows = re - rs;
+        if (b > 0) __syncthreads();
+        for (int p = tid; p < TL * nrows; p += nthr) {
+            int l, row;
+            if (AX == 0) { l = p / nrows; row = rs + p % nrows; }
+            else         { l = p % TL;    row = rs + p / TL; }
+            const long ln = line0 + l;
+            double v[NF];
+            if (ln < ax.nlines) {
+                sw.point(line_base(AX, ln, cm.lxi, cm.plane) + (long)row * st, v);
+            } else {
+#pragma unroll
+                for (int f = 0; f < NF; f++) v[f] = 0.0;
+            }
+#pragma unroll
+            for (int f = 0; f < NF; f++) lds[(f * RB + row - rs) * TLP + l] = v[f];
+        }
+        __syncthreads();
+        if (c >= c0 && c < c1) {
+            auto row_of = [&](int u, double* v) {
+                const int row = s - 2 + u;
+                const bool in = u < L + 4 && row >= 0 && row < N;
+#pragma unroll
+                for (int f = 0; f < NF; f++) v[f] = in ? lds[(f * RB + row - rs) * TLP + lloc] : 0.0;
+            };
+            stream_rhs<K, NF>(row_of, r, L, c == 0 && !ax.seam, c == P - 1 && !ax.seam);
+            if constexpr (SW::kPv) {
+#pragma unroll
+                for (int t = 0; t < K; t++) {
+                    const int row = s + t;
+#pragma unroll
+                    for (int f = 0; f < NF; f++) pv[f][t] = (t < L) ? lds[(f * RB + row - rs) * TLP + lloc] : 0.0;
+                }
+            }
+        }
+    }
+    if (act && ax.seam && (c == 0 || c == P - 1)) seam_rows<K, NF>(sw, base, st, N, L, c == 0, c == P - 1, cm.pb, r);
+
+    double carry[NF], G[NF], X[NF];
+    forward_local<K, NF>(tab, N, s, L, r, carry);
+    __syncthreads();                                   // tile reads done: reuse for carries
+#pragma unroll
+    for (int f = 0; f < NF; f++) lds[(f * P + c) * TL + lloc] = carry[f];
+    __syncthreads();
+    fold_forward<NF>(lds, tab, N, P, TL, c, lloc, G);
+    apply_carry<K, NF>(tab, 3 * N, s, L, r, G);
+    backward_local<K, NF>(tab, N, s, L, r, carry);
+    double* xbeg = lds + NF * P * TL;
+#pragma unroll
+    for (int f = 0; f < NF; f++) xbeg[(f * P + c) * TL + lloc] = carry[f];
+    __syncthreads();
+    fold_backward<NF>(xbeg, tab, N, P, TL, c, lloc, X);
+    apply_carry<K, NF>(tab, 4 * N, s, L, r, X);
+
+    double corr0[NF], corr1[NF];
+    face_corrections<K, NF, SW, GCBC>(sw, base, st, N, L, act && c == 0, act && c == P - 1, r, corr0, corr1);
+    if constexpr (GCBC) {
+        if (act && (c == 0 || c == P - 1)) {
+#pragma unroll
+            for (int t = 0; t < K; t++) {
+                const int row = s + t;
+                if (t < L && (row < kMbci || row >= N - kMbci)) {
+                    const double wgt = cm.sbcc[base + (long)row * st];
+#pragma unroll
+                    for (int f = 0; f < NF; f++) r[f][t] += wgt * (row < kMbci ? corr0[f] : corr1[f]);
+                }
+            }
+        }
+    }
+
+    if (AX != 0) {
+        if (!act) return;
+#pragma unroll
+        for (int t = 0; t < K; t++) {
+            if (t < L) {
+                const long idx = base + (long)(s + t) * st;
+                double d[NF], p[NF];
+#pragma unroll
+                for (int f = 0; f < NF; f++) { d[f] = r[f][t]; p[f] = SW::kPv ? pv[SW::kPv ? f : 0][t] : 0.0; }
+                sw.store(idx, d, p);
+            }
+        }
+        return;
+    }
+    // x lines: results back through the tile, one row block at a time, stored along the line
+    for (int b = 0; b < nb; b++) {
+        const int c0 = b * kCB, c1 = min(P, c0 + kCB);
+        const int rs = (int)(((long)c0 * N) / P);
+        const int re = (int)(((long)c1 * N) / P);
+        const int nrows = re - rs;
+        __syncthreads();
+        if (c >= c0 && c < c1) {
+#pragma unroll
+            for (int t = 0; t < K; t++)
+                if (t < L)
+#pragma unroll
+                    for (int f = 0; f < NF; f++) lds[(f * RB + s + t - rs) * TLP + lloc] = r[f][t];
+        }
+        __syncthreads();
+        for (int p = tid; p < TL * nrows; p += nthr) {
+            const int l = p / nrows, row = rs + p % nrows;
+            const long ln = line0 + l;
+            if (ln >= ax.nlines) continue;
+            double d[NF], pz[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) { d[f] = lds[(f * RB + row - rs) * TLP + l]; pz[f] = 0.0; }
+            sw.store(line_base(AX, ln, cm.lxi, cm.plane) + (long)row * st, d, pz);
+        }
+    }
+}
+
 template <int K, int AX, class SW, bool GCBC>
 static int launch_sweep(const AxisDesc& ax, const SweepCommon& cm, const SW& sw, hipStream_t s) {
     constexpr int NF = SW::NF;
+    if constexpr (K == 8) if (ax.variant == 1) {
+        const size_t tile = (size_t)NF * tile_rows<K>() * (ax.tl + 1);
+        const size_t carries = (size_t)2 * NF * ax.P * ax.tl;
+        const size_t lds = sizeof(double) * (tile > carries ? tile : carries);
+        dim3 block(ax.tl * ax.P), grid((unsigned)((ax.nlines + ax.tl - 1) / ax.tl));
+        // occ: minimum waves per SIMD the register allocation must allow
+#define CS_TILE_LAUNCH(TLV, OCCV) \
+    hipLaunchKernelGGL((tile_sweep_kernel<K, AX, TLV, OCCV, SW, GCBC>), grid, block, lds, s, ax, cm, sw)
+        if (ax.tl == 8) {
+            if (ax.occ >= 4) CS_TILE_LAUNCH(8, 4);
+            else if (ax.occ == 2) CS_TILE_LAUNCH(8, 2);
+            else CS_TILE_LAUNCH(8, 1);
+        } else {
+            if (ax.occ >= 4) CS_TILE_LAUNCH(16, 4);
+            else if (ax.occ == 2) CS_TILE_LAUNCH(16, 2);
+            else CS_TILE_LAUNCH(16, 1);
+        }
+#undef CS_TILE_LAUNCH
+        CS_HIP(hipGetLastError());
+        return CS_OK;
+    }
     dim3 block, grid;
     size_t lds = 0;
     if (AX == 0) {

@@ -19,6 +19,11 @@ struct AxisDesc {
     long st;
     int seam;            // both faces seam (periodic wrap on one subdomain)
     int gcbc;            // both faces non-reflective characteristic BCs
+    // kernel variant (chosen per axis, see cs_solver_tune):
+    int variant;         // 0: direct loads per chunk; 1: LDS row-block tiles
+    int tl;              // lines per workgroup of the tiled variant (8 or 16)
+    int occ;             // tiled variant: minimum waves per SIMD for the register budget (1, 2, 4)
+    int xcd;             // 1: neighbouring line groups go to workgroups on the same XCD
 };
 
 struct SweepCommon {

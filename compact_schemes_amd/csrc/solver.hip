@@ -112,6 +112,10 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
         ax.st = g.stride(a);
         ax.seam = seam[a];
         ax.gcbc = gc[a];
+        ax.variant = 0;
+        ax.tl = 16;
+        ax.occ = 1;
+        ax.xcd = 0;
         std::vector<double> h;
         build_axis_table(ax.N, ax.seam, ax.K, ax.P, h);
         if (hipMalloc(&sv->tabs[a], h.size() * sizeof(double)) != hipSuccess ||
@@ -153,6 +157,19 @@ int cs_solver_sweep(cs_solver* sv, int which, const double* qa, double* de, cons
     if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
         return fail(CS_ERR_ARG, "non-reflective faces need sbcc");
     return one_sweep(sv, which, qa, sbcc, de, (hipStream_t)stream);
+}
+
+int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int occ, int xcd) {
+    if (!sv || axis < 0 || axis > 2) return fail(CS_ERR_ARG, "bad solver or axis");
+    if (variant < 0 || variant > 1 || (lines != 8 && lines != 16) || (occ != 1 && occ != 2 && occ != 4))
+        return fail(CS_ERR_ARG, "variant 0/1, lines 8/16, occ 1/2/4");
+    if (variant == 1 && sv->ax[axis].K != 8) return fail(CS_ERR_UNSUPPORTED, "tiled variant needs 8-point chunks");
+    AxisDesc& ax = sv->ax[axis];
+    ax.variant = variant;
+    ax.tl = lines;
+    ax.occ = occ;
+    ax.xcd = xcd ? 1 : 0;
+    return CS_OK;
 }
 
 int cs_solver_time_step(cs_solver* sv, const double* qa, double* dte, cs_stream stream) {

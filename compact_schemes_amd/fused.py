@@ -79,6 +79,9 @@ class FusedSolver:
         de = self.de if de is None else de
         check(lib().cs_solver_sweep(self.h, int(which), _p(q), _p(de), _p(self.sbcc), _stream(stream)))
 
+    def tune(self, axis, variant, lines=16, occ=1, xcd=0):
+        check(lib().cs_solver_tune(self.h, int(axis), int(variant), int(lines), int(occ), int(xcd)))
+
     def time_step(self, q, stream=None):
         dt = ctypes.c_double()
         check(lib().cs_solver_time_step(self.h, _p(q), ctypes.byref(dt), _stream(stream)))

@@ -187,6 +187,13 @@ int cs_solver_rhs(cs_solver* sv, const double* qa, double* de, const double* sbc
  * depend on the outputs of the earlier ones of the same RHS. */
 int cs_solver_sweep(cs_solver* sv, int which, const double* qa, double* de, const double* sbcc,
                     cs_stream stream);
+/* kernel variant of the sweeps along `axis` (no reference counterpart; the
+ * reference specialises its NVRTC kernels with -D macros instead,
+ * src/numerics_rtc.hpp:120-126): variant 0 = per-chunk direct loads, 1 = LDS
+ * row-block tiles; lines = 8 or 16 lines per workgroup (tiled); occ = 1, 2
+ * or 4 minimum waves per SIMD; xcd = 1 keeps neighbouring line groups on one
+ * XCD.  Results do not depend on the choice. */
+int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int occ, int xcd);
 /* time step of physics::calc_time_step on qa (D9 corrected); dte may be NULL
  * (then nothing synchronises and the value stays on the device). */
 int cs_solver_time_step(cs_solver* sv, const double* qa, double* dte, cs_stream stream);
