@@ -32,8 +32,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="grid points per axis")
-    ap.add_argument("--path", default="ref", choices=["fused", "ref"],
+    ap.add_argument("--path", default="fused", choices=["fused", "ref"],
                     help="fused sweeps (default) or the reference-ordered per-operator path")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only launch the dominant sweep and a plain device copy --reps times each "
+                         "(the program rocprofv3 --pmc runs for the traffic figures)")
+    ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-reps", type=int, default=3)
     return ap.parse_args()
@@ -68,6 +72,8 @@ def main():
     from compact_schemes_amd import workloads as wl
 
     torch.cuda.set_device(0)
+    if args.pmc_probe:
+        return pmc_probe(args)
     n = args.n
     dims = (n, n, n)
     lmx = n ** 3
@@ -130,13 +136,17 @@ def main():
         bytes_per_launch = 16.0 * lmx          # read + write one FP64 field
         kern = {"name": "cs::line_kernel<16,0,1> (y derivative)", "ms": ms_k, "bytes": bytes_per_launch}
     else:
-        kern = solver.time_dominant_kernel(q_init, stream)
+        q.copy_(q_init)
+        kern = solver.time_dominant_kernel(q, stream)
+        kern["traffic"] = traffic_from_profile(kern["kernel"], n)
 
     achieved = kern["bytes"] / (kern["ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": kern.get("traffic"),
-                "kernel": kern["name"], "kernel_ms": round(kern["ms"], 5),
-                "algorithmic_bytes_per_launch": kern["bytes"]}
+                "kernel": kern.get("kernel", kern["name"]), "sweep": kern["name"],
+                "kernel_ms": round(kern["ms"], 5), "algorithmic_bytes_per_launch": kern["bytes"]}
+    if "all" in kern:
+        roofline["sweeps"] = kern["all"]
 
     cpu = None
     if not args.no_cpu:
@@ -163,6 +173,50 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_rk_stage_256.txt")
+
+
+def traffic_from_profile(kernel, n):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the plain-copy factor);
+    None when the summary does not cover this kernel at this size."""
+    if n != 256 or not os.path.exists(PROFILE_TRAFFIC):
+        return None
+    for line in open(PROFILE_TRAFFIC):
+        parts = line.strip().split("\t")
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
+def pmc_probe(args):
+    """The sweeps of one RK stage and a plain copy of known size, --reps
+    launches each, so that per-dispatch counters are easy to attribute."""
+    import torch
+    from compact_schemes_amd import workloads as wl
+    from compact_schemes_amd.fused import FusedSolver, RK_STAGE_SWEEPS, SWEEP_NAMES
+    n = args.n
+    dims = (n, n, n)
+    lmx = n ** 3
+    st = wl.flow_state(dims, seed=2)
+    q = torch.from_numpy(st["qa"]).cuda()
+    solver = FusedSolver(dims, wl.NBC_NON_REFLECTIVE, st)
+    stream = torch.cuda.current_stream()
+    solver.time_step(q, stream=stream)
+    for w in RK_STAGE_SWEEPS:                 # the inputs of every sweep
+        solver.sweep(w, q, stream=stream)
+    for w in RK_STAGE_SWEEPS:
+        for _ in range(args.reps):
+            solver.sweep(w, q, stream=stream)
+    src = torch.ones(5 * lmx, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"sweeps": {SWEEP_NAMES[w]: solver.kernel_name(w) for w in RK_STAGE_SWEEPS}, "reps": args.reps,
+                      "copy_bytes_read": 8 * 5 * lmx, "copy_bytes_written": 8 * 5 * lmx}), flush=True)
 
 
 def cpu_baseline(st, dims, reps):

@@ -115,7 +115,7 @@ def lib():
             "cs_solver_destroy": (I, [P]),
             "cs_solver_rhs": (I, [P, P, P, P, P]),
             "cs_solver_sweep": (I, [P, I, P, P, P, P]),
-            "cs_solver_tune": (I, [P, I, I, I, I, I]),
+            "cs_solver_tune": (I, [P, I, I, I, I]),
             "cs_solver_time_step": (I, [P, P, DP, P]),
             "cs_solver_rk4_step": (I, [P, P, P, D, DP, P]),
         }

@@ -15,15 +15,34 @@
 // two faces from the boundary values of its own solve (the reference's drva).
 //
 // Line solve: chunked LU recurrences as in line_solver.hip, chunks balanced
-// (lengths differ by at most one, every chunk >= 4 rows).  Thread layouts:
-//   x lines: lanes of a wave are the chunks of 64/GS lines (GS = pow2 >= P);
-//            carries by segmented Kogge-Stone scans with wave shuffles;
-//   y/z lines: 16 consecutive lines x P chunks per workgroup (128-byte
-//            coalesced rows), carries through LDS.
+// (lengths differ by at most one, every chunk >= 4 rows).  Kernel variants,
+// chosen per axis (cs_solver_tune):
+//   direct: a thread evaluates the pointwise inputs of its own chunk rows and
+//     gets the two halo rows on each side from the neighbouring chunks (wave
+//     shuffles for x lines, LDS for y/z lines); carries by wave scans (x) or
+//     LDS folds (y/z).  Needs equal chunks (N = P*K); other lengths run the
+//   ragged kernel, where every chunk also evaluates its halo rows;
+//   tile: the inputs of TL lines pass through an LDS tile one row block at a
+//     time with coalesced loads (x lines load along the line).
+// Pointwise arithmetic (primitives, fluxes, closure and seam rows, stress,
+// characteristic BCs, stage update) is not contracted, so it rounds as the
+// CPU oracle does; the recurrences of the solve use FMA.
+//
+// Addressing: a field is at most 4 GiB, so point offsets are 32-bit byte
+// offsets added to the per-field base pointers the kernel arguments hold.
 #include "fused.hpp"
 
 namespace cs {
 namespace fused {
+
+typedef unsigned int uoff;    // byte offset inside one field
+
+__device__ __forceinline__ double ldg(const double* p, uoff bo) {
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(p) + bo);
+}
+__device__ __forceinline__ void stg(double* p, uoff bo, double v) {
+    *reinterpret_cast<double*>(reinterpret_cast<char*>(p) + bo) = v;
+}
 
 // ---------------------------------------------------------------------------
 // sweep functors
@@ -34,38 +53,42 @@ struct Phys {
     double m[3];   // metric scale per axis: xi_x, eta_y, zeta_z
 };
 
+__device__ __forceinline__ Prim prim_at(const double* const* q, uoff bo, double* qq) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) qq[v] = ldg(q[v], bo);
+    return primitive(qq);
+}
+
 // S1: velocity and c^2 gradients along AX
 template <int AX>
 struct S1 {
     static constexpr int NF = 4;
     static constexpr bool kGcbc = false;   // gradients only; BC updates act on the flux sweeps
-    static constexpr bool kPv = AX == 2;   // the z sweep needs u, v, w at its rows
-    const double* q;
-    long lmx;
-    double* gout;          // AX < 2: [4][lmx] index-space derivatives
-    const double* gx;      // AX == 2: x and y derivatives from S1x, S1y
-    const double* gy;
-    double* tau;           // [6][lmx]  xx, yy, zz, xy, yz, zx
-    double* hf;            // [3][lmx]
+    static constexpr bool kPv = AX == 2;   // the z sweep needs u, v, w, c^2 at its rows
+    const double* q[5];
+    double* gout[4];          // AX < 2: index-space derivatives of u, v, w, c^2
+    const double* gx[4];      // AX == 2: derivatives from S1x, S1y
+    const double* gy[4];
+    double* tau[6];           // xx, yy, zz, xy, yz, zx
+    double* hf[3];
     Phys ph;
 
-    __device__ __forceinline__ void point(long idx, double* v) const {
+    __device__ __forceinline__ void point(uoff bo, double* v) const {
         double qq[5];
-        load_q(q, idx, lmx, qq);
-        Prim p = primitive(qq);
+        Prim p = prim_at(q, bo, qq);
         v[0] = p.u;
         v[1] = p.v;
         v[2] = p.w;
         v[3] = p.c2;
     }
-    __device__ __forceinline__ void store(long idx, const double* d, const double* pv) const {
+    __device__ __forceinline__ void store(uoff bo, const double* d, const double* pv) const {
         if (AX < 2) {
 #pragma unroll
-            for (int f = 0; f < 4; f++) gout[(long)f * lmx + idx] = d[f];
+            for (int f = 0; f < 4; f++) stg(gout[f], bo, d[f]);
         } else {
             double a[4], b[4];
 #pragma unroll
-            for (int f = 0; f < 4; f++) { a[f] = gx[(long)f * lmx + idx]; b[f] = gy[(long)f * lmx + idx]; }
+            for (int f = 0; f < 4; f++) { a[f] = ldg(gx[f], bo); b[f] = ldg(gy[f], bo); }
             // chain rule of calc_viscous_shear_stress_post_compute_kernel with
             // diagonal metrics (src/cuda/kernels/physics.hpp:267-281)
             double g[9];
@@ -79,11 +102,12 @@ struct S1 {
             double t6[6], h3[3], d4;
             viscous_closure(ph.mu, ph.J, g, c2g, pv, t6, h3, &d4);
 #pragma unroll
-            for (int k = 0; k < 6; k++) tau[(long)k * lmx + idx] = t6[k];
+            for (int k = 0; k < 6; k++) stg(tau[k], bo, t6[k]);
 #pragma unroll
-            for (int k = 0; k < 3; k++) hf[(long)k * lmx + idx] = h3[k];
+            for (int k = 0; k < 3; k++) stg(hf[k], bo, h3[k]);
         }
     }
+    __device__ __forceinline__ void gcbc(uoff, int, const double*, double*) const {}
 };
 
 // S2: flux divergence along AX.  MODE 0: write the partial RHS; 1: add to it;
@@ -93,23 +117,21 @@ struct S2 {
     static constexpr int NF = 5;
     static constexpr bool kGcbc = true;
     static constexpr bool kPv = false;
-    const double* q;
-    long lmx;
-    const double* tau;
-    const double* hf;
-    const double* din;
-    double* dout;
-    const double* q0;
+    const double* q[5];
+    const double* tr[3];      // row AX of the stress tensor
+    const double* hx;         // heat flux component AX
+    const double* din[5];
+    double* dout[5];
+    const double* q0[5];
     const double* dtp;
     int div;
     Phys ph;
 
     // the AX row of calc_fluxes_pre_compute_kernel with metric row m*e_AX
     // (src/cuda/kernels/physics.hpp:98-191); zero metric terms dropped
-    __device__ __forceinline__ void point(long idx, double* F) const {
+    __device__ __forceinline__ void point(uoff bo, double* F) const {
         double qq[5];
-        load_q(q, idx, lmx, qq);
-        Prim p = primitive(qq);
+        Prim p = prim_at(q, bo, qq);
         const double m = ph.m[AX];
         const double vel = (AX == 0) ? p.u : (AX == 1 ? p.v : p.w);
         const double uf = (AX == 0) ? ph.umf.x : (AX == 1 ? ph.umf.y : ph.umf.z);
@@ -121,39 +143,35 @@ struct S2 {
         F[1 + AX] += m * p.p;
         F[4] = (qq[4] + p.p) * ss - p.p * (uf * m);
         if (VISC) {
-            constexpr int t0 = AX == 0 ? 0 : (AX == 1 ? 3 : 5);   // row AX of the stress tensor
-            constexpr int t1 = AX == 0 ? 3 : (AX == 1 ? 1 : 4);
-            constexpr int t2 = AX == 0 ? 5 : (AX == 1 ? 4 : 2);
-            F[1] -= m * tau[(long)t0 * lmx + idx];
-            F[2] -= m * tau[(long)t1 * lmx + idx];
-            F[3] -= m * tau[(long)t2 * lmx + idx];
-            F[4] -= m * hf[(long)AX * lmx + idx];
+            F[1] -= m * ldg(tr[0], bo);
+            F[2] -= m * ldg(tr[1], bo);
+            F[3] -= m * ldg(tr[2], bo);
+            F[4] -= m * ldg(hx, bo);
         }
     }
-    __device__ __forceinline__ void store(long idx, const double* d, const double*) const {
+    __device__ __forceinline__ void store(uoff bo, const double* d, const double*) const {
         if (MODE == 0) {
 #pragma unroll
-            for (int f = 0; f < 5; f++) dout[(long)f * lmx + idx] = d[f];
+            for (int f = 0; f < 5; f++) stg(dout[f], bo, d[f]);
         } else if (MODE == 1 || MODE == 2) {
 #pragma unroll
-            for (int f = 0; f < 5; f++) dout[(long)f * lmx + idx] = din[(long)f * lmx + idx] + d[f];
+            for (int f = 0; f < 5; f++) stg(dout[f], bo, ldg(din[f], bo) + d[f]);
         } else {
             const double dtk = dtp[0] / (double)div;
             const double fj = dtk * ph.J;
 #pragma unroll
             for (int f = 0; f < 5; f++) {
-                const double de = din[(long)f * lmx + idx] + d[f];
-                dout[(long)f * lmx + idx] = q0[(long)f * lmx + idx] - fj * de;
+                const double de = ldg(din[f], bo) + d[f];
+                stg(dout[f], bo, ldg(q0[f], bo) - fj * de);
             }
         }
     }
     // non-reflective characteristic correction at a face point from the
     // boundary derivatives of this sweep (gcbc_setup_kernel +
     // gcbc_update_non_reflective_kernel, src/cuda/kernels/gcbc.hpp:251-454)
-    __device__ __forceinline__ void gcbc(long idx, int face, const double* drv, double* corr) const {
+    __device__ __forceinline__ void gcbc(uoff bo, int face, const double* drv, double* corr) const {
         double qq[5];
-        load_q(q, idx, lmx, qq);
-        Prim p = primitive(qq);
+        Prim p = prim_at(q, bo, qq);
         const double cm[3] = {AX == 0 ? 1.0 : 0.0, AX == 1 ? 1.0 : 0.0, AX == 2 ? 1.0 : 0.0};
         Eig e;
         e.eleme(cm, qq[0], qq + 1, p.p, ph.umf);
@@ -172,6 +190,13 @@ struct S2 {
 // ---------------------------------------------------------------------------
 // pieces shared by the sweep kernels
 // ---------------------------------------------------------------------------
+// byte offset of row 0 of line f of an axis (f runs over the face's points
+// with the first remaining index fastest)
+__device__ __forceinline__ uoff line_off(int axis, uoff f, uoff lxi, uoff plane) {
+    if (axis == 0) return f * lxi * 8u;
+    if (axis == 1) return ((f / lxi) * plane + (f % lxi)) * 8u;
+    return f * 8u;
+}
 
 // interior rows and closure rows from a window w[f][u] holding rows s-2+u
 // (build_system, numerics_device_pc.hpp:176-293)
@@ -270,7 +295,7 @@ __device__ __forceinline__ void stream_rhs(ROW&& row_of, double (&r)[NF][K], int
 // (build_system seam branch, numerics_device_pc.hpp:199-282, with the
 // neighbour sums formed as fill_gpu_buffer_kernel does, kernels/numerics_pc.hpp:86-121)
 template <int K, int NF, class SW>
-__device__ __forceinline__ void seam_rows(const SW& sw, long base, long st, int N, int L, bool first, bool last,
+__device__ __forceinline__ void seam_rows(const SW& sw, uoff base, uoff st, int N, int L, bool first, bool last,
                                           const double* pb, double (&r)[NF][K]) {
     const double* pbco = pb;
     const double* pbci = pb + 16;
@@ -282,13 +307,13 @@ __device__ __forceinline__ void seam_rows(const SW& sw, long base, long st, int 
         for (int f = 0; f < NF; f++) { s0[f] = 0.0; s1[f] = 0.0; o0[f] = 0.0; o1[f] = 0.0; }
         for (int i = 0; i < kLmd; i++) {
             double v[NF];
-            sw.point(base + (long)(end == 0 ? i : N - 1 - i) * st, v);
+            sw.point(base + (uoff)(end == 0 ? i : N - 1 - i) * st, v);
 #pragma unroll
             for (int f = 0; f < NF; f++) { s0[f] += pbci[i] * v[f]; s1[f] += pbci[kLmd + i] * v[f]; }
         }
         for (int i = 0; i < kLmd; i++) {
             double v[NF];
-            sw.point(base + (long)(end == 0 ? N - 1 - i : i) * st, v);
+            sw.point(base + (uoff)(end == 0 ? N - 1 - i : i) * st, v);
 #pragma unroll
             for (int f = 0; f < NF; f++) { o0[f] += pbco[i] * v[f]; o1[f] += pbco[kLmd + i] * v[f]; }
         }
@@ -310,14 +335,16 @@ __device__ __forceinline__ void seam_rows(const SW& sw, long base, long st, int 
     }
 }
 
-template <int K, int NF>
+// recurrences of the solve (EX: every chunk has K rows)
+template <int K, int NF, bool EX>
 __device__ __forceinline__ void forward_local(const double* tab, int N, int s, int L, double (&r)[NF][K],
                                               double (&carry)[NF]) {
+#pragma clang fp contract(fast)
 #pragma unroll
     for (int f = 0; f < NF; f++) carry[f] = 0.0;
 #pragma unroll
     for (int t = 0; t < K; t++) {
-        if (t < L) {
+        if (EX || t < L) {
             const double nu = tab[s + t], mu = tab[N + s + t];
 #pragma unroll
             for (int f = 0; f < NF; f++) { carry[f] = nu * r[f][t] + mu * carry[f]; r[f][t] = carry[f]; }
@@ -325,14 +352,15 @@ __device__ __forceinline__ void forward_local(const double* tab, int N, int s, i
     }
 }
 
-template <int K, int NF>
+template <int K, int NF, bool EX>
 __device__ __forceinline__ void backward_local(const double* tab, int N, int s, int L, double (&r)[NF][K],
                                                double (&carry)[NF]) {
+#pragma clang fp contract(fast)
 #pragma unroll
     for (int f = 0; f < NF; f++) carry[f] = 0.0;
 #pragma unroll
     for (int t = K - 1; t >= 0; t--) {
-        if (t < L) {
+        if (EX || t < L) {
             const double rh = tab[2 * N + s + t];
 #pragma unroll
             for (int f = 0; f < NF; f++) { carry[f] = r[f][t] + rh * carry[f]; r[f][t] = carry[f]; }
@@ -340,12 +368,13 @@ __device__ __forceinline__ void backward_local(const double* tab, int N, int s, 
     }
 }
 
-template <int K, int NF>
+template <int K, int NF, bool EX>
 __device__ __forceinline__ void apply_carry(const double* tab, int off, int s, int L, double (&r)[NF][K],
                                             const double (&G)[NF]) {
+#pragma clang fp contract(fast)
 #pragma unroll
     for (int t = 0; t < K; t++) {
-        if (t < L) {
+        if (EX || t < L) {
             const double M = tab[off + s + t];
 #pragma unroll
             for (int f = 0; f < NF; f++) r[f][t] += M * G[f];
@@ -354,55 +383,117 @@ __device__ __forceinline__ void apply_carry(const double* tab, int off, int s, i
 }
 
 // carries across the chunks of a line through LDS: buf = [NF][P][TL]
-template <int NF>
+template <int NF, bool EX, int K>
 __device__ __forceinline__ void fold_forward(const double* buf, const double* tab, int N, int P, int TL, int c,
                                              int lloc, double (&G)[NF]) {
+#pragma clang fp contract(fast)
 #pragma unroll
     for (int f = 0; f < NF; f++) G[f] = 0.0;
     for (int cc = 0; cc < c; cc++) {
-        const int e = (int)(((long)(cc + 1) * N) / P) - 1;
+        const int e = EX ? cc * K + K - 1 : (int)(((long)(cc + 1) * N) / P) - 1;
         const double M = tab[3 * N + e];
 #pragma unroll
         for (int f = 0; f < NF; f++) G[f] = buf[(f * P + cc) * TL + lloc] + M * G[f];
     }
 }
 
-template <int NF>
+template <int NF, bool EX, int K>
 __device__ __forceinline__ void fold_backward(const double* buf, const double* tab, int N, int P, int TL, int c,
                                               int lloc, double (&X)[NF]) {
+#pragma clang fp contract(fast)
 #pragma unroll
     for (int f = 0; f < NF; f++) X[f] = 0.0;
     for (int cc = P - 1; cc > c; cc--) {
-        const int sc = (int)(((long)cc * N) / P);
+        const int sc = EX ? cc * K : (int)(((long)cc * N) / P);
         const double R = tab[4 * N + sc];
 #pragma unroll
         for (int f = 0; f < NF; f++) X[f] = buf[(f * P + cc) * TL + lloc] + R * X[f];
     }
 }
 
-template <int K, int NF, class SW, bool GCBC>
-__device__ __forceinline__ void face_corrections(const SW& sw, long base, long st, int N, int L, bool first,
-                                                 bool last, const double (&r)[NF][K], double (&corr0)[NF],
-                                                 double (&corr1)[NF]) {
+// carries across the chunks of a line held by GS consecutive lanes of a wave:
+// segmented Kogge-Stone scans of the affine maps x -> A*x + B
+template <int NF>
+__device__ __forceinline__ void scan_forward(double A, const double (&carry)[NF], bool cv, int c, int GS,
+                                             double (&G)[NF]) {
+#pragma clang fp contract(fast)
+    double B[NF];
 #pragma unroll
-    for (int f = 0; f < NF; f++) { corr0[f] = 0.0; corr1[f] = 0.0; }
-    if constexpr (GCBC) {
-        if (first) {
-            double drv[NF];
+    for (int f = 0; f < NF; f++) B[f] = cv ? carry[f] : 0.0;
+    if (!cv) A = 1.0;
+    for (int d = 1; d < GS; d <<= 1) {
+        const double Au = __shfl_up(A, d, GS);
+        double Bu[NF];
 #pragma unroll
-            for (int f = 0; f < NF; f++) drv[f] = r[f][0];
-            sw.gcbc(base, 0, drv, corr0);
+        for (int f = 0; f < NF; f++) Bu[f] = __shfl_up(B[f], d, GS);
+        if (c >= d) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) B[f] += A * Bu[f];
+            A *= Au;
         }
-        if (last) {
-            double drv[NF];
+    }
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        const double g = __shfl_up(B[f], 1, GS);
+        G[f] = (c == 0) ? 0.0 : g;
+    }
+}
+
+template <int NF>
+__device__ __forceinline__ void scan_backward(double A, const double (&carry)[NF], bool cv, int c, int P, int GS,
+                                              double (&X)[NF]) {
+#pragma clang fp contract(fast)
+    double B[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) B[f] = cv ? carry[f] : 0.0;
+    if (!cv) A = 0.0;
+    for (int d = 1; d < GS; d <<= 1) {
+        const double Ad = __shfl_down(A, d, GS);
+        double Bd[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) Bd[f] = __shfl_down(B[f], d, GS);
+        if (c + d < GS) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) B[f] += A * Bd[f];
+            A *= Ad;
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        const double x = __shfl_down(B[f], 1, GS);
+        X[f] = (c + 1 < P) ? x : 0.0;
+    }
+}
+
+// characteristic correction of a face (first or last chunk of a line, never
+// both: lines have at least two chunks), weighted by sbcc on the kMbci rows
+// next to the face; the face derivative is the uncorrected solve result
+template <int K, int NF, class SW, bool GCBC, bool EX>
+__device__ __forceinline__ void face_corrections(const SW& sw, uoff base, uoff st, int N, int s, int L, bool first,
+                                                 bool last, double (&r)[NF][K], const double* sbcc) {
+    if constexpr (GCBC) {
+        if (first || last) {
+            const int face = last ? 1 : 0;
+            double drv[NF], corr[NF];
 #pragma unroll
             for (int f = 0; f < NF; f++) {
-                drv[f] = 0.0;
+                double d = r[f][0];
 #pragma unroll
-                for (int t = 0; t < K; t++)
-                    if (t == L - 1) drv[f] = r[f][t];
+                for (int t = 1; t < K; t++)
+                    if (last && ((EX && t == K - 1) || (!EX && t == L - 1))) d = r[f][t];
+                drv[f] = d;
             }
-            sw.gcbc(base + (long)(N - 1) * st, 1, drv, corr1);
+            sw.gcbc(base + (uoff)(face ? N - 1 : 0) * st, face, drv, corr);
+#pragma unroll
+            for (int t = 0; t < K; t++) {
+                const int row = s + t;
+                const bool in = (EX || t < L) && (face ? row >= N - kMbci : row < kMbci);
+                if (in) {
+                    const double wgt = ldg(sbcc, base + (uoff)row * st);
+#pragma unroll
+                    for (int f = 0; f < NF; f++) r[f][t] += wgt * corr[f];
+                }
+            }
         }
     }
 }
@@ -418,22 +509,267 @@ __device__ __forceinline__ long group_of(int xcd) {
 }
 
 // ---------------------------------------------------------------------------
-// variant 0: every chunk thread loads its rows (and halo) straight from HBM
+// variant "direct" (equal chunks, N = P*K): each thread evaluates the
+// pointwise inputs of its own K rows once; the two halo rows on each side come
+// from the neighbouring chunks (shuffles within a wave segment for x lines,
+// LDS for y/z lines)
 // ---------------------------------------------------------------------------
-template <int K, int AX, class SW, bool GCBC>
-__global__ void __launch_bounds__(AX == 0 ? 256 : 512) sweep_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
+template <int K, int AX, int TL, class SW, bool GCBC>
+__global__ void __launch_bounds__(AX == 0 ? 256 : TL * kMaxP) __attribute__((amdgpu_waves_per_eu(AX == 0 ? 1 : 3)))
+direct_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
     constexpr int NF = SW::NF;
     constexpr int W = K + 4;
     extern __shared__ double lds[];
 
     const int N = ax.N, P = ax.P;
-    int c, lloc = 0;
+    int c, lloc = 0, GS = 1;
     long line;
     if (AX == 0) {
+        GS = ax.GS;
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        const int lpw = 64 / ax.GS;
-        c = lane % ax.GS;
-        line = (long)blockIdx.x * (4 * lpw) + wave * lpw + lane / ax.GS;
+        const int lpw = 64 / GS;
+        c = lane % GS;
+        line = (long)blockIdx.x * (4 * lpw) + wave * lpw + lane / GS;
+    } else {
+        lloc = threadIdx.x % TL;
+        c = threadIdx.x / TL;
+        line = group_of(ax.xcd) * TL + lloc;
+    }
+    const bool act = line < ax.nlines;
+    const bool cv = c < P;
+    const bool work = act && cv;
+    const int s = c * K;
+    const uoff st = (uoff)ax.st * 8u;
+    const uoff base = line_off(AX, act ? (uoff)line : 0u, (uoff)cm.lxi, (uoff)cm.plane);
+    const double* tab = ax.tab;
+
+    double w[NF][W];
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        double x[NF];
+        if (work) {
+            sw.point(base + (uoff)(s + t) * st, x);
+        } else {
+#pragma unroll
+            for (int f = 0; f < NF; f++) x[f] = 0.0;
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) w[f][t + 2] = x[f];
+    }
+    if (AX == 0) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            w[f][0] = __shfl_up(w[f][K], 1, GS);
+            w[f][1] = __shfl_up(w[f][K + 1], 1, GS);
+            w[f][K + 2] = __shfl_down(w[f][2], 1, GS);
+            w[f][K + 3] = __shfl_down(w[f][3], 1, GS);
+        }
+    } else {
+        // halo rows through LDS: hb[(f*4 + k)*P + c][TL], k = chunk rows 0, 1, K-2, K-1
+        const int cl = c > 0 ? c - 1 : 0, cr = c + 1 < P ? c + 1 : P - 1;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            lds[((f * 4 + 0) * P + c) * TL + lloc] = w[f][2];
+            lds[((f * 4 + 1) * P + c) * TL + lloc] = w[f][3];
+            lds[((f * 4 + 2) * P + c) * TL + lloc] = w[f][K];
+            lds[((f * 4 + 3) * P + c) * TL + lloc] = w[f][K + 1];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            w[f][0] = lds[((f * 4 + 2) * P + cl) * TL + lloc];
+            w[f][1] = lds[((f * 4 + 3) * P + cl) * TL + lloc];
+            w[f][K + 2] = lds[((f * 4 + 0) * P + cr) * TL + lloc];
+            w[f][K + 3] = lds[((f * 4 + 1) * P + cr) * TL + lloc];
+        }
+    }
+    double pv[SW::kPv ? NF : 1][K];
+    if constexpr (SW::kPv) {
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+            for (int t = 0; t < K; t++) pv[f][t] = w[f][t + 2];
+    }
+    double r[NF][K];
+    window_rhs<K, NF, W>(w, r, K, work && c == 0 && !ax.seam, work && c == P - 1 && !ax.seam);
+    if (work && ax.seam && (c == 0 || c == P - 1)) seam_rows<K, NF>(sw, base, st, N, K, c == 0, c == P - 1, cm.pb, r);
+
+    double carry[NF], G[NF], X[NF];
+    forward_local<K, NF, true>(tab, N, s, K, r, carry);
+    if (AX == 0) {
+        scan_forward<NF>(cv ? tab[3 * N + s + K - 1] : 1.0, carry, cv, c, GS, G);
+    } else {
+        __syncthreads();                                // halo reads done: reuse LDS
+#pragma unroll
+        for (int f = 0; f < NF; f++) lds[(f * P + c) * TL + lloc] = carry[f];
+        __syncthreads();
+        fold_forward<NF, true, K>(lds, tab, N, P, TL, c, lloc, G);
+    }
+    apply_carry<K, NF, true>(tab, 3 * N, s, K, r, G);
+    backward_local<K, NF, true>(tab, N, s, K, r, carry);
+    if (AX == 0) {
+        scan_backward<NF>(cv ? tab[4 * N + s] : 0.0, carry, cv, c, P, GS, X);
+    } else {
+        double* xb = lds + NF * P * TL;
+#pragma unroll
+        for (int f = 0; f < NF; f++) xb[(f * P + c) * TL + lloc] = carry[f];
+        __syncthreads();
+        fold_backward<NF, true, K>(xb, tab, N, P, TL, c, lloc, X);
+    }
+    apply_carry<K, NF, true>(tab, 4 * N, s, K, r, X);
+
+    if (!work) return;
+    face_corrections<K, NF, SW, GCBC, true>(sw, base, st, N, s, K, c == 0, c == P - 1, r, cm.sbcc);
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        double d[NF], p[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) { d[f] = r[f][t]; p[f] = SW::kPv ? pv[SW::kPv ? f : 0][t] : 0.0; }
+        sw.store(base + (uoff)(s + t) * st, d, p);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// variant "direct" for x lines (equal chunks): one wave holds the GS-lane
+// chunk groups of 64/GS consecutive lines, which are one contiguous run of
+// memory.  Lanes load and evaluate that run with fully coalesced accesses (J
+// points per lane), the values are transposed field by field through a
+// wave-private LDS buffer into the chunk layout (halo rows included), the
+// line is solved with wave scans, and the results go back through the buffer
+// for coalesced stores.  No workgroup barrier: a workgroup is one wave.
+// ---------------------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ int xt_slot(int li, int P, int c, int t) { return (li * P + c) * (K + 1) + t; }
+
+template <int K, class SW, bool GCBC>
+__global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
+    constexpr int NF = SW::NF;
+    constexpr int J = K;                  // points per lane: (64/GS)*P*K <= 64*K
+    constexpr int W = K + 4;
+    extern __shared__ double lds[];       // [64/GS lines][P chunks][K+1]
+
+    const int N = ax.N, P = ax.P, GS = ax.GS;
+    const int lane = threadIdx.x;
+    const int lpw = 64 / GS;
+    const long line0 = (long)blockIdx.x * lpw;
+    const int nl = (int)min((long)lpw, ax.nlines - line0);   // lines of this wave
+    const int npts = nl * N;
+    const uoff st = (uoff)ax.st * 8u;                        // 8: x rows are adjacent
+    const uoff run = line_off(0, (uoff)line0, (uoff)cm.lxi, (uoff)cm.plane);
+
+    // coalesced evaluation: lane handles points lane + 64*j of the run
+    double xq[NF][J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const int p = lane + 64 * j;
+        double x[NF];
+        if (p < npts) {
+            sw.point(run + (uoff)p * st, x);
+        } else {
+#pragma unroll
+            for (int f = 0; f < NF; f++) x[f] = 0.0;
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) xq[f][j] = x[f];
+    }
+
+    // LDS slot of each of the lane's points (row and line tracked without division)
+    int slot[J];
+    {
+        int l = 0, row = lane;
+        while (row >= N) { row -= N; l++; }
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            slot[j] = lane + 64 * j < npts ? xt_slot<K>(l, P, row / K, row % K) : 0;
+            row += 64;
+            while (row >= N) { row -= N; l++; }
+        }
+    }
+
+    const int li = lane / GS, c = lane % GS;
+    const bool work = li < nl && c < P;
+    const int lr = li < nl ? li : 0;
+    const int cc = c < P ? c : P - 1;
+    const int cl = cc > 0 ? cc - 1 : 0, cr = cc + 1 < P ? cc + 1 : P - 1;
+    const int s = cc * K;
+    const double* tab = ax.tab;
+    const uoff base = run + (uoff)lr * (uoff)N * st;
+
+    double r[NF][K];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < J; j++)
+            if (lane + 64 * j < npts) lds[slot[j]] = xq[f][j];
+        __syncthreads();
+        double w[1][W];
+        w[0][0] = lds[xt_slot<K>(lr, P, cl, K - 2)];
+        w[0][1] = lds[xt_slot<K>(lr, P, cl, K - 1)];
+#pragma unroll
+        for (int t = 0; t < K; t++) w[0][t + 2] = lds[xt_slot<K>(lr, P, cc, t)];
+        w[0][K + 2] = lds[xt_slot<K>(lr, P, cr, 0)];
+        w[0][K + 3] = lds[xt_slot<K>(lr, P, cr, 1)];
+        double rf[1][K];
+        window_rhs<K, 1, W>(w, rf, K, work && c == 0 && !ax.seam, work && c == P - 1 && !ax.seam);
+#pragma unroll
+        for (int t = 0; t < K; t++) r[f][t] = rf[0][t];
+    }
+    if (work && ax.seam && (c == 0 || c == P - 1)) seam_rows<K, NF>(sw, base, st, N, K, c == 0, c == P - 1, cm.pb, r);
+
+    const bool cv = c < P;
+    double carry[NF], G[NF], X[NF];
+    forward_local<K, NF, true>(tab, N, s, K, r, carry);
+    scan_forward<NF>(cv ? tab[3 * N + s + K - 1] : 1.0, carry, cv, c, GS, G);
+    apply_carry<K, NF, true>(tab, 3 * N, s, K, r, G);
+    backward_local<K, NF, true>(tab, N, s, K, r, carry);
+    scan_backward<NF>(cv ? tab[4 * N + s] : 0.0, carry, cv, c, P, GS, X);
+    apply_carry<K, NF, true>(tab, 4 * N, s, K, r, X);
+    if (work) face_corrections<K, NF, SW, GCBC, true>(sw, base, st, N, s, K, c == 0, c == P - 1, r, cm.sbcc);
+
+    // back to the coalesced layout, one field at a time
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        __syncthreads();
+        if (work) {
+#pragma unroll
+            for (int t = 0; t < K; t++) lds[xt_slot<K>(li, P, c, t)] = r[f][t];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < J; j++) xq[f][j] = lds[slot[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const int p = lane + 64 * j;
+        if (p < npts) {
+            double d[NF], pz[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) { d[f] = xq[f][j]; pz[f] = 0.0; }
+            sw.store(run + (uoff)p * st, d, pz);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// variant "ragged": any line length; every chunk thread evaluates its rows and
+// halo rows itself (x lines: carries by shuffles; y/z: LDS folds)
+// ---------------------------------------------------------------------------
+template <int K, int AX, class SW, bool GCBC>
+__global__ void __launch_bounds__(AX == 0 ? 256 : kYLines * kMaxP) ragged_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
+    constexpr int NF = SW::NF;
+    constexpr int W = K + 4;
+    extern __shared__ double lds[];
+
+    const int N = ax.N, P = ax.P;
+    int c, lloc = 0, GS = 1;
+    long line;
+    if (AX == 0) {
+        GS = ax.GS;
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int lpw = 64 / GS;
+        c = lane % GS;
+        line = (long)blockIdx.x * (4 * lpw) + wave * lpw + lane / GS;
     } else {
         lloc = threadIdx.x % kYLines;
         c = threadIdx.x / kYLines;
@@ -444,142 +780,92 @@ __global__ void __launch_bounds__(AX == 0 ? 256 : 512) sweep_kernel(AxisDesc ax,
     const bool work = act && cv;
     const int s = cv ? (int)(((long)c * N) / P) : N;
     const int L = cv ? (int)(((long)(c + 1) * N) / P) - s : 0;
-    const long base = line_base(AX, act ? line : 0, cm.lxi, cm.plane);
-    const long st = ax.st;
+    const uoff st = (uoff)ax.st * 8u;
+    const uoff base = line_off(AX, act ? (uoff)line : 0u, (uoff)cm.lxi, (uoff)cm.plane);
     const double* tab = ax.tab;
 
     double r[NF][K];
-    double pv[NF][K];      // field values at the chunk rows (for the epilogue)
+    double pv[SW::kPv ? NF : 1][K];
     {
         double w[NF][W];
 #pragma unroll
         for (int u = 0; u < W; u++) {
             const int row = s - 2 + u;
-            double v[NF];
+            double x[NF];
             if (work && u < L + 4 && row >= 0 && row < N) {
-                sw.point(base + (long)row * st, v);
+                sw.point(base + (uoff)row * st, x);
             } else {
 #pragma unroll
-                for (int f = 0; f < NF; f++) v[f] = 0.0;
+                for (int f = 0; f < NF; f++) x[f] = 0.0;
             }
 #pragma unroll
-            for (int f = 0; f < NF; f++) w[f][u] = v[f];
+            for (int f = 0; f < NF; f++) w[f][u] = x[f];
         }
         window_rhs<K, NF, W>(w, r, L, work && c == 0 && !ax.seam, work && c == P - 1 && !ax.seam);
+        if constexpr (SW::kPv) {
 #pragma unroll
-        for (int f = 0; f < NF; f++)
+            for (int f = 0; f < NF; f++)
 #pragma unroll
-            for (int t = 0; t < K; t++) pv[f][t] = w[f][t + 2];
+                for (int t = 0; t < K; t++) pv[f][t] = w[f][t + 2];
+        }
     }
     if (work && ax.seam && (c == 0 || c == P - 1)) seam_rows<K, NF>(sw, base, st, N, L, c == 0, c == P - 1, cm.pb, r);
 
     double carry[NF], G[NF], X[NF];
-    forward_local<K, NF>(tab, N, s, L, r, carry);
+    forward_local<K, NF, false>(tab, N, s, L, r, carry);
     if (AX == 0) {
-        const int GS = ax.GS;
-        double A = cv ? tab[3 * N + s + L - 1] : 1.0;
-        double B[NF];
-#pragma unroll
-        for (int f = 0; f < NF; f++) B[f] = cv ? carry[f] : 0.0;
-        for (int d = 1; d < GS; d <<= 1) {
-            const double Au = __shfl_up(A, d, GS);
-            double Bu[NF];
-#pragma unroll
-            for (int f = 0; f < NF; f++) Bu[f] = __shfl_up(B[f], d, GS);
-            if (c >= d) {
-#pragma unroll
-                for (int f = 0; f < NF; f++) B[f] += A * Bu[f];
-                A *= Au;
-            }
-        }
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-            const double g = __shfl_up(B[f], 1, GS);
-            G[f] = (c == 0) ? 0.0 : g;
-        }
+        scan_forward<NF>(cv ? tab[3 * N + s + L - 1] : 1.0, carry, cv, c, GS, G);
     } else {
         if (cv) {
 #pragma unroll
             for (int f = 0; f < NF; f++) lds[(f * P + c) * kYLines + lloc] = carry[f];
         }
         __syncthreads();
-        fold_forward<NF>(lds, tab, N, P, kYLines, c, lloc, G);
+        fold_forward<NF, false, K>(lds, tab, N, P, kYLines, c, lloc, G);
     }
-    apply_carry<K, NF>(tab, 3 * N, s, L, r, G);
-    backward_local<K, NF>(tab, N, s, L, r, carry);
+    apply_carry<K, NF, false>(tab, 3 * N, s, L, r, G);
+    backward_local<K, NF, false>(tab, N, s, L, r, carry);
     if (AX == 0) {
-        const int GS = ax.GS;
-        double A = cv ? tab[4 * N + s] : 0.0;
-        double B[NF];
-#pragma unroll
-        for (int f = 0; f < NF; f++) B[f] = cv ? carry[f] : 0.0;
-        for (int d = 1; d < GS; d <<= 1) {
-            const double Ad = __shfl_down(A, d, GS);
-            double Bd[NF];
-#pragma unroll
-            for (int f = 0; f < NF; f++) Bd[f] = __shfl_down(B[f], d, GS);
-            if (c + d < GS) {
-#pragma unroll
-                for (int f = 0; f < NF; f++) B[f] += A * Bd[f];
-                A *= Ad;
-            }
-        }
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-            const double x = __shfl_down(B[f], 1, GS);
-            X[f] = (c + 1 < P) ? x : 0.0;
-        }
+        scan_backward<NF>(cv ? tab[4 * N + s] : 0.0, carry, cv, c, P, GS, X);
     } else {
-        double* xbeg = lds + NF * P * kYLines;
+        double* xb = lds + NF * P * kYLines;
         if (cv) {
 #pragma unroll
-            for (int f = 0; f < NF; f++) xbeg[(f * P + c) * kYLines + lloc] = carry[f];
+            for (int f = 0; f < NF; f++) xb[(f * P + c) * kYLines + lloc] = carry[f];
         }
         __syncthreads();
-        fold_backward<NF>(xbeg, tab, N, P, kYLines, c, lloc, X);
+        fold_backward<NF, false, K>(xb, tab, N, P, kYLines, c, lloc, X);
     }
-    apply_carry<K, NF>(tab, 4 * N, s, L, r, X);
+    apply_carry<K, NF, false>(tab, 4 * N, s, L, r, X);
 
     if (!work) return;
-    double corr0[NF], corr1[NF];
-    face_corrections<K, NF, SW, GCBC>(sw, base, st, N, L, c == 0, c == P - 1, r, corr0, corr1);
+    face_corrections<K, NF, SW, GCBC, false>(sw, base, st, N, s, L, c == 0, c == P - 1, r, cm.sbcc);
 #pragma unroll
     for (int t = 0; t < K; t++) {
         if (t < L) {
-            const int row = s + t;
-            const long idx = base + (long)row * st;
             double d[NF], p[NF];
 #pragma unroll
-            for (int f = 0; f < NF; f++) { d[f] = r[f][t]; p[f] = pv[f][t]; }
-            if constexpr (GCBC) {
-                if (row < kMbci || row >= N - kMbci) {
-                    const double wgt = cm.sbcc[idx];
-#pragma unroll
-                    for (int f = 0; f < NF; f++) d[f] += wgt * (row < kMbci ? corr0[f] : corr1[f]);
-                }
-            }
-            sw.store(idx, d, p);
+            for (int f = 0; f < NF; f++) { d[f] = r[f][t]; p[f] = SW::kPv ? pv[SW::kPv ? f : 0][t] : 0.0; }
+            sw.store(base + (uoff)(s + t) * st, d, p);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// variant 1: the pointwise inputs of TL lines pass through an LDS tile one
-// row block (kCB chunks + halo) at a time with coalesced loads -- every point
-// is evaluated once -- and chunk threads read their windows from the tile.
-// The tile memory is then reused for the carries and, for x lines, for a
-// transposed coalesced store.
+// variant "tile": the pointwise inputs of TL lines pass through an LDS tile
+// one row block (kCB chunks + halo) at a time with coalesced loads -- every
+// point is evaluated once -- and chunk threads stream their windows from the
+// tile.  The tile memory is then reused for the carries and, for x lines, for
+// a transposed coalesced store.
 // ---------------------------------------------------------------------------
 constexpr int kCB = 8;   // chunks per row block
 
 template <int K>
 __host__ __device__ constexpr int tile_rows() { return kCB * K + 4; }
 
-template <int K, int AX, int TL, int OCC, class SW, bool GCBC>
-__global__ void __launch_bounds__(TL * kMaxP) __attribute__((amdgpu_waves_per_eu(OCC)))
-tile_sweep_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
+template <int K, int AX, int TL, bool EX, class SW, bool GCBC>
+__global__ void __launch_bounds__(TL * kMaxP) tile_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
     constexpr int NF = SW::NF;
-    constexpr int W = K + 4;
     constexpr int TLP = TL + 1;
     constexpr int RB = tile_rows<K>();
     extern __shared__ double lds[];    // tile [NF][RB][TLP]
@@ -591,10 +877,11 @@ tile_sweep_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
     const long line0 = group_of(ax.xcd) * TL;
     const long line = line0 + lloc;
     const bool act = line < ax.nlines;
-    const int s = (int)(((long)c * N) / P);
-    const int L = (int)(((long)(c + 1) * N) / P) - s;
-    const long base = line_base(AX, act ? line : 0, cm.lxi, cm.plane);
-    const long st = ax.st;
+    const int s = EX ? c * K : (int)(((long)c * N) / P);
+    const int L = EX ? K : (int)(((long)(c + 1) * N) / P) - s;
+    const uoff st = (uoff)ax.st * 8u;
+    const uoff lxi = (uoff)cm.lxi, plane = (uoff)cm.plane;
+    const uoff base = line_off(AX, act ? (uoff)line : 0u, lxi, plane);
     const double* tab = ax.tab;
     const int nb = (P + kCB - 1) / kCB;
 
@@ -602,8 +889,8 @@ tile_sweep_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
     double pv[SW::kPv ? NF : 1][K];
     for (int b = 0; b < nb; b++) {
         const int c0 = b * kCB, c1 = min(P, c0 + kCB);
-        const int rs = max(0, (int)(((long)c0 * N) / P) - 2);
-        const int re = min(N, (int)(((long)c1 * N) / P) + 2);
+        const int rs = max(0, (EX ? c0 * K : (int)(((long)c0 * N) / P)) - 2);
+        const int re = min(N, (EX ? c1 * K : (int)(((long)c1 * N) / P)) + 2);
         const int nrows = re - rs;
         if (b > 0) __syncthreads();
         for (int p = tid; p < TL * nrows; p += nthr) {
@@ -611,23 +898,23 @@ tile_sweep_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
             if (AX == 0) { l = p / nrows; row = rs + p % nrows; }
             else         { l = p % TL;    row = rs + p / TL; }
             const long ln = line0 + l;
-            double v[NF];
+            double x[NF];
             if (ln < ax.nlines) {
-                sw.point(line_base(AX, ln, cm.lxi, cm.plane) + (long)row * st, v);
+                sw.point(line_off(AX, (uoff)ln, lxi, plane) + (uoff)row * st, x);
             } else {
 #pragma unroll
-                for (int f = 0; f < NF; f++) v[f] = 0.0;
+                for (int f = 0; f < NF; f++) x[f] = 0.0;
             }
 #pragma unroll
-            for (int f = 0; f < NF; f++) lds[(f * RB + row - rs) * TLP + l] = v[f];
+            for (int f = 0; f < NF; f++) lds[(f * RB + row - rs) * TLP + l] = x[f];
         }
         __syncthreads();
         if (c >= c0 && c < c1) {
-            auto row_of = [&](int u, double* v) {
+            auto row_of = [&](int u, double* x) {
                 const int row = s - 2 + u;
-                const bool in = u < L + 4 && row >= 0 && row < N;
+                const bool in = (EX || u < L + 4) && row >= 0 && row < N;
 #pragma unroll
-                for (int f = 0; f < NF; f++) v[f] = in ? lds[(f * RB + row - rs) * TLP + lloc] : 0.0;
+                for (int f = 0; f < NF; f++) x[f] = in ? lds[(f * RB + row - rs) * TLP + lloc] : 0.0;
             };
             stream_rhs<K, NF>(row_of, r, L, c == 0 && !ax.seam, c == P - 1 && !ax.seam);
             if constexpr (SW::kPv) {
@@ -635,7 +922,8 @@ tile_sweep_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
                 for (int t = 0; t < K; t++) {
                     const int row = s + t;
 #pragma unroll
-                    for (int f = 0; f < NF; f++) pv[f][t] = (t < L) ? lds[(f * RB + row - rs) * TLP + lloc] : 0.0;
+                    for (int f = 0; f < NF; f++)
+                        pv[f][t] = (EX || t < L) ? lds[(f * RB + row - rs) * TLP + lloc] : 0.0;
                 }
             }
         }
@@ -643,47 +931,32 @@ tile_sweep_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
     if (act && ax.seam && (c == 0 || c == P - 1)) seam_rows<K, NF>(sw, base, st, N, L, c == 0, c == P - 1, cm.pb, r);
 
     double carry[NF], G[NF], X[NF];
-    forward_local<K, NF>(tab, N, s, L, r, carry);
+    forward_local<K, NF, EX>(tab, N, s, L, r, carry);
     __syncthreads();                                   // tile reads done: reuse for carries
 #pragma unroll
     for (int f = 0; f < NF; f++) lds[(f * P + c) * TL + lloc] = carry[f];
     __syncthreads();
-    fold_forward<NF>(lds, tab, N, P, TL, c, lloc, G);
-    apply_carry<K, NF>(tab, 3 * N, s, L, r, G);
-    backward_local<K, NF>(tab, N, s, L, r, carry);
-    double* xbeg = lds + NF * P * TL;
+    fold_forward<NF, EX, K>(lds, tab, N, P, TL, c, lloc, G);
+    apply_carry<K, NF, EX>(tab, 3 * N, s, L, r, G);
+    backward_local<K, NF, EX>(tab, N, s, L, r, carry);
+    double* xb = lds + NF * P * TL;
 #pragma unroll
-    for (int f = 0; f < NF; f++) xbeg[(f * P + c) * TL + lloc] = carry[f];
+    for (int f = 0; f < NF; f++) xb[(f * P + c) * TL + lloc] = carry[f];
     __syncthreads();
-    fold_backward<NF>(xbeg, tab, N, P, TL, c, lloc, X);
-    apply_carry<K, NF>(tab, 4 * N, s, L, r, X);
+    fold_backward<NF, EX, K>(xb, tab, N, P, TL, c, lloc, X);
+    apply_carry<K, NF, EX>(tab, 4 * N, s, L, r, X);
 
-    double corr0[NF], corr1[NF];
-    face_corrections<K, NF, SW, GCBC>(sw, base, st, N, L, act && c == 0, act && c == P - 1, r, corr0, corr1);
-    if constexpr (GCBC) {
-        if (act && (c == 0 || c == P - 1)) {
-#pragma unroll
-            for (int t = 0; t < K; t++) {
-                const int row = s + t;
-                if (t < L && (row < kMbci || row >= N - kMbci)) {
-                    const double wgt = cm.sbcc[base + (long)row * st];
-#pragma unroll
-                    for (int f = 0; f < NF; f++) r[f][t] += wgt * (row < kMbci ? corr0[f] : corr1[f]);
-                }
-            }
-        }
-    }
+    if (act) face_corrections<K, NF, SW, GCBC, EX>(sw, base, st, N, s, L, c == 0, c == P - 1, r, cm.sbcc);
 
     if (AX != 0) {
         if (!act) return;
 #pragma unroll
         for (int t = 0; t < K; t++) {
-            if (t < L) {
-                const long idx = base + (long)(s + t) * st;
+            if (EX || t < L) {
                 double d[NF], p[NF];
 #pragma unroll
                 for (int f = 0; f < NF; f++) { d[f] = r[f][t]; p[f] = SW::kPv ? pv[SW::kPv ? f : 0][t] : 0.0; }
-                sw.store(idx, d, p);
+                sw.store(base + (uoff)(s + t) * st, d, p);
             }
         }
         return;
@@ -691,14 +964,14 @@ tile_sweep_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
     // x lines: results back through the tile, one row block at a time, stored along the line
     for (int b = 0; b < nb; b++) {
         const int c0 = b * kCB, c1 = min(P, c0 + kCB);
-        const int rs = (int)(((long)c0 * N) / P);
-        const int re = (int)(((long)c1 * N) / P);
+        const int rs = EX ? c0 * K : (int)(((long)c0 * N) / P);
+        const int re = EX ? c1 * K : (int)(((long)c1 * N) / P);
         const int nrows = re - rs;
         __syncthreads();
         if (c >= c0 && c < c1) {
 #pragma unroll
             for (int t = 0; t < K; t++)
-                if (t < L)
+                if (EX || t < L)
 #pragma unroll
                     for (int f = 0; f < NF; f++) lds[(f * RB + s + t - rs) * TLP + lloc] = r[f][t];
         }
@@ -710,32 +983,67 @@ tile_sweep_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
             double d[NF], pz[NF];
 #pragma unroll
             for (int f = 0; f < NF; f++) { d[f] = lds[(f * RB + row - rs) * TLP + l]; pz[f] = 0.0; }
-            sw.store(line_base(AX, ln, cm.lxi, cm.plane) + (long)row * st, d, pz);
+            sw.store(line_off(AX, (uoff)ln, lxi, plane) + (uoff)row * st, d, pz);
         }
     }
 }
 
+// ---------------------------------------------------------------------------
+// launch
+// ---------------------------------------------------------------------------
 template <int K, int AX, class SW, bool GCBC>
 static int launch_sweep(const AxisDesc& ax, const SweepCommon& cm, const SW& sw, hipStream_t s) {
     constexpr int NF = SW::NF;
-    if constexpr (K == 8) if (ax.variant == 1) {
-        const size_t tile = (size_t)NF * tile_rows<K>() * (ax.tl + 1);
-        const size_t carries = (size_t)2 * NF * ax.P * ax.tl;
-        const size_t lds = sizeof(double) * (tile > carries ? tile : carries);
-        dim3 block(ax.tl * ax.P), grid((unsigned)((ax.nlines + ax.tl - 1) / ax.tl));
-        // occ: minimum waves per SIMD the register allocation must allow
-#define CS_TILE_LAUNCH(TLV, OCCV) \
-    hipLaunchKernelGGL((tile_sweep_kernel<K, AX, TLV, OCCV, SW, GCBC>), grid, block, lds, s, ax, cm, sw)
-        if (ax.tl == 8) {
-            if (ax.occ >= 4) CS_TILE_LAUNCH(8, 4);
-            else if (ax.occ == 2) CS_TILE_LAUNCH(8, 2);
-            else CS_TILE_LAUNCH(8, 1);
-        } else {
-            if (ax.occ >= 4) CS_TILE_LAUNCH(16, 4);
-            else if (ax.occ == 2) CS_TILE_LAUNCH(16, 2);
-            else CS_TILE_LAUNCH(16, 1);
+    const bool exact = ax.N == ax.P * K;
+    if constexpr (K == 8) {
+        if (ax.variant == 1) {
+            const size_t tile = (size_t)NF * tile_rows<K>() * (ax.tl + 1);
+            const size_t carries = (size_t)2 * NF * ax.P * ax.tl;
+            const size_t lds = sizeof(double) * (tile > carries ? tile : carries);
+            dim3 block(ax.tl * ax.P), grid((unsigned)((ax.nlines + ax.tl - 1) / ax.tl));
+            if (exact) {
+                if (ax.tl == 8) hipLaunchKernelGGL((tile_kernel<K, AX, 8, true, SW, GCBC>), grid, block, lds, s, ax, cm, sw);
+                else hipLaunchKernelGGL((tile_kernel<K, AX, 16, true, SW, GCBC>), grid, block, lds, s, ax, cm, sw);
+            } else {
+                if (ax.tl == 8) hipLaunchKernelGGL((tile_kernel<K, AX, 8, false, SW, GCBC>), grid, block, lds, s, ax, cm, sw);
+                else hipLaunchKernelGGL((tile_kernel<K, AX, 16, false, SW, GCBC>), grid, block, lds, s, ax, cm, sw);
+            }
+            CS_HIP(hipGetLastError());
+            return CS_OK;
         }
-#undef CS_TILE_LAUNCH
+    }
+    if constexpr (K == 8 && AX == 0) {
+        if (exact) {
+            const int lpw = 64 / ax.GS;
+            const size_t lds = sizeof(double) * lpw * ax.P * (K + 1);
+            dim3 block(64), grid((unsigned)((ax.nlines + lpw - 1) / lpw));
+            hipLaunchKernelGGL((xt_kernel<K, SW, GCBC>), grid, block, lds, s, ax, cm, sw);
+            CS_HIP(hipGetLastError());
+            return CS_OK;
+        }
+    }
+    if (exact) {
+        dim3 block, grid;
+        size_t lds = 0;
+        const int TL = ax.tl == 8 ? 8 : 16;
+        if (AX == 0) {
+            const int lpb = 4 * (64 / ax.GS);
+            block = dim3(256);
+            grid = dim3((unsigned)((ax.nlines + lpb - 1) / lpb));
+        } else {
+            block = dim3(TL * ax.P);
+            grid = dim3((unsigned)((ax.nlines + TL - 1) / TL));
+            lds = sizeof(double) * 4 * NF * ax.P * TL;
+            if (lds > 65536) {
+                const void* k = TL == 8 ? (const void*)direct_kernel<K, AX, 8, SW, GCBC>
+                                        : (const void*)direct_kernel<K, AX, 16, SW, GCBC>;
+                CS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            }
+        }
+        if (AX != 0 && TL == 8)
+            hipLaunchKernelGGL((direct_kernel<K, AX, 8, SW, GCBC>), grid, block, lds, s, ax, cm, sw);
+        else
+            hipLaunchKernelGGL((direct_kernel<K, AX, 16, SW, GCBC>), grid, block, lds, s, ax, cm, sw);
         CS_HIP(hipGetLastError());
         return CS_OK;
     }
@@ -750,7 +1058,7 @@ static int launch_sweep(const AxisDesc& ax, const SweepCommon& cm, const SW& sw,
         grid = dim3((unsigned)((ax.nlines + kYLines - 1) / kYLines));
         lds = sizeof(double) * 2 * NF * ax.P * kYLines;
     }
-    hipLaunchKernelGGL((sweep_kernel<K, AX, SW, GCBC>), grid, block, lds, s, ax, cm, sw);
+    hipLaunchKernelGGL((ragged_kernel<K, AX, SW, GCBC>), grid, block, lds, s, ax, cm, sw);
     CS_HIP(hipGetLastError());
     return CS_OK;
 }
@@ -842,69 +1150,104 @@ static Phys phys_of(const cs_solver* sv) {
     return ph;
 }
 
-int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, const double* q0, int div,
-               hipStream_t s) {
-    const Phys ph = phys_of(sv);
-    const long lmx = sv->g.lmx;
+static SweepCommon common_of(const cs_solver* sv, const double* sbcc) {
     SweepCommon cm;
     cm.lxi = sv->g.lxi;
     cm.plane = (long)sv->g.lxi * sv->g.let;
     cm.pb = sv->num->pb;
     cm.sbcc = sbcc;
+    return cm;
+}
+
+template <class T, int N>
+static void fields(T* base, long lmx, T* (&out)[N]) {
+    for (int k = 0; k < N; k++) out[k] = base ? base + (long)k * lmx : nullptr;
+}
+
+template <int AX>
+static S1<AX> make_s1(const cs_solver* sv, const double* q) {
+    S1<AX> a{};
+    const long lmx = sv->g.lmx;
+    fields(q, lmx, a.q);
+    if (AX < 2) fields(AX == 0 ? sv->gx : sv->gy, lmx, a.gout);
+    if (AX == 2) {
+        fields((const double*)sv->gx, lmx, a.gx);
+        fields((const double*)sv->gy, lmx, a.gy);
+        fields(sv->tau, lmx, a.tau);
+        fields(sv->hf, lmx, a.hf);
+    }
+    a.ph = phys_of(sv);
+    return a;
+}
+
+template <int AX, int MODE, bool V>
+static S2<AX, MODE, V> make_s2(const cs_solver* sv, const double* q, const double* din, double* dout,
+                               const double* q0, int div) {
+    S2<AX, MODE, V> a{};
+    const long lmx = sv->g.lmx;
+    fields(q, lmx, a.q);
+    if (V) {
+        const int t0 = AX == 0 ? 0 : (AX == 1 ? 3 : 5);
+        const int t1 = AX == 0 ? 3 : (AX == 1 ? 1 : 4);
+        const int t2 = AX == 0 ? 5 : (AX == 1 ? 4 : 2);
+        a.tr[0] = sv->tau + (long)t0 * lmx;
+        a.tr[1] = sv->tau + (long)t1 * lmx;
+        a.tr[2] = sv->tau + (long)t2 * lmx;
+        a.hx = sv->hf + (long)AX * lmx;
+    }
+    fields(din, lmx, a.din);
+    fields(dout, lmx, a.dout);
+    fields(q0, lmx, a.q0);
+    a.dtp = sv->dt;
+    a.div = div;
+    a.ph = phys_of(sv);
+    return a;
+}
+
+int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, const double* q0, int div,
+               hipStream_t s) {
+    const SweepCommon cm = common_of(sv, sbcc);
     int rc;
     const bool visc = sv->cfg.viscous != 0;
     if (visc) {
-        S1<0> a{q, lmx, sv->gx, nullptr, nullptr, nullptr, nullptr, ph};
-        if ((rc = dispatch<0>(sv->ax[0], cm, a, s))) return rc;
-        S1<1> b{q, lmx, sv->gy, nullptr, nullptr, nullptr, nullptr, ph};
-        if ((rc = dispatch<1>(sv->ax[1], cm, b, s))) return rc;
-        S1<2> c{q, lmx, nullptr, sv->gx, sv->gy, sv->tau, sv->hf, ph};
-        if ((rc = dispatch<2>(sv->ax[2], cm, c, s))) return rc;
+        if ((rc = dispatch<0>(sv->ax[0], cm, make_s1<0>(sv, q), s))) return rc;
+        if ((rc = dispatch<1>(sv->ax[1], cm, make_s1<1>(sv, q), s))) return rc;
+        if ((rc = dispatch<2>(sv->ax[2], cm, make_s1<2>(sv, q), s))) return rc;
     }
     auto run = [&](auto vtag) -> int {
         constexpr bool V = decltype(vtag)::value;
-        S2<0, 0, V> x{q, lmx, sv->tau, sv->hf, nullptr, sv->dpart, nullptr, nullptr, 0, ph};
-        int r2 = dispatch<0>(sv->ax[0], cm, x, s);
+        int r2 = dispatch<0>(sv->ax[0], cm, make_s2<0, 0, V>(sv, q, nullptr, sv->dpart, nullptr, 0), s);
         if (r2) return r2;
-        S2<1, 1, V> y{q, lmx, sv->tau, sv->hf, sv->dpart, sv->dpart, nullptr, nullptr, 0, ph};
-        if ((r2 = dispatch<1>(sv->ax[1], cm, y, s))) return r2;
-        if (q0) {
-            S2<2, 3, V> z{q, lmx, sv->tau, sv->hf, sv->dpart, out, q0, sv->dt, div, ph};
-            return dispatch<2>(sv->ax[2], cm, z, s);
-        }
-        S2<2, 2, V> z{q, lmx, sv->tau, sv->hf, sv->dpart, out, nullptr, nullptr, 0, ph};
-        return dispatch<2>(sv->ax[2], cm, z, s);
+        if ((r2 = dispatch<1>(sv->ax[1], cm, make_s2<1, 1, V>(sv, q, sv->dpart, sv->dpart, nullptr, 0), s))) return r2;
+        if (q0) return dispatch<2>(sv->ax[2], cm, make_s2<2, 3, V>(sv, q, sv->dpart, out, q0, div), s);
+        return dispatch<2>(sv->ax[2], cm, make_s2<2, 2, V>(sv, q, sv->dpart, out, nullptr, 0), s);
     };
     if (visc) return run(std::true_type{});
     return run(std::false_type{});
 }
 
-// One sweep of the right-hand side in RHS mode on the solver's scratch buffers
-// (0..5 = S1x, S1y, S1z, S2x, S2y, S2z); for timing single kernels.
+// One sweep of the right-hand side on the solver's scratch buffers (0..5 =
+// S1x, S1y, S1z, S2x, S2y, S2z in RHS mode; 6 = S2z with the RK stage update
+// out = q - dt*J*RHS as the last stage of an RK4 step, dt from the device
+// time-step slot); for timing single kernels.
 int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, double* out, hipStream_t s) {
-    const Phys ph = phys_of(sv);
-    const long lmx = sv->g.lmx;
-    SweepCommon cm;
-    cm.lxi = sv->g.lxi;
-    cm.plane = (long)sv->g.lxi * sv->g.let;
-    cm.pb = sv->num->pb;
-    cm.sbcc = sbcc;
+    const SweepCommon cm = common_of(sv, sbcc);
     const bool visc = sv->cfg.viscous != 0;
+    if (which < 0 || which > 6) return fail(CS_ERR_ARG, "sweep index out of 0..6");
     if (which < 3 && !visc) return fail(CS_ERR_STATE, "gradient sweeps exist only for the viscous RHS");
     switch (which) {
-        case 0: { S1<0> a{q, lmx, sv->gx, nullptr, nullptr, nullptr, nullptr, ph}; return dispatch<0>(sv->ax[0], cm, a, s); }
-        case 1: { S1<1> a{q, lmx, sv->gy, nullptr, nullptr, nullptr, nullptr, ph}; return dispatch<1>(sv->ax[1], cm, a, s); }
-        case 2: { S1<2> a{q, lmx, nullptr, sv->gx, sv->gy, sv->tau, sv->hf, ph}; return dispatch<2>(sv->ax[2], cm, a, s); }
+        case 0: return dispatch<0>(sv->ax[0], cm, make_s1<0>(sv, q), s);
+        case 1: return dispatch<1>(sv->ax[1], cm, make_s1<1>(sv, q), s);
+        case 2: return dispatch<2>(sv->ax[2], cm, make_s1<2>(sv, q), s);
         default: break;
     }
     auto run = [&](auto vtag) -> int {
         constexpr bool V = decltype(vtag)::value;
-        if (which == 3) { S2<0, 0, V> a{q, lmx, sv->tau, sv->hf, nullptr, sv->dpart, nullptr, nullptr, 0, ph}; return dispatch<0>(sv->ax[0], cm, a, s); }
-        if (which == 4) { S2<1, 1, V> a{q, lmx, sv->tau, sv->hf, sv->dpart, sv->dpart, nullptr, nullptr, 0, ph}; return dispatch<1>(sv->ax[1], cm, a, s); }
-        S2<2, 2, V> a{q, lmx, sv->tau, sv->hf, sv->dpart, out, nullptr, nullptr, 0, ph};
-        return dispatch<2>(sv->ax[2], cm, a, s);
+        if (which == 3) return dispatch<0>(sv->ax[0], cm, make_s2<0, 0, V>(sv, q, nullptr, sv->dpart, nullptr, 0), s);
+        if (which == 4) return dispatch<1>(sv->ax[1], cm, make_s2<1, 1, V>(sv, q, sv->dpart, sv->dpart, nullptr, 0), s);
+        if (which == 6) return dispatch<2>(sv->ax[2], cm, make_s2<2, 3, V>(sv, q, sv->dpart, out, q, 1), s);
+        return dispatch<2>(sv->ax[2], cm, make_s2<2, 2, V>(sv, q, sv->dpart, out, nullptr, 0), s);
     };
-    if (which > 5 || which < 0) return fail(CS_ERR_ARG, "sweep index out of 0..5");
     if (visc) return run(std::true_type{});
     return run(std::false_type{});
 }

@@ -20,9 +20,8 @@ struct AxisDesc {
     int seam;            // both faces seam (periodic wrap on one subdomain)
     int gcbc;            // both faces non-reflective characteristic BCs
     // kernel variant (chosen per axis, see cs_solver_tune):
-    int variant;         // 0: direct loads per chunk; 1: LDS row-block tiles
-    int tl;              // lines per workgroup of the tiled variant (8 or 16)
-    int occ;             // tiled variant: minimum waves per SIMD for the register budget (1, 2, 4)
+    int variant;         // 0: direct (ragged when N != P*K); 1: LDS row-block tiles
+    int tl;              // lines per workgroup (y/z lines of variant 0, all of variant 1): 8 or 16
     int xcd;             // 1: neighbouring line groups go to workgroups on the same XCD
 };
 

@@ -91,6 +91,7 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
         if (g.n(a) < 16) return fail(CS_ERR_SHAPE, "fused path needs at least 16 points per axis");
         if (g.n(a) > 512) return fail(CS_ERR_SHAPE, "fused path supports at most 512 points per axis");
     }
+    if (g.lmx > (1L << 29)) return fail(CS_ERR_SHAPE, "fused path: a field must stay below 4 GiB (2^29 points)");
     cs_solver* sv = new cs_solver();
     sv->cfg = *cfg;
     sv->g = g;
@@ -114,7 +115,6 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
         ax.gcbc = gc[a];
         ax.variant = 0;
         ax.tl = 16;
-        ax.occ = 1;
         ax.xcd = 0;
         std::vector<double> h;
         build_axis_table(ax.N, ax.seam, ax.K, ax.P, h);
@@ -137,7 +137,7 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
              hipMalloc(&sv->gy, sizeof(double) * 4 * lmx) == hipSuccess &&
              hipMalloc(&sv->tau, sizeof(double) * 6 * lmx) == hipSuccess &&
              hipMalloc(&sv->hf, sizeof(double) * 3 * lmx) == hipSuccess;
-    if (!ok) {
+    if (!ok || hipMemset(sv->dt, 0, sizeof(double) * 4) != hipSuccess) {
         cs_solver_destroy(sv);
         return fail(CS_ERR_HIP, "allocation of solver buffers failed");
     }
@@ -159,15 +159,14 @@ int cs_solver_sweep(cs_solver* sv, int which, const double* qa, double* de, cons
     return one_sweep(sv, which, qa, sbcc, de, (hipStream_t)stream);
 }
 
-int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int occ, int xcd) {
+int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd) {
     if (!sv || axis < 0 || axis > 2) return fail(CS_ERR_ARG, "bad solver or axis");
-    if (variant < 0 || variant > 1 || (lines != 8 && lines != 16) || (occ != 1 && occ != 2 && occ != 4))
-        return fail(CS_ERR_ARG, "variant 0/1, lines 8/16, occ 1/2/4");
+    if (variant < 0 || variant > 1 || (lines != 8 && lines != 16))
+        return fail(CS_ERR_ARG, "variant 0/1, lines 8/16");
     if (variant == 1 && sv->ax[axis].K != 8) return fail(CS_ERR_UNSUPPORTED, "tiled variant needs 8-point chunks");
     AxisDesc& ax = sv->ax[axis];
     ax.variant = variant;
     ax.tl = lines;
-    ax.occ = occ;
     ax.xcd = xcd ? 1 : 0;
     return CS_OK;
 }

@@ -11,9 +11,13 @@ from . import Dims, Point, SolverConfig, _p, _stream, check, lib
 
 # algorithmic HBM bytes per grid point of each sweep (FP64, viscous RHS):
 # fields read + fields written once each
-SWEEP_NAMES = ["S1x grad", "S1y grad", "S1z grad+stress", "S2x flux", "S2y flux", "S2z flux"]
-SWEEP_BYTES_VISC = [8 * (5 + 4), 8 * (5 + 4), 8 * (5 + 8 + 9), 8 * (5 + 4 + 5), 8 * (5 + 4 + 5 + 5), 8 * (5 + 4 + 5 + 5)]
-SWEEP_BYTES_EULER = [0, 0, 0, 8 * (5 + 5), 8 * (5 + 5 + 5), 8 * (5 + 5 + 5)]
+# (sweep 6 = S2z with the RK stage update, which also reads q0)
+SWEEP_NAMES = ["S1x grad", "S1y grad", "S1z grad+stress", "S2x flux", "S2y flux", "S2z flux", "S2z flux+RK stage"]
+SWEEP_BYTES_VISC = [8 * (5 + 4), 8 * (5 + 4), 8 * (5 + 8 + 9), 8 * (5 + 4 + 5), 8 * (5 + 4 + 5 + 5),
+                    8 * (5 + 4 + 5 + 5), 8 * (5 + 4 + 5 + 5 + 5)]
+SWEEP_BYTES_EULER = [0, 0, 0, 8 * (5 + 5), 8 * (5 + 5 + 5), 8 * (5 + 5 + 5), 8 * (5 + 5 + 5 + 5)]
+# the sweeps of one RK stage, in launch order
+RK_STAGE_SWEEPS = [0, 1, 2, 3, 4, 6]
 
 
 def _uniform(a, name):
@@ -58,6 +62,8 @@ class FusedSolver:
         self.sbcc = torch.as_tensor(np.asarray(state["sbcc"], dtype=np.float64)).cuda().contiguous()
         self.de = torch.zeros(5 * lmx, dtype=torch.float64, device="cuda")
         self.num = None
+        self.nbc_flat = flat
+        self._tuned = {0: (0, 16, 0), 1: (0, 16, 0), 2: (0, 16, 0)}
 
     def close(self):
         if getattr(self, "h", None):
@@ -79,8 +85,34 @@ class FusedSolver:
         de = self.de if de is None else de
         check(lib().cs_solver_sweep(self.h, int(which), _p(q), _p(de), _p(self.sbcc), _stream(stream)))
 
-    def tune(self, axis, variant, lines=16, occ=1, xcd=0):
-        check(lib().cs_solver_tune(self.h, int(axis), int(variant), int(lines), int(occ), int(xcd)))
+    def tune(self, axis, variant, lines=16, xcd=0):
+        check(lib().cs_solver_tune(self.h, int(axis), int(variant), int(lines), int(xcd)))
+        self._tuned[int(axis)] = (int(variant), int(lines), int(xcd))
+
+    def kernel_name(self, which):
+        """Name of the HIP kernel instantiation a sweep launches (as profilers
+        print it, without the argument list)."""
+        ax = 0 if which in (0, 3) else (1 if which in (1, 4) else 2)
+        n = self.dims[ax]
+        K = 8 if n <= 256 else 16
+        P = (n + K - 1) // K
+        exact = n == P * K
+        variant, lines, _ = self._tuned[ax]
+        nbc = self.nbc_flat
+        gcbc = nbc[ax] == 10 and nbc[3 + ax] == 10
+        if which < 3:
+            sw, gc = f"cs::fused::S1<{ax}>", "false"
+        else:
+            mode = {3: 0, 4: 1, 5: 2, 6: 3}[which]
+            sw, gc = f"cs::fused::S2<{ax}, {mode}, {str(self.viscous).lower()}>", str(gcbc).lower()
+        if variant == 1 and K == 8:
+            return f"cs::fused::tile_kernel<{K}, {ax}, {lines}, {str(exact).lower()}, {sw}, {gc}>"
+        if exact and ax == 0 and K == 8:
+            return f"cs::fused::xt_kernel<{K}, {sw}, {gc}>"
+        if exact:
+            tl = 16 if ax == 0 or lines != 8 else 8
+            return f"cs::fused::direct_kernel<{K}, {ax}, {tl}, {sw}, {gc}>"
+        return f"cs::fused::ragged_kernel<{K}, {ax}, {sw}, {gc}>"
 
     def time_step(self, q, stream=None):
         dt = ctypes.c_double()
@@ -93,13 +125,15 @@ class FusedSolver:
                                        ctypes.byref(out) if report_dt else None, _stream(stream)))
         return out.value if report_dt else None
 
-    def time_sweeps(self, q, stream, reps=10):
-        """Average duration (ms) of each sweep, HIP events on the launch stream."""
+    def time_sweeps(self, q, stream, reps=10, sweeps=None):
+        """Average duration (ms) of each sweep, HIP events on the launch stream
+        (default: the sweeps of one RK stage)."""
         import torch
+        self.time_step(q, stream=stream)
         self.rhs(q, stream=stream)
         res = []
         nbytes = SWEEP_BYTES_VISC if self.viscous else SWEEP_BYTES_EULER
-        for w in range(6):
+        for w in (RK_STAGE_SWEEPS if sweeps is None else sweeps):
             if nbytes[w] == 0:
                 continue
             e0 = torch.cuda.Event(enable_timing=True)
@@ -110,15 +144,15 @@ class FusedSolver:
                 self.sweep(w, q, stream=stream)
             e1.record(stream)
             torch.cuda.synchronize()
-            res.append({"name": SWEEP_NAMES[w], "ms": e0.elapsed_time(e1) / reps,
-                        "bytes": float(nbytes[w]) * self.lmx})
+            res.append({"which": w, "name": SWEEP_NAMES[w], "kernel": self.kernel_name(w),
+                        "ms": e0.elapsed_time(e1) / reps, "bytes": float(nbytes[w]) * self.lmx})
         return res
 
     def time_dominant_kernel(self, q, stream, reps=10):
+        """The sweep with the longest launch among those of one RK stage."""
         sw = self.time_sweeps(q, stream, reps)
         dom = max(sw, key=lambda r: r["ms"])
         out = dict(dom)
-        out["name"] = f"cs::fused::sweep_kernel ({dom['name']})"
-        out["all"] = [{"sweep": r["name"], "ms": round(r["ms"], 5),
+        out["all"] = [{"sweep": r["name"], "kernel": r["kernel"], "ms": round(r["ms"], 5),
                        "GBs": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 1)} for r in sw]
         return out

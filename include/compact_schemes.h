@@ -182,18 +182,21 @@ int cs_solver_destroy(cs_solver* sv);
  * non-reflective face). */
 int cs_solver_rhs(cs_solver* sv, const double* qa, double* de, const double* sbcc, cs_stream stream);
 /* one of the six sweeps of cs_solver_rhs (0..5 = gradients x, y, z, flux
- * divergence x, y, z) on the solver's scratch state; no reference
- * counterpart -- it exists to time and profile single kernels.  Sweeps
- * depend on the outputs of the earlier ones of the same RHS. */
+ * divergence x, y, z; 6 = flux divergence z fused with the last RK4 stage
+ * update de = qa - dt*yaco*RHS, dt from the last time step, 0 before any)
+ * on the solver's scratch state; no reference counterpart -- it exists to
+ * time and profile single kernels.  Sweeps depend on the outputs of the
+ * earlier ones of the same RHS. */
 int cs_solver_sweep(cs_solver* sv, int which, const double* qa, double* de, const double* sbcc,
                     cs_stream stream);
 /* kernel variant of the sweeps along `axis` (no reference counterpart; the
  * reference specialises its NVRTC kernels with -D macros instead,
- * src/numerics_rtc.hpp:120-126): variant 0 = per-chunk direct loads, 1 = LDS
- * row-block tiles; lines = 8 or 16 lines per workgroup (tiled); occ = 1, 2
- * or 4 minimum waves per SIMD; xcd = 1 keeps neighbouring line groups on one
- * XCD.  Results do not depend on the choice. */
-int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int occ, int xcd);
+ * src/numerics_rtc.hpp:120-126): variant 0 = each chunk thread evaluates its
+ * own rows (halo rows exchanged between chunks), 1 = LDS row-block tiles
+ * with coalesced loads; lines = 8 or 16 lines per workgroup (y/z lines of
+ * variant 0, all lines of variant 1); xcd = 1 keeps neighbouring line groups
+ * on one XCD.  Results agree to rounding whatever the choice. */
+int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd);
 /* time step of physics::calc_time_step on qa (D9 corrected); dte may be NULL
  * (then nothing synchronises and the value stays on the device). */
 int cs_solver_time_step(cs_solver* sv, const double* qa, double* dte, cs_stream stream);
