@@ -37,8 +37,18 @@ def test_library_exports_every_declared_symbol():
 def test_cpp_header_compiles():
     """The header-only C++ mirror of the reference classes compiles against the C header."""
     src = os.path.join(ROOT, "include", "compact_schemes.hpp")
-    if not os.path.exists(src):
-        pytest.skip("no C++ mirror header")
-    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), "-x", "c++",
-                        src], capture_output=True, text=True)
+    if not os.path.exists("/opt/rocm/include/hip/hip_runtime_api.h"):
+        pytest.skip("HIP headers not installed")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__",
+                        "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include", "-x", "c++", src],
+                       capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_cpp_mirror_programs_build():
+    """The C++ programs written against the mirror classes link the library."""
+    bins = [os.path.join(ROOT, "tests", "cpp", "_bin", n) for n in ("test_numerics_mirror", "test_rhs_mirror")]
+    missing = [b for b in bins if not os.path.exists(b)]
+    assert not missing, f"not built (run __graft_entry__.build()): {missing}"
+    out = subprocess.run(["ldd", bins[0]], capture_output=True, text=True).stdout
+    assert "libcompact_schemes_amd.so" in out and "not found" not in out.split("libcompact_schemes_amd.so")[1].split("\n")[0]

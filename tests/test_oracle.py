@@ -521,3 +521,15 @@ def test_rk_stage_amplification(oracle):
         oracle.rk_stage(n, q, q0, de, yaco, dt / (4 - nk + 1))
     z = np.tile(lam, 5) * dt
     assert rel(q / q0, 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24) < 1e-15
+
+
+def test_interface_table_fixture_reproducible():
+    """The committed fixture is what its generator script computes."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_interface_table", os.path.join(GOLD, "make_interface_table.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    new = mod.build()
+    old = json.load(open(os.path.join(GOLD, "interface_table.json")))
+    for k in ("pbco_shipped", "pbci_shipped", "pbci_fixed_5", "pbci_fixed_7"):
+        assert np.max(np.abs(np.array(old[k]) - np.array(new[k]))) < 1e-15, k
