@@ -196,7 +196,7 @@ def pmc_probe(args):
     launches each, so that per-dispatch counters are easy to attribute."""
     import torch
     from compact_schemes_amd import workloads as wl
-    from compact_schemes_amd.fused import FusedSolver, RK_STAGE_SWEEPS, SWEEP_NAMES
+    from compact_schemes_amd.fused import FusedSolver, RK_STAGE_SWEEPS, sweep_names
     n = args.n
     dims = (n, n, n)
     lmx = n ** 3
@@ -215,7 +215,8 @@ def pmc_probe(args):
     for _ in range(args.reps):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"sweeps": {SWEEP_NAMES[w]: solver.kernel_name(w) for w in RK_STAGE_SWEEPS}, "reps": args.reps,
+    names = sweep_names(solver.order)
+    print(json.dumps({"sweeps": {names[w]: solver.kernel_name(w) for w in RK_STAGE_SWEEPS}, "reps": args.reps,
                       "copy_bytes_read": 8 * 5 * lmx, "copy_bytes_written": 8 * 5 * lmx}), flush=True)
 
 

@@ -35,6 +35,10 @@ EXPORTS = [
     "cs_primitives", "cs_calc_fluxes", "cs_calc_viscous_shear_stress", "cs_calc_time_step",
     "cs_gcbc_go", "cs_wall_condition_update", "cs_rk_stage",
     "cs_solver_create", "cs_solver_destroy", "cs_solver_rhs", "cs_solver_sweep", "cs_solver_tune",
+    "cs_solver_set_order",
+    "cs_batch_create", "cs_batch_destroy", "cs_batch_bind", "cs_batch_deriv", "cs_batch_filter", "cs_batch_step",
+    "cs_batch_capture", "cs_batch_replay", "cs_batch_info", "cs_batch_autotune", "cs_tune_set", "cs_tune_get",
+    "cs_tune_clear", "cs_tune_save", "cs_tune_load",
     "cs_solver_time_step", "cs_solver_rk4_step",
 ]
 
@@ -116,6 +120,22 @@ def lib():
             "cs_solver_rhs": (I, [P, P, P, P, P]),
             "cs_solver_sweep": (I, [P, I, P, P, P, P]),
             "cs_solver_tune": (I, [P, I, I, I, I]),
+            "cs_solver_set_order": (I, [P, I]),
+            "cs_batch_create": (I, [I, P, P, P]),
+            "cs_batch_destroy": (I, [P]),
+            "cs_batch_bind": (I, [P, P, P]),
+            "cs_batch_deriv": (I, [P, P]),
+            "cs_batch_filter": (I, [P, P]),
+            "cs_batch_step": (I, [P, P]),
+            "cs_batch_capture": (I, [P, P]),
+            "cs_batch_replay": (I, [P, P]),
+            "cs_batch_info": (I, [P, P, P]),
+            "cs_batch_autotune": (I, [P, I, P]),
+            "cs_tune_set": (I, [I, I, I]),
+            "cs_tune_get": (I, [I, I, P]),
+            "cs_tune_clear": (I, []),
+            "cs_tune_save": (I, [ctypes.c_char_p]),
+            "cs_tune_load": (I, [ctypes.c_char_p]),
             "cs_solver_time_step": (I, [P, P, DP, P]),
             "cs_solver_rk4_step": (I, [P, P, P, D, DP, P]),
         }

@@ -1,0 +1,367 @@
+// batch.hip -- many independent subdomains through one launch per phase.
+//
+// The plain method of the reference (compact first derivative along x, y, z,
+// deriv1d/deriv2d of src/numerics_pc.hpp:74-168 with the seam wrap of a
+// single periodic rank) followed by the compact filter along x, y, z, applied
+// to a set of subdomains of unrelated shapes.  Every (subdomain, axis) line
+// operator is cut into workgroup-sized items; one persistent launch runs all
+// derivative items of all subdomains (the three derivatives are independent),
+// and one launch per axis runs the filter items (x, then y, then z: each
+// filters the previous result in place).  Items are ordered longest first and
+// dealt round-robin to a grid sized to fill the device, so the uneven
+// subdomains share the CUs instead of leaving them idle at the tail of small
+// launches.
+//
+// An item runs the same device body as the single-subdomain kernels
+// (line_body in line_solver.hip) with the same operator table, so every
+// subdomain's result is bit-identical to cs_deriv / cs_filter on it alone.
+//
+// The four launches can be recorded into a hipGraph (cs_batch_capture) and
+// replayed with one call; nothing in them allocates or clears memory.
+#include "cs_internal.hpp"
+
+#include <algorithm>
+#include <cstdio>
+
+using namespace cs;
+
+namespace {
+
+struct Phase {
+    int op = OP_DERIV;
+    LineArgs* args = nullptr;   // device
+    int* kinds = nullptr;       // device
+    int2* items = nullptr;      // device
+    int nitems = 0;
+    int grid = 0;
+    size_t lds = 0;
+};
+
+int kind_of(int K, int axis) { return 3 * (K == 8 ? 0 : (K == 16 ? 1 : 2)) + axis; }
+
+}  // namespace
+
+struct cs_batch {
+    int count = 0;
+    std::vector<cs_numerics*> num;
+    std::vector<double*> f;     // bound fields
+    std::vector<double*> d;     // bound derivative outputs [3*count]
+    Phase ph[4];                // derivative (all axes), filter x, y, z
+    bool bound = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void free_phase(Phase& p) {
+    if (p.args) (void)hipFree(p.args);
+    if (p.kinds) (void)hipFree(p.kinds);
+    if (p.items) (void)hipFree(p.items);
+    p = Phase();
+}
+
+static void drop_graph(cs_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// line operator `op` along `axis` of subdomain `s`, reading `in`, writing `out`
+static int line_args(cs_numerics* num, int op, int axis, const double* in, double* out, LineArgs* a, int* K) {
+    const int start = num->ndf[0][axis], end = num->ndf[1][axis];
+    if (op == OP_DERIV && (start == 1 || end == 1) && !num->table_set)
+        return fail(CS_ERR_STATE, "seam rows need an interface table");
+    LineOpTable* t = nullptr;
+    int rc = get_line_op(num, op, axis, start, end, &t);
+    if (rc) return rc;
+    *a = LineArgs{};
+    a->in = in;
+    a->out = out;
+    a->tab = t->dev;
+    a->pb = num->pb;
+    a->recv = nullptr;     // seam sums formed inside the kernel (single-subdomain wrap)
+    a->drva = nullptr;
+    a->var = 0;
+    a->N = t->n;
+    a->nlines = (int)num->g.face_size(axis);
+    a->lxi = num->g.lxi;
+    a->plane = (long)num->g.lxi * num->g.let;
+    a->st = num->g.stride(axis);
+    a->P = t->P;
+    a->seam0 = op == OP_DERIV ? start : 0;
+    a->seam1 = op == OP_DERIV ? end : 0;
+    a->cyclic = op == OP_FILTER ? (start == 1 && end == 1) : 0;
+    *K = t->K;
+    return CS_OK;
+}
+
+static int build_phase(cs_batch* b, Phase& p, int op, int axis_lo, int axis_hi) {
+    std::vector<LineArgs> args;
+    std::vector<int> kinds;
+    std::vector<std::pair<double, int2>> work;    // (cost, item)
+    p.op = op;
+    p.lds = 0;
+    for (int s = 0; s < b->count; s++) {
+        for (int axis = axis_lo; axis <= axis_hi; axis++) {
+            LineArgs a;
+            int K = 0;
+            double* out = op == OP_DERIV ? b->d[3 * s + axis] : b->f[s];
+            int rc = line_args(b->num[s], op, axis, b->f[s], out, &a, &K);
+            if (rc) return rc;
+            const int idx = (int)args.size();
+            args.push_back(a);
+            kinds.push_back(kind_of(K, axis));
+            const int TL = batch_lines_per_item(axis);
+            const int nblk = (a.nlines + TL - 1) / TL;
+            const double cost = (double)a.N * std::min(TL, a.nlines);
+            for (int k = 0; k < nblk; k++) work.push_back({cost, make_int2(idx, k)});
+            p.lds = std::max(p.lds, batch_lds_bytes(axis, a.N));
+        }
+    }
+    std::stable_sort(work.begin(), work.end(),
+                     [](const std::pair<double, int2>& x, const std::pair<double, int2>& y) { return x.first > y.first; });
+    std::vector<int2> items(work.size());
+    for (size_t i = 0; i < work.size(); i++) items[i] = work[i].second;
+    p.nitems = (int)items.size();
+
+    int dev = 0, ncu = 0, nb = 1;
+    CS_HIP(hipGetDevice(&dev));
+    CS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    int rc = batch_blocks_per_cu(op, p.lds, &nb);
+    if (rc) return rc;
+    p.grid = std::max(1, std::min(p.nitems, ncu * nb));
+
+    CS_HIP(hipMalloc(&p.args, sizeof(LineArgs) * std::max<size_t>(1, args.size())));
+    CS_HIP(hipMalloc(&p.kinds, sizeof(int) * std::max<size_t>(1, kinds.size())));
+    CS_HIP(hipMalloc(&p.items, sizeof(int2) * std::max<size_t>(1, items.size())));
+    CS_HIP(hipMemcpy(p.args, args.data(), sizeof(LineArgs) * args.size(), hipMemcpyHostToDevice));
+    CS_HIP(hipMemcpy(p.kinds, kinds.data(), sizeof(int) * kinds.size(), hipMemcpyHostToDevice));
+    CS_HIP(hipMemcpy(p.items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
+    return CS_OK;
+}
+
+static int run_phase(const Phase& p, hipStream_t s) {
+    return launch_batch_lines(p.op, p.args, p.kinds, p.items, p.nitems, p.grid, p.lds, s);
+}
+
+extern "C" {
+
+int cs_batch_destroy(cs_batch* b) {
+    if (!b) return CS_OK;
+    drop_graph(b);
+    for (Phase& p : b->ph) free_phase(p);
+    for (cs_numerics* n : b->num) cs_numerics_destroy(n);
+    delete b;
+    return CS_OK;
+}
+
+int cs_batch_create(int count, const cs_dims* dims, const int* nbc6, cs_batch** out) {
+    if (!out || !dims || !nbc6 || count < 1) return fail(CS_ERR_ARG, "null argument or empty batch");
+    *out = nullptr;
+    cs_batch* b = new cs_batch();
+    b->count = count;
+    for (int s = 0; s < count; s++) {
+        cs_numerics* n = nullptr;
+        int rc = cs_numerics_create(&dims[s], nbc6 + 6 * s, &n);
+        if (!rc) rc = cs_deriv_setup(n);
+        if (rc) {
+            std::string msg = "subdomain " + std::to_string(s) + ": " + cs_last_error();
+            if (n) cs_numerics_destroy(n);
+            cs_batch_destroy(b);
+            return fail(rc, msg);
+        }
+        b->num.push_back(n);
+    }
+    *out = b;
+    return CS_OK;
+}
+
+int cs_batch_bind(cs_batch* b, double* const* f, double* const* d) {
+    if (!b || !f || !d) return fail(CS_ERR_ARG, "null argument");
+    drop_graph(b);
+    for (Phase& p : b->ph) free_phase(p);
+    b->f.assign(f, f + b->count);
+    b->d.assign(d, d + 3 * b->count);
+    for (int s = 0; s < b->count; s++)
+        if (!b->f[s] || !b->d[3 * s] || !b->d[3 * s + 1] || !b->d[3 * s + 2])
+            return fail(CS_ERR_ARG, "null field pointer for subdomain " + std::to_string(s));
+    b->bound = false;
+    int rc = build_phase(b, b->ph[0], OP_DERIV, 0, 2);
+    for (int axis = 0; axis < 3 && !rc; axis++) rc = build_phase(b, b->ph[1 + axis], OP_FILTER, axis, axis);
+    if (rc) {
+        for (Phase& p : b->ph) free_phase(p);
+        return rc;
+    }
+    b->bound = true;
+    return CS_OK;
+}
+
+int cs_batch_deriv(cs_batch* b, cs_stream stream) {
+    if (!b || !b->bound) return fail(CS_ERR_STATE, "batch has no bound fields");
+    return run_phase(b->ph[0], (hipStream_t)stream);
+}
+
+int cs_batch_filter(cs_batch* b, cs_stream stream) {
+    if (!b || !b->bound) return fail(CS_ERR_STATE, "batch has no bound fields");
+    for (int axis = 0; axis < 3; axis++) {
+        int rc = run_phase(b->ph[1 + axis], (hipStream_t)stream);
+        if (rc) return rc;
+    }
+    return CS_OK;
+}
+
+int cs_batch_step(cs_batch* b, cs_stream stream) {
+    int rc = cs_batch_deriv(b, stream);
+    if (rc) return rc;
+    return cs_batch_filter(b, stream);
+}
+
+int cs_batch_capture(cs_batch* b, cs_stream stream) {
+    if (!b || !b->bound) return fail(CS_ERR_STATE, "batch has no bound fields");
+    if (!stream) return fail(CS_ERR_ARG, "capture needs a created stream (not the null stream)");
+    drop_graph(b);
+    hipStream_t s = (hipStream_t)stream;
+    CS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = cs_batch_step(b, stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    CS_HIP(e);
+    b->graph = g;
+    CS_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    return CS_OK;
+}
+
+int cs_batch_replay(cs_batch* b, cs_stream stream) {
+    if (!b || !b->exec) return fail(CS_ERR_STATE, "no captured graph (cs_batch_capture)");
+    CS_HIP(hipGraphLaunch(b->exec, (hipStream_t)stream));
+    return CS_OK;
+}
+
+int cs_batch_info(const cs_batch* b, int* items, int* grid) {
+    if (!b || !b->bound) return fail(CS_ERR_STATE, "batch has no bound fields");
+    for (int k = 0; k < 4; k++) {
+        if (items) items[k] = b->ph[k].nitems;
+        if (grid) grid[k] = b->ph[k].grid;
+    }
+    return CS_OK;
+}
+
+// ---- autotuner -------------------------------------------------------------
+
+// Times every admissible chunk length for each (operator, line length) the
+// batch holds, on one of its subdomains, and records the fastest; then
+// rebuilds the batch with the choices.  The choices apply to every operator
+// table built afterwards in this process (cs_deriv / cs_filter on a new
+// context included), so batched and single-subdomain results stay identical.
+int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
+    if (!b || !b->bound) return fail(CS_ERR_STATE, "batch has no bound fields");
+    if (reps < 1) reps = 3;
+    hipStream_t s = (hipStream_t)stream;
+    struct Case { int op, n, s, axis; };
+    std::vector<Case> cases;
+    for (int sd = 0; sd < b->count; sd++)
+        for (int axis = 0; axis < 3; axis++)
+            for (int op : {OP_DERIV, OP_FILTER}) {
+                const int n = b->num[sd]->g.n(axis);
+                bool seen = false;
+                for (auto& c : cases) seen |= (c.op == op && c.n == n);
+                if (!seen) cases.push_back({op, n, sd, axis});
+            }
+    hipEvent_t e0, e1;
+    CS_HIP(hipEventCreate(&e0));
+    CS_HIP(hipEventCreate(&e1));
+    int rc = CS_OK;
+    for (auto& c : cases) {
+        cs_numerics* num = b->num[c.s];
+        const int start = num->ndf[0][c.axis], end = num->ndf[1][c.axis];
+        float best = 1e30f;
+        int bestK = 0;
+        for (int K : {8, 16, 32}) {
+            if ((c.n + K - 1) / K > 16) continue;
+            LineOpTable t;
+            if ((rc = make_line_op(num, c.op, c.n, start, end, K, &t))) break;
+            LineArgs a;
+            int Kd = 0;
+            double* out = c.op == OP_DERIV ? b->d[3 * c.s + c.axis] : b->f[c.s];
+            if ((rc = line_args(num, c.op, c.axis, b->f[c.s], out, &a, &Kd))) { (void)hipFree(t.dev); break; }
+            a.tab = t.dev;
+            a.P = t.P;
+            float ms = 0.f;
+            rc = launch_line(c.op, c.axis, K, a, s);
+            if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = fail(CS_ERR_HIP, "event record failed");
+            for (int r = 0; r < reps && !rc; r++) rc = launch_line(c.op, c.axis, K, a, s);
+            if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = fail(CS_ERR_HIP, "event timing failed");
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(t.dev);
+            if (rc) break;
+            if (ms < best) { best = ms; bestK = K; }
+        }
+        if (rc) break;
+        set_tuned_chunk(c.op, c.n, bestK);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    // drop the cached tables of the batch and rebuild with the choices
+    for (cs_numerics* num : b->num) {
+        for (auto& kv : num->ops) (void)hipFree(kv.second.dev);
+        num->ops.clear();
+    }
+    std::vector<double*> f = b->f, d = b->d;
+    return cs_batch_bind(b, f.data(), d.data());
+}
+
+int cs_tune_set(int op, int n, int K) {
+    if ((op != OP_DERIV && op != OP_FILTER) || n < 8 || (K != 0 && K != 8 && K != 16 && K != 32))
+        return fail(CS_ERR_ARG, "op 0/1, n >= 8, K in {0, 8, 16, 32}");
+    if (K != 0 && (n + K - 1) / K > 16) return fail(CS_ERR_SHAPE, "chunk length too short for this line");
+    set_tuned_chunk(op, n, K);
+    return CS_OK;
+}
+
+int cs_tune_get(int op, int n, int* K) {
+    if (!K) return fail(CS_ERR_ARG, "null argument");
+    *K = tuned_chunk(op, n);
+    return CS_OK;
+}
+
+int cs_tune_clear(void) {
+    clear_tuned_chunks();
+    return CS_OK;
+}
+
+// text file, one line per choice: "<op> <line length> <chunk length>"
+// (op 0 = derivative, 1 = filter; FP64)
+int cs_tune_save(const char* path) {
+    if (!path) return fail(CS_ERR_ARG, "null path");
+    FILE* fp = std::fopen(path, "w");
+    if (!fp) return fail(CS_ERR_ARG, std::string("cannot write ") + path);
+    std::fprintf(fp, "# compact_schemes_amd chunk-length choices: op length K (op 0 deriv, 1 filter; fp64)\n");
+    for (auto& e : tuned_chunks()) std::fprintf(fp, "%d %d %d\n", e[0], e[1], e[2]);
+    std::fclose(fp);
+    return CS_OK;
+}
+
+int cs_tune_load(const char* path) {
+    if (!path) return fail(CS_ERR_ARG, "null path");
+    FILE* fp = std::fopen(path, "r");
+    if (!fp) return fail(CS_ERR_ARG, std::string("cannot read ") + path);
+    char line[256];
+    int rc = CS_OK;
+    while (std::fgets(line, sizeof line, fp)) {
+        if (line[0] == '#') continue;
+        int op, n, K;
+        if (std::sscanf(line, "%d %d %d", &op, &n, &K) != 3) continue;
+        if ((rc = cs_tune_set(op, n, K))) break;
+    }
+    std::fclose(fp);
+    return rc;
+}
+
+}  // extern "C"

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 #include <map>
+#include <array>
 
 #include "../../include/compact_schemes.h"
 
@@ -124,6 +125,20 @@ struct LineArgs {
 
 int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s);
 
+// batched line operators (batch.hip drives them, line_solver.hip launches)
+int batch_lines_per_item(int axis);
+size_t batch_lds_bytes(int axis, int N);
+int batch_blocks_per_cu(int op, size_t lds, int* nb);
+int launch_batch_lines(int op, const LineArgs* args, const int* kinds, const int2* items, int nitems, int grid,
+                       size_t lds, hipStream_t s);
+
+// chunk-length choices per (operator, line length) made by the autotuner;
+// 0 = none (the default rule of choose_chunk applies)
+int tuned_chunk(int op, int n);
+void set_tuned_chunk(int op, int n, int K);
+void clear_tuned_chunks();
+std::vector<std::array<int, 3>> tuned_chunks();
+
 // ---- contexts -----------------------------------------------------------------
 }  // namespace cs
 
@@ -159,6 +174,7 @@ struct cs_physics {
 
 namespace cs {
 int get_line_op(cs_numerics* num, int op, int axis, int start, int end, LineOpTable** t);
+int make_line_op(const cs_numerics* num, int op, int n, int start, int end, int K, LineOpTable* t);
 int line_apply(cs_numerics* num, int op, int axis, const double* in, double* out, int start, int end,
                const double* recv, int var, bool write_drva, hipStream_t s);
 int check_dims(const cs_dims* d, Geo* g);

@@ -59,16 +59,17 @@ __device__ __forceinline__ Prim prim_at(const double* const* q, uoff bo, double*
     return primitive(qq);
 }
 
-// S1: velocity and c^2 gradients along AX
-template <int AX>
+// S1: velocity and c^2 gradients along AX.  The first two S1 sweeps of an
+// RHS store their index-space gradients; the third (CLOSE) combines its own
+// gradients with the stored ones into the stress tensor and heat flux.
+template <int AX, bool CLOSE>
 struct S1 {
     static constexpr int NF = 4;
     static constexpr bool kGcbc = false;   // gradients only; BC updates act on the flux sweeps
-    static constexpr bool kPv = AX == 2;   // the z sweep needs u, v, w, c^2 at its rows
+    static constexpr bool kPv = CLOSE;     // the closure needs u, v, w, c^2 at its rows
     const double* q[5];
-    double* gout[4];          // AX < 2: index-space derivatives of u, v, w, c^2
-    const double* gx[4];      // AX == 2: derivatives from S1x, S1y
-    const double* gy[4];
+    double* gout[4];          // !CLOSE: d/d(axis AX) of u, v, w, c^2
+    const double* gin[3][4];  // CLOSE: stored gradients along the other two axes
     double* tau[6];           // xx, yy, zz, xy, yz, zx
     double* hf[3];
     Phys ph;
@@ -82,23 +83,23 @@ struct S1 {
         v[3] = p.c2;
     }
     __device__ __forceinline__ void store(uoff bo, const double* d, const double* pv) const {
-        if (AX < 2) {
+        if constexpr (!CLOSE) {
 #pragma unroll
             for (int f = 0; f < 4; f++) stg(gout[f], bo, d[f]);
         } else {
-            double a[4], b[4];
+            double gr[3][4];
 #pragma unroll
-            for (int f = 0; f < 4; f++) { a[f] = ldg(gx[f], bo); b[f] = ldg(gy[f], bo); }
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int f = 0; f < 4; f++) gr[a][f] = (a == AX) ? d[f] : ldg(gin[a][f], bo);
             // chain rule of calc_viscous_shear_stress_post_compute_kernel with
             // diagonal metrics (src/cuda/kernels/physics.hpp:267-281)
             double g[9];
 #pragma unroll
-            for (int f = 0; f < 3; f++) {
-                g[3 * f + 0] = ph.m[0] * a[f];
-                g[3 * f + 1] = ph.m[1] * b[f];
-                g[3 * f + 2] = ph.m[2] * d[f];
-            }
-            double c2g[3] = {ph.m[0] * a[3], ph.m[1] * b[3], ph.m[2] * d[3]};
+            for (int f = 0; f < 3; f++)
+#pragma unroll
+                for (int a = 0; a < 3; a++) g[3 * f + a] = ph.m[a] * gr[a][f];
+            double c2g[3] = {ph.m[0] * gr[0][3], ph.m[1] * gr[1][3], ph.m[2] * gr[2][3]};
             double t6[6], h3[3], d4;
             viscous_closure(ph.mu, ph.J, g, c2g, pv, t6, h3, &d4);
 #pragma unroll
@@ -727,7 +728,9 @@ __global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW 
     apply_carry<K, NF, true>(tab, 4 * N, s, K, r, X);
     if (work) face_corrections<K, NF, SW, GCBC, true>(sw, base, st, N, s, K, c == 0, c == P - 1, r, cm.sbcc);
 
-    // back to the coalesced layout, one field at a time
+    // back to the coalesced layout, one field at a time (the point values
+    // xq stay live only for a store that needs them)
+    double xo[NF][J];
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         __syncthreads();
@@ -737,16 +740,16 @@ __global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW 
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < J; j++) xq[f][j] = lds[slot[j]];
+        for (int j = 0; j < J; j++) xo[f][j] = lds[slot[j]];
     }
 #pragma unroll
     for (int j = 0; j < J; j++) {
         const int p = lane + 64 * j;
         if (p < npts) {
-            double d[NF], pz[NF];
+            double d[NF], pv[NF];
 #pragma unroll
-            for (int f = 0; f < NF; f++) { d[f] = xq[f][j]; pz[f] = 0.0; }
-            sw.store(run + (uoff)p * st, d, pz);
+            for (int f = 0; f < NF; f++) { d[f] = xo[f][j]; pv[f] = SW::kPv ? xq[f][j] : 0.0; }
+            sw.store(run + (uoff)p * st, d, pv);
         }
     }
 }
@@ -948,7 +951,7 @@ __global__ void __launch_bounds__(TL * kMaxP) tile_kernel(AxisDesc ax, SweepComm
 
     if (act) face_corrections<K, NF, SW, GCBC, EX>(sw, base, st, N, s, L, c == 0, c == P - 1, r, cm.sbcc);
 
-    if (AX != 0) {
+    if (AX != 0 || SW::kPv) {             // (stores needing the point values stay per thread)
         if (!act) return;
 #pragma unroll
         for (int t = 0; t < K; t++) {
@@ -1164,15 +1167,24 @@ static void fields(T* base, long lmx, T* (&out)[N]) {
     for (int k = 0; k < N; k++) out[k] = base ? base + (long)k * lmx : nullptr;
 }
 
-template <int AX>
-static S1<AX> make_s1(const cs_solver* sv, const double* q) {
-    S1<AX> a{};
+// axes of the S1 sweeps (the third one closes) and of the S2 sweeps (the
+// third one finishes the RHS) for each sweep order
+static const int kS1Axes[2][3] = {{0, 1, 2}, {1, 2, 0}};
+static const int kS2Axes[2][3] = {{0, 1, 2}, {2, 1, 0}};
+
+template <int AX, bool CLOSE>
+static S1<AX, CLOSE> make_s1(const cs_solver* sv, const double* q) {
+    S1<AX, CLOSE> a{};
     const long lmx = sv->g.lmx;
     fields(q, lmx, a.q);
-    if (AX < 2) fields(AX == 0 ? sv->gx : sv->gy, lmx, a.gout);
-    if (AX == 2) {
-        fields((const double*)sv->gx, lmx, a.gx);
-        fields((const double*)sv->gy, lmx, a.gy);
+    const int* axes = kS1Axes[sv->order];
+    if constexpr (!CLOSE) {
+        fields(AX == axes[0] ? sv->ga : sv->gb, lmx, a.gout);
+    } else {
+        for (int a2 = 0; a2 < 3; a2++) {
+            const double* g = a2 == axes[0] ? sv->ga : (a2 == axes[1] ? sv->gb : nullptr);
+            fields(g, lmx, a.gin[a2]);
+        }
         fields(sv->tau, lmx, a.tau);
         fields(sv->hf, lmx, a.hf);
     }
@@ -1204,52 +1216,64 @@ static S2<AX, MODE, V> make_s2(const cs_solver* sv, const double* q, const doubl
     return a;
 }
 
+// sweep `pos` of one RHS (0..2 = S1 sweeps, 3..5 = S2 sweeps, in the
+// solver's order); the last S2 sweep writes the RHS to `out`, or, when q0 is
+// given, the RK stage q0 - dt/div*J*RHS
+template <bool V>
+static int s2_sweep(cs_solver* sv, int k, const SweepCommon& cm, const double* q, double* out, const double* q0,
+                    int div, hipStream_t s) {
+    const int ax = kS2Axes[sv->order][k];
+    if (k == 0) {
+        if (ax == 0) return dispatch<0>(sv->ax[0], cm, make_s2<0, 0, V>(sv, q, nullptr, sv->dpart, nullptr, 0), s);
+        return dispatch<2>(sv->ax[2], cm, make_s2<2, 0, V>(sv, q, nullptr, sv->dpart, nullptr, 0), s);
+    }
+    if (k == 1) return dispatch<1>(sv->ax[1], cm, make_s2<1, 1, V>(sv, q, sv->dpart, sv->dpart, nullptr, 0), s);
+    if (ax == 2) {
+        if (q0) return dispatch<2>(sv->ax[2], cm, make_s2<2, 3, V>(sv, q, sv->dpart, out, q0, div), s);
+        return dispatch<2>(sv->ax[2], cm, make_s2<2, 2, V>(sv, q, sv->dpart, out, nullptr, 0), s);
+    }
+    if (q0) return dispatch<0>(sv->ax[0], cm, make_s2<0, 3, V>(sv, q, sv->dpart, out, q0, div), s);
+    return dispatch<0>(sv->ax[0], cm, make_s2<0, 2, V>(sv, q, sv->dpart, out, nullptr, 0), s);
+}
+
+static int s1_sweep(cs_solver* sv, int k, const SweepCommon& cm, const double* q, hipStream_t s) {
+    if (sv->order == 0) {
+        if (k == 0) return dispatch<0>(sv->ax[0], cm, make_s1<0, false>(sv, q), s);
+        if (k == 1) return dispatch<1>(sv->ax[1], cm, make_s1<1, false>(sv, q), s);
+        return dispatch<2>(sv->ax[2], cm, make_s1<2, true>(sv, q), s);
+    }
+    if (k == 0) return dispatch<1>(sv->ax[1], cm, make_s1<1, false>(sv, q), s);
+    if (k == 1) return dispatch<2>(sv->ax[2], cm, make_s1<2, false>(sv, q), s);
+    return dispatch<0>(sv->ax[0], cm, make_s1<0, true>(sv, q), s);
+}
+
+static int sweep_at(cs_solver* sv, int pos, const SweepCommon& cm, const double* q, double* out, const double* q0,
+                    int div, hipStream_t s) {
+    if (pos < 3) return s1_sweep(sv, pos, cm, q, s);
+    if (sv->cfg.viscous) return s2_sweep<true>(sv, pos - 3, cm, q, out, q0, div, s);
+    return s2_sweep<false>(sv, pos - 3, cm, q, out, q0, div, s);
+}
+
 int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, const double* q0, int div,
                hipStream_t s) {
     const SweepCommon cm = common_of(sv, sbcc);
-    int rc;
-    const bool visc = sv->cfg.viscous != 0;
-    if (visc) {
-        if ((rc = dispatch<0>(sv->ax[0], cm, make_s1<0>(sv, q), s))) return rc;
-        if ((rc = dispatch<1>(sv->ax[1], cm, make_s1<1>(sv, q), s))) return rc;
-        if ((rc = dispatch<2>(sv->ax[2], cm, make_s1<2>(sv, q), s))) return rc;
+    for (int pos = sv->cfg.viscous ? 0 : 3; pos < 6; pos++) {
+        const int rc = sweep_at(sv, pos, cm, q, out, pos == 5 ? q0 : nullptr, div, s);
+        if (rc) return rc;
     }
-    auto run = [&](auto vtag) -> int {
-        constexpr bool V = decltype(vtag)::value;
-        int r2 = dispatch<0>(sv->ax[0], cm, make_s2<0, 0, V>(sv, q, nullptr, sv->dpart, nullptr, 0), s);
-        if (r2) return r2;
-        if ((r2 = dispatch<1>(sv->ax[1], cm, make_s2<1, 1, V>(sv, q, sv->dpart, sv->dpart, nullptr, 0), s))) return r2;
-        if (q0) return dispatch<2>(sv->ax[2], cm, make_s2<2, 3, V>(sv, q, sv->dpart, out, q0, div), s);
-        return dispatch<2>(sv->ax[2], cm, make_s2<2, 2, V>(sv, q, sv->dpart, out, nullptr, 0), s);
-    };
-    if (visc) return run(std::true_type{});
-    return run(std::false_type{});
+    return CS_OK;
 }
 
 // One sweep of the right-hand side on the solver's scratch buffers (0..5 =
-// S1x, S1y, S1z, S2x, S2y, S2z in RHS mode; 6 = S2z with the RK stage update
-// out = q - dt*J*RHS as the last stage of an RK4 step, dt from the device
-// time-step slot); for timing single kernels.
+// the six sweeps in launch order, the last in RHS mode; 6 = the last sweep with
+// the RK stage update out = q - dt*J*RHS of the last RK4 stage, dt from the
+// device time-step slot); for timing single kernels.
 int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, double* out, hipStream_t s) {
     const SweepCommon cm = common_of(sv, sbcc);
-    const bool visc = sv->cfg.viscous != 0;
     if (which < 0 || which > 6) return fail(CS_ERR_ARG, "sweep index out of 0..6");
-    if (which < 3 && !visc) return fail(CS_ERR_STATE, "gradient sweeps exist only for the viscous RHS");
-    switch (which) {
-        case 0: return dispatch<0>(sv->ax[0], cm, make_s1<0>(sv, q), s);
-        case 1: return dispatch<1>(sv->ax[1], cm, make_s1<1>(sv, q), s);
-        case 2: return dispatch<2>(sv->ax[2], cm, make_s1<2>(sv, q), s);
-        default: break;
-    }
-    auto run = [&](auto vtag) -> int {
-        constexpr bool V = decltype(vtag)::value;
-        if (which == 3) return dispatch<0>(sv->ax[0], cm, make_s2<0, 0, V>(sv, q, nullptr, sv->dpart, nullptr, 0), s);
-        if (which == 4) return dispatch<1>(sv->ax[1], cm, make_s2<1, 1, V>(sv, q, sv->dpart, sv->dpart, nullptr, 0), s);
-        if (which == 6) return dispatch<2>(sv->ax[2], cm, make_s2<2, 3, V>(sv, q, sv->dpart, out, q, 1), s);
-        return dispatch<2>(sv->ax[2], cm, make_s2<2, 2, V>(sv, q, sv->dpart, out, nullptr, 0), s);
-    };
-    if (visc) return run(std::true_type{});
-    return run(std::false_type{});
+    if (which < 3 && !sv->cfg.viscous) return fail(CS_ERR_STATE, "gradient sweeps exist only for the viscous RHS");
+    if (which == 6) return sweep_at(sv, 5, cm, q, out, q, 1, s);
+    return sweep_at(sv, which, cm, q, out, nullptr, 0, s);
 }
 
 int time_step(cs_solver* sv, const double* q, double dt_fixed, hipStream_t s) {

@@ -41,8 +41,9 @@ struct cs_solver {
     cs_numerics* num = nullptr;         // interface table, filter operators
     cs::fused::AxisDesc ax[3];
     double* tabs[3] = {nullptr, nullptr, nullptr};
-    double* gx = nullptr;      // [4][lmx]
-    double* gy = nullptr;      // [4][lmx]
+    int order = 1;             // sweep order (cs_solver_set_order)
+    double* ga = nullptr;      // [4][lmx] gradients of the first S1 sweep
+    double* gb = nullptr;      // [4][lmx] gradients of the second S1 sweep
     double* tau = nullptr;     // [6][lmx]
     double* hf = nullptr;      // [3][lmx]
     double* dpart = nullptr;   // [5][lmx]

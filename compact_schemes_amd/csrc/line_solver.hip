@@ -24,12 +24,17 @@
 #include "cs_internal.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <map>
+#include <mutex>
 
 namespace cs {
 
 constexpr int kXLines = 16;   // lines per workgroup for x lines
 constexpr int kMaxP   = 16;   // chunks per line
+constexpr int kBatchYZLines = 32;   // y/z lines per item of the batched launch
+constexpr int kBatchThreads = 512;  // threads of the batched launch (32 lines x 16 chunks)
 
 // lines per workgroup: 16 for x (LDS tile), 64 for y/z, 32 for y/z with
 // 32-point chunks so that the register file holds them (512 threads)
@@ -149,14 +154,16 @@ int build_line_table(int op, int n, int start, int end, double af, int K, std::v
 // ---------------------------------------------------------------------------
 // device
 // ---------------------------------------------------------------------------
-template <int K, int OP, int AX>
-__global__ void __launch_bounds__(max_threads(K)) line_kernel(LineArgs a) {
+// One workgroup's share of a line operator: lines blk*TL .. blk*TL+TL-1.
+// Threads beyond TL*P of the workgroup are inert (their chunk index is past
+// the line) but take part in the barriers.  The per-line arithmetic does not
+// depend on TL, so any TL gives the same results bit for bit.
+template <int K, int OP, int AX, int TL>
+__device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* smem) {
     constexpr int H = (OP == OP_DERIV) ? 2 : 5;     // stencil half-width
     constexpr int W = K + 2 * H;
-    constexpr int TL = lines_per_block(K, AX);
     constexpr int TP = TL + 1;                      // padded LDS row (x tile)
 
-    extern __shared__ double smem[];
     double* gend = smem;                            // [kMaxP][TL]
     double* xbeg = smem + kMaxP * TL;               // [kMaxP][TL]
     double* tile = smem + 2 * kMaxP * TL;           // x only: [N][TP]
@@ -167,7 +174,7 @@ __global__ void __launch_bounds__(max_threads(K)) line_kernel(LineArgs a) {
     if (AX != 0 && TL == 64) c = __builtin_amdgcn_readfirstlane(c);   // one chunk per wave
     const int N = a.N;
     const int P = a.P;
-    const long line0 = (long)blockIdx.x * TL;
+    const long line0 = blk * TL;
     const long line = line0 + lloc;
     const bool active = line < a.nlines;
     const int s = c * K;
@@ -395,6 +402,71 @@ __global__ void __launch_bounds__(max_threads(K)) line_kernel(LineArgs a) {
 }
 
 template <int K, int OP, int AX>
+__global__ void __launch_bounds__(max_threads(K)) line_kernel(LineArgs a) {
+    extern __shared__ double smem[];
+    line_body<K, OP, AX, lines_per_block(K, AX)>(a, blockIdx.x, smem);
+}
+
+// ---------------------------------------------------------------------------
+// many line operators in one persistent launch (batched subdomains): item i
+// runs workgroup-share items[i].blk of operator args[items[i].arg], whose
+// chunk length and axis are kinds[items[i].arg] = 3*log2(K/8) + axis.  y/z
+// lines run 32 lines per item so that every variant fits 512 threads.
+// ---------------------------------------------------------------------------
+template <int OP>
+__global__ void __launch_bounds__(kBatchThreads) batch_line_kernel(const LineArgs* args, const int* kinds,
+                                                                   const int2* items, int nitems) {
+    extern __shared__ double smem[];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int2 w = items[it];
+        const LineArgs a = args[w.x];
+        switch (kinds[w.x]) {
+            case 0: line_body<8, OP, 0, kXLines>(a, w.y, smem); break;
+            case 1: line_body<8, OP, 1, kBatchYZLines>(a, w.y, smem); break;
+            case 2: line_body<8, OP, 2, kBatchYZLines>(a, w.y, smem); break;
+            case 3: line_body<16, OP, 0, kXLines>(a, w.y, smem); break;
+            case 4: line_body<16, OP, 1, kBatchYZLines>(a, w.y, smem); break;
+            case 5: line_body<16, OP, 2, kBatchYZLines>(a, w.y, smem); break;
+            case 6: line_body<32, OP, 0, kXLines>(a, w.y, smem); break;
+            case 7: line_body<32, OP, 1, kBatchYZLines>(a, w.y, smem); break;
+            default: line_body<32, OP, 2, kBatchYZLines>(a, w.y, smem); break;
+        }
+        __syncthreads();   // the LDS of this item is free for the next
+    }
+}
+
+int batch_lines_per_item(int axis) { return axis == 0 ? kXLines : kBatchYZLines; }
+
+size_t batch_lds_bytes(int axis, int N) {
+    const int TL = batch_lines_per_item(axis);
+    size_t lds = sizeof(double) * 2 * kMaxP * TL;
+    if (axis == 0) lds += sizeof(double) * (size_t)N * (TL + 1);
+    return lds;
+}
+
+int batch_blocks_per_cu(int op, size_t lds, int* nb) {
+    const void* k = op == OP_DERIV ? (const void*)batch_line_kernel<OP_DERIV> : (const void*)batch_line_kernel<OP_FILTER>;
+    if (lds > 160 * 1024) return fail(CS_ERR_SHAPE, "batched x lines too long for the LDS tile");
+    if (lds > 64 * 1024) CS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int n = 0;
+    CS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBatchThreads, lds));
+    *nb = n < 1 ? 1 : n;
+    return CS_OK;
+}
+
+int launch_batch_lines(int op, const LineArgs* args, const int* kinds, const int2* items, int nitems, int grid,
+                       size_t lds, hipStream_t s) {
+    if (nitems <= 0) return CS_OK;
+    if (lds > 160 * 1024) return fail(CS_ERR_SHAPE, "batched x lines too long for the LDS tile");
+    if (op == OP_DERIV)
+        hipLaunchKernelGGL(batch_line_kernel<OP_DERIV>, dim3(grid), dim3(kBatchThreads), lds, s, args, kinds, items, nitems);
+    else
+        hipLaunchKernelGGL(batch_line_kernel<OP_FILTER>, dim3(grid), dim3(kBatchThreads), lds, s, args, kinds, items, nitems);
+    CS_HIP(hipGetLastError());
+    return CS_OK;
+}
+
+template <int K, int OP, int AX>
 static int launch_one(const LineArgs& a, hipStream_t s) {
     constexpr int TL = lines_per_block(K, AX);
     dim3 block(TL * a.P);
@@ -436,23 +508,62 @@ int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // context helpers
 // ---------------------------------------------------------------------------
+static std::mutex g_tune_lock;
+static std::map<std::pair<int, int>, int> g_tuned;
+
+int tuned_chunk(int op, int n) {
+    std::lock_guard<std::mutex> g(g_tune_lock);
+    auto it = g_tuned.find({op, n});
+    return it == g_tuned.end() ? 0 : it->second;
+}
+void set_tuned_chunk(int op, int n, int K) {
+    std::lock_guard<std::mutex> g(g_tune_lock);
+    g_tuned[{op, n}] = K;
+}
+void clear_tuned_chunks() {
+    std::lock_guard<std::mutex> g(g_tune_lock);
+    g_tuned.clear();
+}
+std::vector<std::array<int, 3>> tuned_chunks() {
+    std::lock_guard<std::mutex> g(g_tune_lock);
+    std::vector<std::array<int, 3>> v;
+    for (auto& e : g_tuned) v.push_back({e.first.first, e.first.second, e.second});
+    return v;
+}
+
+// Device table of one line operator with chunk length K (0 = the tuned or
+// default choice).
+int make_line_op(const cs_numerics* num, int op, int n, int start, int end, int K, LineOpTable* t) {
+    const int minlen = (op == OP_FILTER && start == 1) ? 11 : 8;
+    if (n < minlen) return fail(CS_ERR_SHAPE, "line length " + std::to_string(n) + " below the stencil size");
+    t->n = n;
+    if (K == 0) K = tuned_chunk(op, n);
+    if (K != 0) {
+        if (K != 8 && K != 16 && K != 32) return fail(CS_ERR_ARG, "chunk length must be 8, 16 or 32");
+        t->K = K;
+        t->P = (n + K - 1) / K;
+        if (t->P > kMaxP) return fail(CS_ERR_SHAPE, "chunk length too short for this line");
+    } else {
+        int rc = choose_chunk(n, &t->K, &t->P);
+        if (rc) return rc;
+    }
+    std::vector<double> h;
+    int rc = build_line_table(op, n, start, end, num->alpha_f, t->K, h);
+    if (rc) return rc;
+    CS_HIP(hipMalloc(&t->dev, h.size() * sizeof(double)));
+    CS_HIP(hipMemcpy(t->dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    t->count = h.size();
+    return CS_OK;
+}
+
 int get_line_op(cs_numerics* num, int op, int axis, int start, int end, LineOpTable** out) {
     const int n = num->g.n(axis);
     LineOpKey key{op, n, start, end};
     auto it = num->ops.find(key);
     if (it != num->ops.end()) { *out = &it->second; return CS_OK; }
-    int minlen = (op == OP_FILTER && start == 1) ? 11 : 8;
-    if (n < minlen) return fail(CS_ERR_SHAPE, "line length " + std::to_string(n) + " below the stencil size");
     LineOpTable t;
-    t.n = n;
-    int rc = choose_chunk(n, &t.K, &t.P);
+    int rc = make_line_op(num, op, n, start, end, 0, &t);
     if (rc) return rc;
-    std::vector<double> h;
-    rc = build_line_table(op, n, start, end, num->alpha_f, t.K, h);
-    if (rc) return rc;
-    CS_HIP(hipMalloc(&t.dev, h.size() * sizeof(double)));
-    CS_HIP(hipMemcpy(t.dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
-    t.count = h.size();
     auto res = num->ops.emplace(key, t);
     *out = &res.first->second;
     return CS_OK;

@@ -69,7 +69,7 @@ extern "C" {
 
 int cs_solver_destroy(cs_solver* sv) {
     if (!sv) return CS_OK;
-    double* bufs[] = {sv->gx, sv->gy, sv->tau, sv->hf, sv->dpart, sv->qb, sv->qc, sv->red, sv->dt,
+    double* bufs[] = {sv->ga, sv->gb, sv->tau, sv->hf, sv->dpart, sv->qb, sv->qc, sv->red, sv->dt,
                       sv->tabs[0], sv->tabs[1], sv->tabs[2]};
     for (double* p : bufs)
         if (p) (void)hipFree(p);
@@ -113,9 +113,11 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
         ax.st = g.stride(a);
         ax.seam = seam[a];
         ax.gcbc = gc[a];
+        // measured defaults (256^3, MI355X): x lines run the one-wave transposing
+        // kernel; y lines 8 per workgroup with XCD-grouped workgroups; z lines 16
         ax.variant = 0;
-        ax.tl = 16;
-        ax.xcd = 0;
+        ax.tl = a == 1 ? 8 : 16;
+        ax.xcd = a == 1 ? 1 : 0;
         std::vector<double> h;
         build_axis_table(ax.N, ax.seam, ax.K, ax.P, h);
         if (hipMalloc(&sv->tabs[a], h.size() * sizeof(double)) != hipSuccess ||
@@ -133,8 +135,8 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
               hipMalloc(&sv->dt, sizeof(double) * 4) == hipSuccess &&
               hipHostMalloc(&sv->dt_host, sizeof(double) * 4) == hipSuccess;
     if (ok && cfg->viscous)
-        ok = hipMalloc(&sv->gx, sizeof(double) * 4 * lmx) == hipSuccess &&
-             hipMalloc(&sv->gy, sizeof(double) * 4 * lmx) == hipSuccess &&
+        ok = hipMalloc(&sv->ga, sizeof(double) * 4 * lmx) == hipSuccess &&
+             hipMalloc(&sv->gb, sizeof(double) * 4 * lmx) == hipSuccess &&
              hipMalloc(&sv->tau, sizeof(double) * 6 * lmx) == hipSuccess &&
              hipMalloc(&sv->hf, sizeof(double) * 3 * lmx) == hipSuccess;
     if (!ok || hipMemset(sv->dt, 0, sizeof(double) * 4) != hipSuccess) {
@@ -168,6 +170,12 @@ int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd) {
     ax.variant = variant;
     ax.tl = lines;
     ax.xcd = xcd ? 1 : 0;
+    return CS_OK;
+}
+
+int cs_solver_set_order(cs_solver* sv, int order) {
+    if (!sv || (order != 0 && order != 1)) return fail(CS_ERR_ARG, "order is 0 or 1");
+    sv->order = order;
     return CS_OK;
 }
 

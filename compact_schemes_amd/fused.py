@@ -11,8 +11,21 @@ from . import Dims, Point, SolverConfig, _p, _stream, check, lib
 
 # algorithmic HBM bytes per grid point of each sweep (FP64, viscous RHS):
 # fields read + fields written once each
-# (sweep 6 = S2z with the RK stage update, which also reads q0)
-SWEEP_NAMES = ["S1x grad", "S1y grad", "S1z grad+stress", "S2x flux", "S2y flux", "S2z flux", "S2z flux+RK stage"]
+# by launch position: three gradient sweeps (the third closes the stress
+# tensor), three flux sweeps (the third writes the RHS); sweep 6 = the last
+# flux sweep with the RK stage update, which also reads q0
+S1_AXES = {0: (0, 1, 2), 1: (1, 2, 0)}
+S2_AXES = {0: (0, 1, 2), 1: (2, 1, 0)}
+
+
+def sweep_names(order):
+    a1, a2 = S1_AXES[order], S2_AXES[order]
+    x = "xyz"
+    return [f"S1{x[a1[0]]} grad", f"S1{x[a1[1]]} grad", f"S1{x[a1[2]]} grad+stress", f"S2{x[a2[0]]} flux",
+            f"S2{x[a2[1]]} flux", f"S2{x[a2[2]]} flux", f"S2{x[a2[2]]} flux+RK stage"]
+
+
+SWEEP_NAMES = sweep_names(1)
 SWEEP_BYTES_VISC = [8 * (5 + 4), 8 * (5 + 4), 8 * (5 + 8 + 9), 8 * (5 + 4 + 5), 8 * (5 + 4 + 5 + 5),
                     8 * (5 + 4 + 5 + 5), 8 * (5 + 4 + 5 + 5 + 5)]
 SWEEP_BYTES_EULER = [0, 0, 0, 8 * (5 + 5), 8 * (5 + 5 + 5), 8 * (5 + 5 + 5), 8 * (5 + 5 + 5 + 5)]
@@ -63,7 +76,8 @@ class FusedSolver:
         self.de = torch.zeros(5 * lmx, dtype=torch.float64, device="cuda")
         self.num = None
         self.nbc_flat = flat
-        self._tuned = {0: (0, 16, 0), 1: (0, 16, 0), 2: (0, 16, 0)}
+        self._tuned = {0: (0, 16, 0), 1: (0, 8, 1), 2: (0, 16, 0)}
+        self.order = 1
 
     def close(self):
         if getattr(self, "h", None):
@@ -89,10 +103,14 @@ class FusedSolver:
         check(lib().cs_solver_tune(self.h, int(axis), int(variant), int(lines), int(xcd)))
         self._tuned[int(axis)] = (int(variant), int(lines), int(xcd))
 
+    def set_order(self, order):
+        check(lib().cs_solver_set_order(self.h, int(order)))
+        self.order = int(order)
+
     def kernel_name(self, which):
         """Name of the HIP kernel instantiation a sweep launches (as profilers
         print it, without the argument list)."""
-        ax = 0 if which in (0, 3) else (1 if which in (1, 4) else 2)
+        ax = S1_AXES[self.order][which] if which < 3 else S2_AXES[self.order][min(which, 5) - 3]
         n = self.dims[ax]
         K = 8 if n <= 256 else 16
         P = (n + K - 1) // K
@@ -101,7 +119,7 @@ class FusedSolver:
         nbc = self.nbc_flat
         gcbc = nbc[ax] == 10 and nbc[3 + ax] == 10
         if which < 3:
-            sw, gc = f"cs::fused::S1<{ax}>", "false"
+            sw, gc = f"cs::fused::S1<{ax}, {str(which == 2).lower()}>", "false"
         else:
             mode = {3: 0, 4: 1, 5: 2, 6: 3}[which]
             sw, gc = f"cs::fused::S2<{ax}, {mode}, {str(self.viscous).lower()}>", str(gcbc).lower()
@@ -144,7 +162,7 @@ class FusedSolver:
                 self.sweep(w, q, stream=stream)
             e1.record(stream)
             torch.cuda.synchronize()
-            res.append({"which": w, "name": SWEEP_NAMES[w], "kernel": self.kernel_name(w),
+            res.append({"which": w, "name": sweep_names(self.order)[w], "kernel": self.kernel_name(w),
                         "ms": e0.elapsed_time(e1) / reps, "bytes": float(nbytes[w]) * self.lmx})
         return res
 

@@ -197,6 +197,12 @@ int cs_solver_sweep(cs_solver* sv, int which, const double* qa, double* de, cons
  * variant 0, all lines of variant 1); xcd = 1 keeps neighbouring line groups
  * on one XCD.  Results agree to rounding whatever the choice. */
 int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd);
+/* order of the six sweeps (no reference counterpart): 0 = gradients along
+ * x, y, then z with the stress closure, fluxes along x, y, then z with the
+ * RHS/stage write; 1 (default) = gradients y, z, x (closure in the x sweep),
+ * fluxes z, y, x (RHS/stage write in the x sweep).  Results agree to
+ * rounding (the three flux divergences are summed in the other order). */
+int cs_solver_set_order(cs_solver* sv, int order);
 /* time step of physics::calc_time_step on qa (D9 corrected); dte may be NULL
  * (then nothing synchronises and the value stays on the device). */
 int cs_solver_time_step(cs_solver* sv, const double* qa, double* dte, cs_stream stream);
@@ -206,6 +212,44 @@ int cs_solver_time_step(cs_solver* sv, const double* qa, double* dte, cs_stream 
  * receives the step used; passing it synchronises the stream. */
 int cs_solver_rk4_step(cs_solver* sv, double* qa, const double* sbcc, double dt, double* dt_out,
                        cs_stream stream);
+
+/* ---- batched subdomains (no reference counterpart) ----------------------
+ * The plain method -- derivative of a field along x, y and z
+ * (numerics_pc::deriv1d, src/numerics_pc.hpp:74-119, seam faces wrapped on
+ * the subdomain itself) and the compact filter along x, y, z -- on many
+ * independent subdomains of unrelated shapes: one persistent launch for all
+ * derivatives, one per axis for the filter.  Each subdomain's results are
+ * bit-identical to cs_deriv / cs_filter on it alone. */
+typedef struct cs_batch cs_batch;
+/* count subdomains; dims[count]; nbc6 = [count][2 face][3 axis] */
+int cs_batch_create(int count, const cs_dims* dims, const int* nbc6, cs_batch** out);
+int cs_batch_destroy(cs_batch* b);
+/* binds the device fields: f[count] (input, filtered in place by the filter
+ * phase) and d[3*count] (d[3*s + axis] = derivative of f[s] along axis).
+ * Builds the work lists; the only call of the batch that uploads. */
+int cs_batch_bind(cs_batch* b, double* const* f, double* const* d);
+int cs_batch_deriv(cs_batch* b, cs_stream stream);      /* one launch  */
+int cs_batch_filter(cs_batch* b, cs_stream stream);     /* three launches, x then y then z */
+int cs_batch_step(cs_batch* b, cs_stream stream);       /* deriv, then filter */
+/* records cs_batch_step into a hipGraph on `stream` (not the null stream);
+ * cs_batch_replay launches it */
+int cs_batch_capture(cs_batch* b, cs_stream stream);
+int cs_batch_replay(cs_batch* b, cs_stream stream);
+/* work items and grid size of the four launches (deriv, filter x, y, z) */
+int cs_batch_info(const cs_batch* b, int* items4, int* grid4);
+/* autotuner: times the admissible chunk lengths (8, 16, 32 points) of every
+ * (operator, line length) of the batch, records the fastest and rebuilds
+ * the batch.  The choices hold for every operator table built afterwards in
+ * the process (the numerics contexts created later included), so batched and
+ * single-subdomain results stay identical. */
+int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream);
+/* the chunk-length choices: op 0 = derivative, 1 = filter; K = 0 clears one */
+int cs_tune_set(int op, int n, int K);
+int cs_tune_get(int op, int n, int* K);
+int cs_tune_clear(void);
+/* persisted as text, one "op length K" line per choice */
+int cs_tune_save(const char* path);
+int cs_tune_load(const char* path);
 
 #ifdef __cplusplus
 }

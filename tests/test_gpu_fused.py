@@ -46,6 +46,30 @@ def test_fused_rhs_matches_oracle(oracle, dims, nbc, visc):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dims,nbc", [((32, 28, 24), CL), ((40, 24, 20), MIX), ((20, 36, 28), PER), ((100, 33, 17), CL)])
+@pytest.mark.parametrize("visc", [True, False], ids=["viscous", "euler"])
+def test_fused_sweep_orders_agree(oracle, dims, nbc, visc):
+    """Both sweep orders (closure and stage write on z, or on x) give the
+    oracle's RHS, and each other's to rounding."""
+    from compact_schemes_amd.fused import FusedSolver
+    st = _state(dims, seed=5)
+    lmx = dims[0] * dims[1] * dims[2]
+    q = dev(st["qa"])
+    res = []
+    for order in (0, 1):
+        sv = FusedSolver(dims, nbc, st, viscous=visc)
+        sv.set_order(order)
+        case(f"fused rhs order={order} dims={dims} nbc={nbc[0]} visc={visc}")
+        res.append(host(sv.rhs(q)).copy())
+    ref = _oracle_rhs(oracle, dims, nbc, st, visc)
+    for m in range(5):
+        sl = slice(m * lmx, (m + 1) * lmx)
+        assert nrel(res[0][sl], ref[sl]) < 1e-12, f"order 0 variable {m}"
+        assert nrel(res[1][sl], ref[sl]) < 1e-12, f"order 1 variable {m}"
+        assert nrel(res[0][sl], res[1][sl]) < 1e-13, f"orders differ, variable {m}"
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("dims,nbc", [((32, 28, 24), CL), ((40, 24, 20), MIX), ((24, 24, 24), PER)])
 def test_fused_rk4_step_matches_oracle(oracle, dims, nbc):
     from compact_schemes_amd.fused import FusedSolver
