@@ -37,7 +37,7 @@ EXPORTS = [
     "cs_solver_create", "cs_solver_destroy", "cs_solver_rhs", "cs_solver_sweep", "cs_solver_tune",
     "cs_solver_set_order",
     "cs_batch_create", "cs_batch_destroy", "cs_batch_bind", "cs_batch_deriv", "cs_batch_filter", "cs_batch_step",
-    "cs_batch_capture", "cs_batch_replay", "cs_batch_info", "cs_batch_autotune", "cs_tune_set", "cs_tune_get",
+    "cs_batch_capture", "cs_batch_replay", "cs_batch_set_persistent", "cs_batch_info", "cs_batch_autotune", "cs_tune_set", "cs_tune_get",
     "cs_tune_clear", "cs_tune_save", "cs_tune_load",
     "cs_solver_time_step", "cs_solver_rk4_step",
 ]
@@ -128,6 +128,7 @@ def lib():
             "cs_batch_filter": (I, [P, P]),
             "cs_batch_step": (I, [P, P]),
             "cs_batch_capture": (I, [P, P]),
+            "cs_batch_set_persistent": (I, [P, I]),
             "cs_batch_replay": (I, [P, P]),
             "cs_batch_info": (I, [P, P, P]),
             "cs_batch_autotune": (I, [P, I, P]),

@@ -54,6 +54,9 @@ class Batch:
     def step(self, stream=None):
         check(lib().cs_batch_step(self.h, _stream(stream)))
 
+    def set_persistent(self, on):
+        check(lib().cs_batch_set_persistent(self.h, int(bool(on))))
+
     def capture(self, stream):
         check(lib().cs_batch_capture(self.h, _stream(stream)))
 
@@ -61,10 +64,11 @@ class Batch:
         check(lib().cs_batch_replay(self.h, _stream(stream)))
 
     def info(self):
+        """(work items, kernel launches) of the phases deriv, filter x, y, z."""
         items = (ctypes.c_int * 4)()
-        grid = (ctypes.c_int * 4)()
-        check(lib().cs_batch_info(self.h, items, grid))
-        return list(items), list(grid)
+        launches = (ctypes.c_int * 4)()
+        check(lib().cs_batch_info(self.h, items, launches))
+        return list(items), list(launches)
 
     def autotune(self, reps=3, stream=None):
         check(lib().cs_batch_autotune(self.h, int(reps), _stream(stream)))

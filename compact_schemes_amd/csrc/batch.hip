@@ -1,22 +1,23 @@
-// batch.hip -- many independent subdomains through one launch per phase.
+// batch.hip -- many independent subdomains through a few persistent launches.
 //
 // The plain method of the reference (compact first derivative along x, y, z,
 // deriv1d/deriv2d of src/numerics_pc.hpp:74-168 with the seam wrap of a
 // single periodic rank) followed by the compact filter along x, y, z, applied
 // to a set of subdomains of unrelated shapes.  Every (subdomain, axis) line
-// operator is cut into workgroup-sized items; one persistent launch runs all
-// derivative items of all subdomains (the three derivatives are independent),
-// and one launch per axis runs the filter items (x, then y, then z: each
-// filters the previous result in place).  Items are ordered longest first and
-// dealt round-robin to a grid sized to fill the device, so the uneven
-// subdomains share the CUs instead of leaving them idle at the tail of small
-// launches.
+// operator is cut into workgroup-sized items, and the items of all
+// subdomains that share a kernel variant (chunk length K, axis) run in one
+// persistent launch: the derivative phase has one launch per variant present
+// (the three derivatives are independent), the filter one per variant and
+// axis, x, then y, then z (each filters the previous result in place).  Items
+// are ordered longest first and dealt round-robin to a grid sized to fill the
+// device, so the uneven subdomains share the CUs instead of each paying the
+// tail of a small launch.
 //
 // An item runs the same device body as the single-subdomain kernels
 // (line_body in line_solver.hip) with the same operator table, so every
 // subdomain's result is bit-identical to cs_deriv / cs_filter on it alone.
 //
-// The four launches can be recorded into a hipGraph (cs_batch_capture) and
+// The launches can be recorded into a hipGraph (cs_batch_capture) and
 // replayed with one call; nothing in them allocates or clears memory.
 #include "cs_internal.hpp"
 
@@ -27,17 +28,20 @@ using namespace cs;
 
 namespace {
 
-struct Phase {
-    int op = OP_DERIV;
-    LineArgs* args = nullptr;   // device
-    int* kinds = nullptr;       // device
-    int2* items = nullptr;      // device
-    int nitems = 0;
-    int grid = 0;
+struct Launch {                // one kernel variant (K, axis) of a phase
+    int K = 8, axis = 0;
+    int first = 0, nitems = 0;  // range in the phase's item list
+    int threads = 0, grid = 0;
     size_t lds = 0;
 };
 
-int kind_of(int K, int axis) { return 3 * (K == 8 ? 0 : (K == 16 ? 1 : 2)) + axis; }
+struct Phase {
+    int op = OP_DERIV;
+    LineArgs* args = nullptr;   // device
+    int2* items = nullptr;      // device
+    int nitems = 0;
+    std::vector<Launch> launches;
+};
 
 }  // namespace
 
@@ -48,13 +52,14 @@ struct cs_batch {
     std::vector<double*> d;     // bound derivative outputs [3*count]
     Phase ph[4];                // derivative (all axes), filter x, y, z
     bool bound = false;
+    int persistent = 0;         // 1: grid = resident workgroups, items round-robin;
+                                // 0: one workgroup per item (the dispatcher balances)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
 
 static void free_phase(Phase& p) {
     if (p.args) (void)hipFree(p.args);
-    if (p.kinds) (void)hipFree(p.kinds);
     if (p.items) (void)hipFree(p.items);
     p = Phase();
 }
@@ -96,11 +101,11 @@ static int line_args(cs_numerics* num, int op, int axis, const double* in, doubl
 }
 
 static int build_phase(cs_batch* b, Phase& p, int op, int axis_lo, int axis_hi) {
+    struct Item { int variant; double cost; int2 it; };
     std::vector<LineArgs> args;
-    std::vector<int> kinds;
-    std::vector<std::pair<double, int2>> work;    // (cost, item)
+    std::vector<Item> work;
+    std::vector<Launch> var;                       // one per (K, axis) present
     p.op = op;
-    p.lds = 0;
     for (int s = 0; s < b->count; s++) {
         for (int axis = axis_lo; axis <= axis_hi; axis++) {
             LineArgs a;
@@ -108,40 +113,63 @@ static int build_phase(cs_batch* b, Phase& p, int op, int axis_lo, int axis_hi) 
             double* out = op == OP_DERIV ? b->d[3 * s + axis] : b->f[s];
             int rc = line_args(b->num[s], op, axis, b->f[s], out, &a, &K);
             if (rc) return rc;
+            int v = -1;
+            for (size_t k = 0; k < var.size(); k++)
+                if (var[k].K == K && var[k].axis == axis) v = (int)k;
+            if (v < 0) {
+                Launch l;
+                l.K = K;
+                l.axis = axis;
+                var.push_back(l);
+                v = (int)var.size() - 1;
+            }
+            const int TL = batch_lines_per_item(K, axis);
+            var[v].threads = std::max(var[v].threads, TL * a.P);
+            var[v].lds = std::max(var[v].lds, batch_lds_bytes(K, axis, a.N));
             const int idx = (int)args.size();
             args.push_back(a);
-            kinds.push_back(kind_of(K, axis));
-            const int TL = batch_lines_per_item(axis);
             const int nblk = (a.nlines + TL - 1) / TL;
             const double cost = (double)a.N * std::min(TL, a.nlines);
-            for (int k = 0; k < nblk; k++) work.push_back({cost, make_int2(idx, k)});
-            p.lds = std::max(p.lds, batch_lds_bytes(axis, a.N));
+            for (int k = 0; k < nblk; k++) work.push_back({v, cost, make_int2(idx, k)});
         }
     }
-    std::stable_sort(work.begin(), work.end(),
-                     [](const std::pair<double, int2>& x, const std::pair<double, int2>& y) { return x.first > y.first; });
+    std::stable_sort(work.begin(), work.end(), [](const Item& x, const Item& y) {
+        return x.variant != y.variant ? x.variant < y.variant : x.cost > y.cost;
+    });
     std::vector<int2> items(work.size());
-    for (size_t i = 0; i < work.size(); i++) items[i] = work[i].second;
+    for (size_t i = 0; i < work.size(); i++) items[i] = work[i].it;
     p.nitems = (int)items.size();
 
-    int dev = 0, ncu = 0, nb = 1;
+    int dev = 0, ncu = 0;
     CS_HIP(hipGetDevice(&dev));
     CS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    int rc = batch_blocks_per_cu(op, p.lds, &nb);
-    if (rc) return rc;
-    p.grid = std::max(1, std::min(p.nitems, ncu * nb));
+    size_t pos = 0;
+    for (size_t v = 0; v < var.size(); v++) {
+        Launch& l = var[v];
+        l.first = (int)pos;
+        while (pos < work.size() && work[pos].variant == (int)v) pos++;
+        l.nitems = (int)pos - l.first;
+        int nb = 1;
+        int rc = batch_blocks_per_cu(op, l.K, l.axis, l.threads, l.lds, &nb);
+        if (rc) return rc;
+        l.grid = std::max(1, std::min(l.nitems, ncu * nb));
+    }
+    p.launches = var;
 
     CS_HIP(hipMalloc(&p.args, sizeof(LineArgs) * std::max<size_t>(1, args.size())));
-    CS_HIP(hipMalloc(&p.kinds, sizeof(int) * std::max<size_t>(1, kinds.size())));
     CS_HIP(hipMalloc(&p.items, sizeof(int2) * std::max<size_t>(1, items.size())));
     CS_HIP(hipMemcpy(p.args, args.data(), sizeof(LineArgs) * args.size(), hipMemcpyHostToDevice));
-    CS_HIP(hipMemcpy(p.kinds, kinds.data(), sizeof(int) * kinds.size(), hipMemcpyHostToDevice));
     CS_HIP(hipMemcpy(p.items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
     return CS_OK;
 }
 
-static int run_phase(const Phase& p, hipStream_t s) {
-    return launch_batch_lines(p.op, p.args, p.kinds, p.items, p.nitems, p.grid, p.lds, s);
+static int run_phase(const Phase& p, int persistent, hipStream_t s) {
+    for (const Launch& l : p.launches) {
+        const int grid = persistent ? l.grid : l.nitems;
+        int rc = launch_batch_lines(p.op, l.K, l.axis, p.args, p.items + l.first, l.nitems, grid, l.threads, l.lds, s);
+        if (rc) return rc;
+    }
+    return CS_OK;
 }
 
 extern "C" {
@@ -198,13 +226,13 @@ int cs_batch_bind(cs_batch* b, double* const* f, double* const* d) {
 
 int cs_batch_deriv(cs_batch* b, cs_stream stream) {
     if (!b || !b->bound) return fail(CS_ERR_STATE, "batch has no bound fields");
-    return run_phase(b->ph[0], (hipStream_t)stream);
+    return run_phase(b->ph[0], b->persistent, (hipStream_t)stream);
 }
 
 int cs_batch_filter(cs_batch* b, cs_stream stream) {
     if (!b || !b->bound) return fail(CS_ERR_STATE, "batch has no bound fields");
     for (int axis = 0; axis < 3; axis++) {
-        int rc = run_phase(b->ph[1 + axis], (hipStream_t)stream);
+        int rc = run_phase(b->ph[1 + axis], b->persistent, (hipStream_t)stream);
         if (rc) return rc;
     }
     return CS_OK;
@@ -214,6 +242,13 @@ int cs_batch_step(cs_batch* b, cs_stream stream) {
     int rc = cs_batch_deriv(b, stream);
     if (rc) return rc;
     return cs_batch_filter(b, stream);
+}
+
+int cs_batch_set_persistent(cs_batch* b, int persistent) {
+    if (!b) return fail(CS_ERR_ARG, "null batch");
+    b->persistent = persistent ? 1 : 0;
+    drop_graph(b);      // a captured graph holds the old grid
+    return CS_OK;
 }
 
 int cs_batch_capture(cs_batch* b, cs_stream stream) {
@@ -241,11 +276,11 @@ int cs_batch_replay(cs_batch* b, cs_stream stream) {
     return CS_OK;
 }
 
-int cs_batch_info(const cs_batch* b, int* items, int* grid) {
+int cs_batch_info(const cs_batch* b, int* items, int* launches) {
     if (!b || !b->bound) return fail(CS_ERR_STATE, "batch has no bound fields");
     for (int k = 0; k < 4; k++) {
         if (items) items[k] = b->ph[k].nitems;
-        if (grid) grid[k] = b->ph[k].grid;
+        if (launches) launches[k] = (int)b->ph[k].launches.size();
     }
     return CS_OK;
 }
@@ -314,7 +349,24 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
         num->ops.clear();
     }
     std::vector<double*> f = b->f, d = b->d;
-    return cs_batch_bind(b, f.data(), d.data());
+    if ((rc = cs_batch_bind(b, f.data(), d.data()))) return rc;
+    // then the grid mode, on the derivative phase (it leaves f unchanged)
+    float t[2] = {0.f, 0.f};
+    CS_HIP(hipEventCreate(&e0));
+    CS_HIP(hipEventCreate(&e1));
+    for (int mode = 0; mode < 2 && !rc; mode++) {
+        rc = run_phase(b->ph[0], mode, s);
+        if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = fail(CS_ERR_HIP, "event record failed");
+        for (int r = 0; r < reps && !rc; r++) rc = run_phase(b->ph[0], mode, s);
+        if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&t[mode], e0, e1) != hipSuccess))
+            rc = fail(CS_ERR_HIP, "event timing failed");
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    b->persistent = t[1] < t[0] ? 1 : 0;
+    return CS_OK;
 }
 
 int cs_tune_set(int op, int n, int K) {

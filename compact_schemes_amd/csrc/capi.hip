@@ -331,7 +331,7 @@ int cs_physics_create(const cs_dims* dims, int viscous, cs_point umf, cs_physics
 int cs_physics_destroy(cs_physics* ph) {
     if (!ph) return CS_OK;
     if (ph->buffer) (void)hipFree(ph->buffer);
-    if (ph->buffer_deriv) hipFree(ph->buffer_deriv);
+    if (ph->buffer_deriv) (void)hipFree(ph->buffer_deriv);
     if (ph->ss) (void)hipFree(ph->ss);
     if (ph->stress) (void)hipFree(ph->stress);
     if (ph->heat) (void)hipFree(ph->heat);

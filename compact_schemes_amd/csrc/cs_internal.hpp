@@ -126,11 +126,11 @@ struct LineArgs {
 int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s);
 
 // batched line operators (batch.hip drives them, line_solver.hip launches)
-int batch_lines_per_item(int axis);
-size_t batch_lds_bytes(int axis, int N);
-int batch_blocks_per_cu(int op, size_t lds, int* nb);
-int launch_batch_lines(int op, const LineArgs* args, const int* kinds, const int2* items, int nitems, int grid,
-                       size_t lds, hipStream_t s);
+int batch_lines_per_item(int K, int axis);
+size_t batch_lds_bytes(int K, int axis, int N);
+int batch_blocks_per_cu(int op, int K, int axis, int threads, size_t lds, int* nb);
+int launch_batch_lines(int op, int K, int axis, const LineArgs* args, const int2* items, int nitems, int grid,
+                       int threads, size_t lds, hipStream_t s);
 
 // chunk-length choices per (operator, line length) made by the autotuner;
 // 0 = none (the default rule of choose_chunk applies)

@@ -33,8 +33,6 @@ namespace cs {
 
 constexpr int kXLines = 16;   // lines per workgroup for x lines
 constexpr int kMaxP   = 16;   // chunks per line
-constexpr int kBatchYZLines = 32;   // y/z lines per item of the batched launch
-constexpr int kBatchThreads = 512;  // threads of the batched launch (32 lines x 16 chunks)
 
 // lines per workgroup: 16 for x (LDS tile), 64 for y/z, 32 for y/z with
 // 32-point chunks so that the register file holds them (512 threads)
@@ -407,65 +405,6 @@ __global__ void __launch_bounds__(max_threads(K)) line_kernel(LineArgs a) {
     line_body<K, OP, AX, lines_per_block(K, AX)>(a, blockIdx.x, smem);
 }
 
-// ---------------------------------------------------------------------------
-// many line operators in one persistent launch (batched subdomains): item i
-// runs workgroup-share items[i].blk of operator args[items[i].arg], whose
-// chunk length and axis are kinds[items[i].arg] = 3*log2(K/8) + axis.  y/z
-// lines run 32 lines per item so that every variant fits 512 threads.
-// ---------------------------------------------------------------------------
-template <int OP>
-__global__ void __launch_bounds__(kBatchThreads) batch_line_kernel(const LineArgs* args, const int* kinds,
-                                                                   const int2* items, int nitems) {
-    extern __shared__ double smem[];
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const int2 w = items[it];
-        const LineArgs a = args[w.x];
-        switch (kinds[w.x]) {
-            case 0: line_body<8, OP, 0, kXLines>(a, w.y, smem); break;
-            case 1: line_body<8, OP, 1, kBatchYZLines>(a, w.y, smem); break;
-            case 2: line_body<8, OP, 2, kBatchYZLines>(a, w.y, smem); break;
-            case 3: line_body<16, OP, 0, kXLines>(a, w.y, smem); break;
-            case 4: line_body<16, OP, 1, kBatchYZLines>(a, w.y, smem); break;
-            case 5: line_body<16, OP, 2, kBatchYZLines>(a, w.y, smem); break;
-            case 6: line_body<32, OP, 0, kXLines>(a, w.y, smem); break;
-            case 7: line_body<32, OP, 1, kBatchYZLines>(a, w.y, smem); break;
-            default: line_body<32, OP, 2, kBatchYZLines>(a, w.y, smem); break;
-        }
-        __syncthreads();   // the LDS of this item is free for the next
-    }
-}
-
-int batch_lines_per_item(int axis) { return axis == 0 ? kXLines : kBatchYZLines; }
-
-size_t batch_lds_bytes(int axis, int N) {
-    const int TL = batch_lines_per_item(axis);
-    size_t lds = sizeof(double) * 2 * kMaxP * TL;
-    if (axis == 0) lds += sizeof(double) * (size_t)N * (TL + 1);
-    return lds;
-}
-
-int batch_blocks_per_cu(int op, size_t lds, int* nb) {
-    const void* k = op == OP_DERIV ? (const void*)batch_line_kernel<OP_DERIV> : (const void*)batch_line_kernel<OP_FILTER>;
-    if (lds > 160 * 1024) return fail(CS_ERR_SHAPE, "batched x lines too long for the LDS tile");
-    if (lds > 64 * 1024) CS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int n = 0;
-    CS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kBatchThreads, lds));
-    *nb = n < 1 ? 1 : n;
-    return CS_OK;
-}
-
-int launch_batch_lines(int op, const LineArgs* args, const int* kinds, const int2* items, int nitems, int grid,
-                       size_t lds, hipStream_t s) {
-    if (nitems <= 0) return CS_OK;
-    if (lds > 160 * 1024) return fail(CS_ERR_SHAPE, "batched x lines too long for the LDS tile");
-    if (op == OP_DERIV)
-        hipLaunchKernelGGL(batch_line_kernel<OP_DERIV>, dim3(grid), dim3(kBatchThreads), lds, s, args, kinds, items, nitems);
-    else
-        hipLaunchKernelGGL(batch_line_kernel<OP_FILTER>, dim3(grid), dim3(kBatchThreads), lds, s, args, kinds, items, nitems);
-    CS_HIP(hipGetLastError());
-    return CS_OK;
-}
-
 template <int K, int OP, int AX>
 static int launch_one(const LineArgs& a, hipStream_t s) {
     constexpr int TL = lines_per_block(K, AX);
@@ -503,6 +442,87 @@ int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s) {
         if (K == 32) return launch_axis<32, OP_FILTER>(axis, a, s);
     }
     return fail(CS_ERR_SHAPE, "unsupported chunk length");
+}
+
+// ---------------------------------------------------------------------------
+// many line operators of one chunk length and axis in one persistent launch
+// (batched subdomains): item i runs workgroup share items[i].y of operator
+// args[items[i].x]; workgroups take items round-robin.  Same body and lines
+// per workgroup as line_kernel, so the same results bit for bit.
+// ---------------------------------------------------------------------------
+template <int K, int OP, int AX>
+__global__ void __launch_bounds__(max_threads(K)) batch_line_kernel(const LineArgs* args, const int2* items,
+                                                                    int nitems) {
+    extern __shared__ double smem[];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int2 w = items[it];
+        line_body<K, OP, AX, lines_per_block(K, AX)>(args[w.x], w.y, smem);
+        __syncthreads();   // the LDS of this item is free for the next
+    }
+}
+
+int batch_lines_per_item(int K, int axis) { return lines_per_block(K, axis); }
+
+size_t batch_lds_bytes(int K, int axis, int N) {
+    const int TL = lines_per_block(K, axis);
+    size_t lds = sizeof(double) * 2 * kMaxP * TL;
+    if (axis == 0) lds += sizeof(double) * (size_t)N * (TL + 1);
+    return lds;
+}
+
+template <int K, int OP>
+static const void* batch_symbol(int axis) {
+    if (axis == 0) return (const void*)batch_line_kernel<K, OP, 0>;
+    if (axis == 1) return (const void*)batch_line_kernel<K, OP, 1>;
+    return (const void*)batch_line_kernel<K, OP, 2>;
+}
+
+static const void* batch_symbol(int op, int K, int axis) {
+    if (op == OP_DERIV) {
+        if (K == 8) return batch_symbol<8, OP_DERIV>(axis);
+        if (K == 16) return batch_symbol<16, OP_DERIV>(axis);
+        return batch_symbol<32, OP_DERIV>(axis);
+    }
+    if (K == 8) return batch_symbol<8, OP_FILTER>(axis);
+    if (K == 16) return batch_symbol<16, OP_FILTER>(axis);
+    return batch_symbol<32, OP_FILTER>(axis);
+}
+
+int batch_blocks_per_cu(int op, int K, int axis, int threads, size_t lds, int* nb) {
+    if (lds > 160 * 1024) return fail(CS_ERR_SHAPE, "batched x lines too long for the LDS tile");
+    const void* k = batch_symbol(op, K, axis);
+    if (lds > 64 * 1024) CS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int n = 0;
+    CS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, lds));
+    *nb = n < 1 ? 1 : n;
+    return CS_OK;
+}
+
+template <int K, int OP>
+static void batch_launch(int axis, dim3 g, dim3 b, size_t lds, hipStream_t s, const LineArgs* args,
+                         const int2* items, int n) {
+    if (axis == 0) hipLaunchKernelGGL((batch_line_kernel<K, OP, 0>), g, b, lds, s, args, items, n);
+    else if (axis == 1) hipLaunchKernelGGL((batch_line_kernel<K, OP, 1>), g, b, lds, s, args, items, n);
+    else hipLaunchKernelGGL((batch_line_kernel<K, OP, 2>), g, b, lds, s, args, items, n);
+}
+
+int launch_batch_lines(int op, int K, int axis, const LineArgs* args, const int2* items, int nitems, int grid,
+                       int threads, size_t lds, hipStream_t s) {
+    if (nitems <= 0) return CS_OK;
+    if (lds > 160 * 1024) return fail(CS_ERR_SHAPE, "batched x lines too long for the LDS tile");
+    if (threads > max_threads(K)) return fail(CS_ERR_SHAPE, "too many chunks for the batched launch");
+    const dim3 g(grid), b(threads);
+    if (op == OP_DERIV) {
+        if (K == 8) batch_launch<8, OP_DERIV>(axis, g, b, lds, s, args, items, nitems);
+        else if (K == 16) batch_launch<16, OP_DERIV>(axis, g, b, lds, s, args, items, nitems);
+        else batch_launch<32, OP_DERIV>(axis, g, b, lds, s, args, items, nitems);
+    } else {
+        if (K == 8) batch_launch<8, OP_FILTER>(axis, g, b, lds, s, args, items, nitems);
+        else if (K == 16) batch_launch<16, OP_FILTER>(axis, g, b, lds, s, args, items, nitems);
+        else batch_launch<32, OP_FILTER>(axis, g, b, lds, s, args, items, nitems);
+    }
+    CS_HIP(hipGetLastError());
+    return CS_OK;
 }
 
 // ---------------------------------------------------------------------------

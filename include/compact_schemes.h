@@ -217,9 +217,10 @@ int cs_solver_rk4_step(cs_solver* sv, double* qa, const double* sbcc, double dt,
  * The plain method -- derivative of a field along x, y and z
  * (numerics_pc::deriv1d, src/numerics_pc.hpp:74-119, seam faces wrapped on
  * the subdomain itself) and the compact filter along x, y, z -- on many
- * independent subdomains of unrelated shapes: one persistent launch for all
- * derivatives, one per axis for the filter.  Each subdomain's results are
- * bit-identical to cs_deriv / cs_filter on it alone. */
+ * independent subdomains of unrelated shapes: one persistent launch per kernel
+ * variant (chunk length, axis) serves the items of all subdomains -- at most
+ * 9 launches for the three derivatives, 3 per filter axis.  Each subdomain's
+ * results are bit-identical to cs_deriv / cs_filter on it alone. */
 typedef struct cs_batch cs_batch;
 /* count subdomains; dims[count]; nbc6 = [count][2 face][3 axis] */
 int cs_batch_create(int count, const cs_dims* dims, const int* nbc6, cs_batch** out);
@@ -228,18 +229,22 @@ int cs_batch_destroy(cs_batch* b);
  * phase) and d[3*count] (d[3*s + axis] = derivative of f[s] along axis).
  * Builds the work lists; the only call of the batch that uploads. */
 int cs_batch_bind(cs_batch* b, double* const* f, double* const* d);
-int cs_batch_deriv(cs_batch* b, cs_stream stream);      /* one launch  */
-int cs_batch_filter(cs_batch* b, cs_stream stream);     /* three launches, x then y then z */
+int cs_batch_deriv(cs_batch* b, cs_stream stream);      /* derivatives along x, y, z */
+int cs_batch_filter(cs_batch* b, cs_stream stream);     /* filter x, then y, then z, in place */
 int cs_batch_step(cs_batch* b, cs_stream stream);       /* deriv, then filter */
+/* grid of the batched launches: 0 (default) = one workgroup per work item,
+ * the hardware dispatcher balances; 1 = persistent, as many workgroups as
+ * fit on the device at once, items dealt round-robin.  Same results. */
+int cs_batch_set_persistent(cs_batch* b, int persistent);
 /* records cs_batch_step into a hipGraph on `stream` (not the null stream);
  * cs_batch_replay launches it */
 int cs_batch_capture(cs_batch* b, cs_stream stream);
 int cs_batch_replay(cs_batch* b, cs_stream stream);
-/* work items and grid size of the four launches (deriv, filter x, y, z) */
-int cs_batch_info(const cs_batch* b, int* items4, int* grid4);
+/* work items and kernel launches of the four phases (deriv, filter x, y, z) */
+int cs_batch_info(const cs_batch* b, int* items4, int* launches4);
 /* autotuner: times the admissible chunk lengths (8, 16, 32 points) of every
- * (operator, line length) of the batch, records the fastest and rebuilds
- * the batch.  The choices hold for every operator table built afterwards in
+ * (operator, line length) of the batch, records the fastest, rebuilds the
+ * batch and picks the faster grid mode (cs_batch_set_persistent).  The choices hold for every operator table built afterwards in
  * the process (the numerics contexts created later included), so batched and
  * single-subdomain results stay identical. */
 int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream);

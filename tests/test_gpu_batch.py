@@ -66,8 +66,8 @@ def test_batch_bit_identical_to_single_subdomain(bcs):
     fields = _fields(dims, 100)
     case(f"batch {len(dims)} subdomains bcs={bcs}")
     b, f, d = _batch(dims, nbc, fields)
-    items, grid = b.info()
-    assert min(items) > 0 and min(grid) > 0
+    items, launches = b.info()
+    assert min(items) > 0 and min(launches) > 0
     b.step()
     torch.cuda.synchronize()
     d_ref, f_ref = _unbatched(dims, nbc, fields)
@@ -75,6 +75,26 @@ def test_batch_bit_identical_to_single_subdomain(bcs):
         for axis in range(3):
             assert torch.equal(d[3 * s + axis], d_ref[3 * s + axis]), f"derivative s={s} axis={axis} dims={dims[s]}"
         assert torch.equal(f[s], f_ref[s]), f"filter s={s} dims={dims[s]}"
+
+
+@pytest.mark.gpu
+def test_batch_grid_modes_agree():
+    """Persistent round-robin grid and one workgroup per item: same bits."""
+    import torch
+    dims = _dims(6, 6)
+    nbc = [PER] * len(dims)
+    fields = _fields(dims, 400)
+    b1, f1, d1 = _batch(dims, nbc, fields)
+    b2, f2, d2 = _batch(dims, nbc, fields)
+    b2.set_persistent(1)
+    case("batch grid modes")
+    b1.step()
+    b2.step()
+    torch.cuda.synchronize()
+    for s in range(len(dims)):
+        assert torch.equal(f1[s], f2[s])
+        for axis in range(3):
+            assert torch.equal(d1[3 * s + axis], d2[3 * s + axis])
 
 
 @pytest.mark.gpu
