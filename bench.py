@@ -38,9 +38,10 @@ def parse():
                     help="only launch the dominant sweep and a plain device copy --reps times each "
                          "(the program rocprofv3 --pmc runs for the traffic figures)")
     ap.add_argument("--reps", type=int, default=5)
-    ap.add_argument("--workload", default="rhs", choices=["rhs", "batch96"],
+    ap.add_argument("--workload", default="rhs", choices=["rhs", "batch96", "precision"],
                     help="rhs: the 256^3 viscous RK4 step (default); batch96: derivative x/y/z + filter "
-                         "x/y/z of 96 periodic subdomains with seeded sizes in [48, 320]")
+                         "x/y/z of 96 periodic subdomains with seeded sizes in [48, 320]; precision: 50 RK4 "
+                         "steps at 512x512x256 with FP64 and with FP32+refinement line solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-reps", type=int, default=3)
     return ap.parse_args()
@@ -79,6 +80,8 @@ def main():
         return pmc_probe(args)
     if args.workload == "batch96":
         return batch_bench(args)
+    if args.workload == "precision":
+        return precision_bench(args)
     n = args.n
     dims = (n, n, n)
     lmx = n ** 3
@@ -276,6 +279,71 @@ def batch_bench(args):
                    "one_subdomain_at_a_time_ms": round(ms_single, 3),
                    "items": items, "launches": launches,
                    "algorithmic_GBs": round(npts * 96 / (ms_graph * 1e-3) / 1e9, 1)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def precision_bench(args, dims=(512, 512, 256), nsteps=50):
+    """Config 4: the flow of the RHS workload on 512x512x256 with periodic
+    faces, 50 RK4 steps (reference-ordered operators, viscous, filter every
+    step, dt fixed from the CFL limit of the initial state) once with FP64
+    line solves and once with FP32 solves + one FP64-residual refinement
+    sweep (fluxes, boundary rows, stage update in FP64).  Reports the error
+    of the mixed run against the FP64 run per conserved variable, relative
+    to the state and to its change over the run."""
+    import numpy as np
+    import torch
+    from compact_schemes_amd import workloads as wl
+    from compact_schemes_amd.stepper import Stepper
+    per = wl.NBC_PERIODIC
+    st = wl.flow_state(dims, seed=4)
+    lmx = dims[0] * dims[1] * dims[2]
+    q0 = torch.from_numpy(st["qa"]).cuda()
+    del st["qa"]
+    res, times = {}, {}
+    dt = None
+    for mode in (0, 1):
+        stepper = Stepper(dims, per, True, st["umf"], st)
+        stepper.num.set_precision(mode)
+        q = q0.clone()
+        if dt is None:
+            dt = stepper.time_step(q, 0.95)
+        stepper.rk4_step(q, dt=dt)                 # warm-up step, then restart
+        q.copy_(q0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(nsteps):
+            stepper.rk4_step(q, dt=dt)
+            if (k + 1) % 10 == 0:
+                torch.cuda.synchronize()
+                print(f"[precision] mode {mode}: {k + 1} steps", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        times[mode] = (time.perf_counter() - t0) / nsteps
+        res[mode] = q
+        del stepper
+        torch.cuda.empty_cache()
+    err = []
+    for v in range(5):
+        sl = slice(v * lmx, (v + 1) * lmx)
+        a, b, c0 = res[1][sl], res[0][sl], q0[sl]
+        d = (a - b)
+        ch = (b - c0)
+        err.append({"var": ["rho", "rho_u", "rho_v", "rho_w", "E"][v],
+                    "max_rel_state": float(d.abs().max() / b.abs().max()),
+                    "l2_rel_state": float(d.norm() / b.norm()),
+                    "max_rel_change": float(d.abs().max() / ch.abs().max()),
+                    "l2_rel_change": float(d.norm() / ch.norm())})
+    worst = max(e["max_rel_state"] for e in err)
+    line = {
+        "metric": "FP32+refinement vs FP64: max relative error after 50 RK4 steps",
+        "value": worst, "unit": "relative", "n_gpus": 1, "steps": nsteps, "warmup": 1,
+        "ms_per_step": round(1e3 * times[1], 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 solves + f64 refinement",
+        "data": "synthetic (seeded acoustic pulse on uniform Mach 0.5 flow, periodic faces)",
+        "config": {"workload": "viscous RHS, 512x512x256, periodic, 50 RK4 steps with GV10 filter, "
+                               "reference-ordered operators", "dt": dt,
+                   "ms_per_step_fp64": round(1e3 * times[0], 3), "ms_per_step_mixed": round(1e3 * times[1], 3),
+                   "errors": err},
     }
     print(json.dumps(line), flush=True)
 

@@ -34,7 +34,7 @@ EXPORTS = [
     "cs_physics_create", "cs_physics_destroy", "cs_physics_set_frame", "cs_physics_buffers",
     "cs_primitives", "cs_calc_fluxes", "cs_calc_viscous_shear_stress", "cs_calc_time_step",
     "cs_gcbc_go", "cs_wall_condition_update", "cs_rk_stage",
-    "cs_solver_create", "cs_solver_destroy", "cs_solver_rhs", "cs_solver_sweep", "cs_solver_tune",
+    "cs_set_precision", "cs_solver_create", "cs_solver_destroy", "cs_solver_rhs", "cs_solver_sweep", "cs_solver_tune",
     "cs_solver_set_order",
     "cs_batch_create", "cs_batch_destroy", "cs_batch_bind", "cs_batch_deriv", "cs_batch_filter", "cs_batch_step",
     "cs_batch_capture", "cs_batch_replay", "cs_batch_set_persistent", "cs_batch_info", "cs_batch_autotune", "cs_tune_set", "cs_tune_get",
@@ -104,6 +104,7 @@ def lib():
             "cs_deriv": (I, [P, I, P, P, I, P]),
             "cs_filter": (I, [P, I, P, P, P]),
             "cs_set_filter_alpha": (I, [P, D]),
+            "cs_set_precision": (I, [P, I]),
             "cs_physics_create": (I, [ctypes.POINTER(Dims), I, Point, PP]),
             "cs_physics_destroy": (I, [P]),
             "cs_physics_set_frame": (I, [P, Point, Point]),
@@ -196,6 +197,10 @@ class Numerics:
             self.close()
         except Exception:
             pass
+
+    def set_precision(self, mode):
+        """0: FP64 line solves; 1: FP32 solve + one FP64-residual refinement sweep."""
+        check(lib().cs_set_precision(self.h, int(mode)))
 
     def deriv_setup(self):
         check(lib().cs_deriv_setup(self.h))

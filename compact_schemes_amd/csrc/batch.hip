@@ -96,6 +96,8 @@ static int line_args(cs_numerics* num, int op, int axis, const double* in, doubl
     a->seam0 = op == OP_DERIV ? start : 0;
     a->seam1 = op == OP_DERIV ? end : 0;
     a->cyclic = op == OP_FILTER ? (start == 1 && end == 1) : 0;
+    a->tab32 = t->dev32;
+    a->mixed = 0;              // batched operators solve in FP64
     *K = t->K;
     return CS_OK;
 }
@@ -322,7 +324,7 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
             LineArgs a;
             int Kd = 0;
             double* out = c.op == OP_DERIV ? b->d[3 * c.s + c.axis] : b->f[c.s];
-            if ((rc = line_args(num, c.op, c.axis, b->f[c.s], out, &a, &Kd))) { (void)hipFree(t.dev); break; }
+            if ((rc = line_args(num, c.op, c.axis, b->f[c.s], out, &a, &Kd))) { free_line_op(t); break; }
             a.tab = t.dev;
             a.P = t.P;
             float ms = 0.f;
@@ -333,7 +335,7 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
                 rc = fail(CS_ERR_HIP, "event timing failed");
             (void)hipStreamSynchronize(s);
-            (void)hipFree(t.dev);
+            free_line_op(t);
             if (rc) break;
             if (ms < best) { best = ms; bestK = K; }
         }
@@ -345,7 +347,7 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
     if (rc) return rc;
     // drop the cached tables of the batch and rebuild with the choices
     for (cs_numerics* num : b->num) {
-        for (auto& kv : num->ops) (void)hipFree(kv.second.dev);
+        for (auto& kv : num->ops) free_line_op(kv.second);
         num->ops.clear();
     }
     std::vector<double*> f = b->f, d = b->d;

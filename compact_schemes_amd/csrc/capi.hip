@@ -167,7 +167,7 @@ int cs_numerics_destroy(cs_numerics* num) {
         if (num->recv[a]) (void)hipFree(num->recv[a]);
     }
     if (num->pb) (void)hipFree(num->pb);
-    for (auto& kv : num->ops) (void)hipFree(kv.second.dev);
+    for (auto& kv : num->ops) free_line_op(kv.second);
     delete num;
     return CS_OK;
 }
@@ -215,9 +215,16 @@ int cs_set_filter_alpha(cs_numerics* num, double af) {
     if (!(af > -0.5 && af < 0.5)) return fail(CS_ERR_ARG, "filter alpha must lie in (-0.5, 0.5)");
     num->alpha_f = af;
     for (auto it = num->ops.begin(); it != num->ops.end();) {
-        if (it->first.op == OP_FILTER) { (void)hipFree(it->second.dev); it = num->ops.erase(it); }
+        if (it->first.op == OP_FILTER) { free_line_op(it->second); it = num->ops.erase(it); }
         else ++it;
     }
+    return CS_OK;
+}
+
+int cs_set_precision(cs_numerics* num, int mode) {
+    if (!num) return fail(CS_ERR_ARG, "null context");
+    if (mode != 0 && mode != 1) return fail(CS_ERR_ARG, "precision mode is 0 (FP64) or 1 (FP32 + FP64 refinement)");
+    num->precision = mode;
     return CS_OK;
 }
 

@@ -96,9 +96,17 @@ struct LineOpKey {
 
 struct LineOpTable {
     int n = 0, K = 0, P = 0;
-    double* dev = nullptr;
+    double* dev = nullptr;     // FP64 table (build_line_table)
+    float* dev32 = nullptr;    // FP32 factorisation for the mixed-precision solve
     size_t count = 0;
 };
+
+inline void free_line_op(LineOpTable& t) {
+    if (t.dev) (void)hipFree(t.dev);
+    if (t.dev32) (void)hipFree(t.dev32);
+    t.dev = nullptr;
+    t.dev32 = nullptr;
+}
 
 int choose_chunk(int n, int* K, int* P);
 int build_line_table(int op, int n, int start, int end, double alpha_f, int K,
@@ -121,6 +129,8 @@ struct LineArgs {
     int P;
     int seam0, seam1;     // deriv: seam rows at the start / end of the line
     int cyclic;           // filter: cyclic system
+    const float* tab32;   // FP32 factorisation (mixed precision)
+    int mixed;            // 1: FP32 solve + one FP64-residual refinement sweep
 };
 
 int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s);
@@ -154,6 +164,7 @@ struct cs_numerics {
     double pb_host[32];
     bool table_set = false;
     double alpha_f = 0.49;
+    int precision = 0;           // 0: FP64 line solves; 1: FP32 solves + FP64 residual refinement
     std::map<cs::LineOpKey, cs::LineOpTable> ops;
 };
 

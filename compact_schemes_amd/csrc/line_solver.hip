@@ -97,7 +97,7 @@ static void thomas_lu(int n, const std::vector<double>& a, const std::vector<dou
 }
 
 int build_line_table(int op, int n, int start, int end, double af, int K, std::vector<double>& h) {
-    h.assign(6 * (size_t)n + 8 + 36, 0.0);
+    h.assign(6 * (size_t)n + 44 + 3 * (size_t)n, 0.0);
     std::vector<double> a(n), b(n, 1.0), c(n);
     bool cyclic = false;
     if (op == OP_DERIV) {
@@ -128,6 +128,21 @@ int build_line_table(int op, int n, int start, int end, double af, int K, std::v
     double* Mf = &h[3 * (size_t)n];
     double* Rb = &h[4 * (size_t)n];
     double* z = &h[5 * (size_t)n];
+    // the system matrix itself (rows a x[j-1] + b x[j] + c x[j+1]; for the
+    // cyclic filter a[0] couples to x[n-1] and c[n-1] to x[0]), for the
+    // residual of the mixed-precision solve
+    double* A = &h[6 * (size_t)n + 44];
+    for (int i = 0; i < n; i++) {
+        A[i] = a[i];
+        A[n + i] = b[i];
+        A[2 * n + i] = c[i];
+    }
+    if (cyclic) {
+        A[0] = af;
+        A[n] = 1.0;
+        A[2 * n - 1] = 1.0;
+        A[3 * n - 1] = af;
+    }
     thomas_lu(n, a, b, c, nu, mu, rho);
     for (int s = 0; s < n; s += K) {
         int e = std::min(s + K, n) - 1;
@@ -152,11 +167,73 @@ int build_line_table(int op, int n, int start, int end, double af, int K, std::v
 // ---------------------------------------------------------------------------
 // device
 // ---------------------------------------------------------------------------
+// Chunked solve of one line system in arithmetic T with tables in T (layout
+// of build_line_table): forward recurrence with zero carry-in, carries across
+// chunks through LDS (gend/xbeg, [kMaxP][TL] entries of T), backward
+// recurrence, carries, and the Sherman-Morrison correction of a cyclic
+// system.  Two workgroup barriers.
+template <int K, typename T>
+__device__ __forceinline__ void chunk_solve(T (&r)[K], const T* tab, int N, int P, int s, int c, bool cvalid,
+                                            bool cyclic, int TL, int lloc, T* gend, T* xbeg) {
+    const T* nu = tab;
+    const T* mu = tab + N;
+    const T* rho = tab + 2 * N;
+    const T* Mf = tab + 3 * N;
+    const T* Rb = tab + 4 * N;
+    T prev = T(0);
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        const int j = s + t;
+        if (j < N) { prev = nu[j] * r[t] + mu[j] * prev; r[t] = prev; }
+    }
+    if (cvalid) gend[c * TL + lloc] = prev;
+    __syncthreads();
+    T G = T(0);
+    for (int cc = 0; cc < c; cc++) G = gend[cc * TL + lloc] + Mf[cc * K + K - 1] * G;
+    T yN1 = T(0);
+    if (cyclic) {
+        T Ga = T(0);
+        for (int cc = 0; cc < P; cc++) Ga = gend[cc * TL + lloc] + Mf[min(cc * K + K, N) - 1] * Ga;
+        yN1 = Ga;
+    }
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        const int j = s + t;
+        if (j < N) r[t] += Mf[j] * G;
+    }
+    T nxt = T(0);
+#pragma unroll
+    for (int t = K - 1; t >= 0; t--) {
+        const int j = s + t;
+        if (j < N) { nxt = r[t] + rho[j] * nxt; r[t] = nxt; }
+    }
+    if (cvalid) xbeg[c * TL + lloc] = nxt;
+    __syncthreads();
+    T X = T(0);
+    for (int cc = P - 1; cc > c; cc--) X = xbeg[cc * TL + lloc] + Rb[cc * K] * X;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        const int j = s + t;
+        if (j < N) r[t] += Rb[j] * X;
+    }
+    if (cyclic) {
+        T X0 = T(0);
+        for (int cc = P - 1; cc >= 0; cc--) X0 = xbeg[cc * TL + lloc] + Rb[cc * K] * X0;
+        const T* z = tab + 5 * N;
+        const T beta = (X0 + tab[6 * N + 1] * yN1) * tab[6 * N];
+#pragma unroll
+        for (int t = 0; t < K; t++) {
+            const int j = s + t;
+            if (j < N) r[t] -= beta * z[j];
+        }
+    }
+}
+
 // One workgroup's share of a line operator: lines blk*TL .. blk*TL+TL-1.
 // Threads beyond TL*P of the workgroup are inert (their chunk index is past
 // the line) but take part in the barriers.  The per-line arithmetic does not
 // depend on TL, so any TL gives the same results bit for bit.
-template <int K, int OP, int AX, int TL>
+template <int K, int OP, int AX, int TL, bool MIX = false>
 __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* smem) {
     constexpr int H = (OP == OP_DERIV) ? 2 : 5;     // stencil half-width
     constexpr int W = K + 2 * H;
@@ -178,11 +255,6 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
     const int s = c * K;
     const bool cvalid = s < N;
     const double* tab = a.tab;
-    const double* nu = tab;
-    const double* mu = tab + N;
-    const double* rho = tab + 2 * N;
-    const double* Mf = tab + 3 * N;
-    const double* Rb = tab + 4 * N;
 
     long base = 0;
     if (AX != 0) base = line_base(AX, active ? line : 0, a.lxi, a.plane);
@@ -314,55 +386,51 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
         for (int t = 0; t < K; t++) r[t] = 0.0;
     }
 
-    // ---- forward recurrence on the chunk, zero carry-in ----
-    double prev = 0.0;
+    const bool cyc = OP == OP_FILTER && a.cyclic;
+    if constexpr (!MIX) {
+        chunk_solve<K, double>(r, tab, N, P, s, c, cvalid, cyc, TL, lloc, gend, xbeg);
+    } else {
+        // FP32 solve, FP64 residual of the FP64 system, FP32 solve of the
+        // residual, FP64 sum: one refinement sweep
+        float x[K], dx[K];
 #pragma unroll
-    for (int t = 0; t < K; t++) {
-        const int j = s + t;
-        if (j < N) { prev = nu[j] * r[t] + mu[j] * prev; r[t] = prev; }
-    }
-    if (cvalid) gend[c * TL + lloc] = prev;
-    __syncthreads();
-    double G = 0.0;
-    for (int cc = 0; cc < c; cc++) G = gend[cc * TL + lloc] + Mf[cc * K + K - 1] * G;
-    double yN1 = 0.0;
-    if (OP == OP_FILTER && a.cyclic) {
-        double Ga = 0.0;
-        for (int cc = 0; cc < P; cc++) Ga = gend[cc * TL + lloc] + Mf[min(cc * K + K, N) - 1] * Ga;
-        yN1 = Ga;
-    }
+        for (int t = 0; t < K; t++) x[t] = (float)r[t];
+        chunk_solve<K, float>(x, a.tab32, N, P, s, c, cvalid, cyc, TL, lloc, (float*)gend, (float*)xbeg);
+        double* first = smem + 2 * kMaxP * TL + (AX == 0 ? (size_t)N * TP : 0);   // [kMaxP][TL]
+        double* last = first + kMaxP * TL;
+        const int L = min(K, N - s);
+        if (cvalid) {
+            double xl = 0.0;
 #pragma unroll
-    for (int t = 0; t < K; t++) {
-        const int j = s + t;
-        if (j < N) r[t] += Mf[j] * G;
-    }
-
-    // ---- backward recurrence, zero carry-in ----
-    double nxt = 0.0;
-#pragma unroll
-    for (int t = K - 1; t >= 0; t--) {
-        const int j = s + t;
-        if (j < N) { nxt = r[t] + rho[j] * nxt; r[t] = nxt; }
-    }
-    if (cvalid) xbeg[c * TL + lloc] = nxt;
-    __syncthreads();
-    double X = 0.0;
-    for (int cc = P - 1; cc > c; cc--) X = xbeg[cc * TL + lloc] + Rb[cc * K] * X;
-#pragma unroll
-    for (int t = 0; t < K; t++) {
-        const int j = s + t;
-        if (j < N) r[t] += Rb[j] * X;
-    }
-    if (OP == OP_FILTER && a.cyclic) {
-        double X0 = 0.0;
-        for (int cc = P - 1; cc >= 0; cc--) X0 = xbeg[cc * TL + lloc] + Rb[cc * K] * X0;
-        const double* z = tab + 5 * N;
-        const double beta = (X0 + tab[6 * N + 1] * yN1) * tab[6 * N];
+            for (int t = 0; t < K; t++)
+                if (t == L - 1) xl = x[t];
+            first[c * TL + lloc] = x[0];
+            last[c * TL + lloc] = xl;
+        }
+        __syncthreads();
+        const double* Am = tab + 6 * N + 44;
+        const double* Bm = Am + N;
+        const double* Cm = Am + 2 * N;
+        const int cl = c > 0 ? c - 1 : P - 1, cr = c + 1 < P ? c + 1 : 0;    // wraps: cyclic corners
+        const double xprev = cvalid ? last[cl * TL + lloc] : 0.0;
+        const double xnext = cvalid ? first[cr * TL + lloc] : 0.0;
 #pragma unroll
         for (int t = 0; t < K; t++) {
             const int j = s + t;
-            if (j < N) r[t] -= beta * z[j];
+            double res = 0.0;
+            if (j < N) {
+                const double xm = t > 0 ? (double)x[t > 0 ? t - 1 : 0] : xprev;
+                double xp = xnext;
+#pragma unroll
+                for (int u = 0; u < K; u++)
+                    if (u == t + 1 && u < L) xp = x[u];
+                res = r[t] - (Am[j] * xm + Bm[j] * (double)x[t] + Cm[j] * xp);
+            }
+            dx[t] = (float)res;
         }
+        chunk_solve<K, float>(dx, a.tab32, N, P, s, c, cvalid, cyc, TL, lloc, (float*)gend, (float*)xbeg);
+#pragma unroll
+        for (int t = 0; t < K; t++) r[t] = (double)x[t] + (double)dx[t];
     }
 
     // ---- boundary values for the characteristic BCs ----
@@ -399,25 +467,32 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
     }
 }
 
-template <int K, int OP, int AX>
+template <int K, int OP, int AX, bool MIX>
 __global__ void __launch_bounds__(max_threads(K)) line_kernel(LineArgs a) {
     extern __shared__ double smem[];
-    line_body<K, OP, AX, lines_per_block(K, AX)>(a, blockIdx.x, smem);
+    line_body<K, OP, AX, lines_per_block(K, AX), MIX>(a, blockIdx.x, smem);
 }
 
-template <int K, int OP, int AX>
-static int launch_one(const LineArgs& a, hipStream_t s) {
+template <int K, int OP, int AX, bool MIX>
+static int launch_mix(const LineArgs& a, hipStream_t s) {
     constexpr int TL = lines_per_block(K, AX);
     dim3 block(TL * a.P);
     dim3 grid((unsigned)((a.nlines + TL - 1) / TL));
     size_t lds = sizeof(double) * 2 * kMaxP * TL;
     if (AX == 0) lds += sizeof(double) * (size_t)a.N * (TL + 1);
+    if (MIX) lds += sizeof(double) * 2 * kMaxP * TL;
     if (lds > 64 * 1024)
-        CS_HIP(hipFuncSetAttribute((const void*)line_kernel<K, OP, AX>,
+        CS_HIP(hipFuncSetAttribute((const void*)line_kernel<K, OP, AX, MIX>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((line_kernel<K, OP, AX>), grid, block, lds, s, a);
+    hipLaunchKernelGGL((line_kernel<K, OP, AX, MIX>), grid, block, lds, s, a);
     CS_HIP(hipGetLastError());
     return CS_OK;
+}
+
+template <int K, int OP, int AX>
+static int launch_one(const LineArgs& a, hipStream_t s) {
+    if (a.mixed) return launch_mix<K, OP, AX, true>(a, s);
+    return launch_mix<K, OP, AX, false>(a, s);
 }
 
 template <int K, int OP>
@@ -430,8 +505,9 @@ static int launch_axis(int axis, const LineArgs& a, hipStream_t s) {
 int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s) {
     if (a.P < 1 || a.P > kMaxP) return fail(CS_ERR_SHAPE, "chunk count out of range");
     if ((long)a.P * K < a.N) return fail(CS_ERR_SHAPE, "chunks do not cover the line");
-    if (axis == 0 && (size_t)a.N * (kXLines + 1) * 8 + 2 * kMaxP * kXLines * 8 > 160 * 1024)
+    if (axis == 0 && (size_t)a.N * (kXLines + 1) * 8 + 4 * kMaxP * kXLines * 8 > 160 * 1024)
         return fail(CS_ERR_SHAPE, "x line too long for the LDS tile");
+    if (a.mixed && !a.tab32) return fail(CS_ERR_STATE, "mixed precision needs the FP32 tables");
     if (op == OP_DERIV) {
         if (K == 8) return launch_axis<8, OP_DERIV>(axis, a, s);
         if (K == 16) return launch_axis<16, OP_DERIV>(axis, a, s);
@@ -573,6 +649,12 @@ int make_line_op(const cs_numerics* num, int op, int n, int start, int end, int 
     CS_HIP(hipMalloc(&t->dev, h.size() * sizeof(double)));
     CS_HIP(hipMemcpy(t->dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     t->count = h.size();
+    // FP32 copy of the factorisation (nu, mu, rho, Mf, Rb, z, 1/den, kappa)
+    // for the mixed-precision solve
+    std::vector<float> h32(6 * (size_t)n + 2);
+    for (size_t i = 0; i < h32.size(); i++) h32[i] = (float)h[i];
+    CS_HIP(hipMalloc(&t->dev32, h32.size() * sizeof(float)));
+    CS_HIP(hipMemcpy(t->dev32, h32.data(), h32.size() * sizeof(float), hipMemcpyHostToDevice));
     return CS_OK;
 }
 
@@ -615,6 +697,8 @@ int line_apply(cs_numerics* num, int op, int axis, const double* in, double* out
     a.seam0 = (op == OP_DERIV) ? start : 0;
     a.seam1 = (op == OP_DERIV) ? end : 0;
     a.cyclic = (op == OP_FILTER) ? (start == 1 && end == 1) : 0;
+    a.tab32 = t->dev32;
+    a.mixed = num->precision;
     return launch_line(op, axis, t->K, a, s);
 }
 

@@ -101,6 +101,12 @@ int cs_deriv(cs_numerics* num, int axis, const double* in, double* out, int var,
  * in == out is allowed. */
 int cs_filter(cs_numerics* num, int axis, const double* in, double* out, cs_stream stream);
 int cs_set_filter_alpha(cs_numerics* num, double alpha_f);
+/* arithmetic of the line solves of this context's derivatives and filters
+ * (no reference counterpart; the reference templates everything on one
+ * Type): 0 = FP64 (default); 1 = the solve in FP32 followed by one
+ * refinement sweep -- FP64 residual of the FP64 system, FP32 solve of it,
+ * FP64 sum.  Right-hand sides, boundary rows and results stay FP64. */
+int cs_set_precision(cs_numerics* num, int mode);
 
 /* ---- physics ----------------------------------------------------------- */
 

@@ -150,3 +150,28 @@ def test_errors_are_reported():
     assert e.value.code == 5
     with pytest.raises(cs.CsError):
         cs.Numerics((16, 16, 16), [[10, 10, 99], [10, 10, 10]])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims", [(64, 64, 64), (100, 33, 17), (300, 9, 8), (48, 40, 36)])
+@pytest.mark.parametrize("nbc", [NBC_CL, NBC_PER], ids=["closure", "periodic"])
+def test_mixed_precision_solves_match_oracle(oracle, dims, nbc):
+    """FP32 line solve + one FP64-residual refinement sweep: derivative and
+    filter within 1e-11 (norm-wise relative) of the FP64 oracle; without the
+    refinement an FP32 solve is ~1e-7 off."""
+    import compact_schemes_amd as cs
+    f = _field(dims, 7)
+    ctx = oracle.Context(dims, nbc)
+    num = cs.Numerics(dims, nbc)
+    num.set_precision(1)
+    fin = dev(f)
+    for axis in range(3):
+        case(f"mixed deriv dims={dims} nbc={nbc[0][0]} axis={axis}")
+        out = dev(np.zeros_like(f))
+        num.deriv(axis, fin, out)
+        err = nrel(host(out), ctx.deriv_field(axis, f))
+        assert err < 1e-11, err
+        case(f"mixed filter dims={dims} nbc={nbc[0][0]} axis={axis}")
+        num.filter(axis, fin, out)
+        err = nrel(host(out), ctx.filter(axis, f))
+        assert err < 1e-11, err

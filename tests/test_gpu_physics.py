@@ -122,6 +122,31 @@ def test_rk4_step_matches_oracle(oracle, dims, nbc):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("nbc", [[[45, 45, 45], [45, 45, 45]], [[10, 10, 10], [10, 10, 10]]], ids=["periodic", "closure"])
+def test_rk4_step_mixed_precision_matches_oracle(oracle, nbc):
+    """The RK4 step with FP32 line solves + FP64-residual refinement against
+    the FP64 oracle: the change of the state over the step within 1e-10."""
+    from compact_schemes_amd.stepper import Stepper
+    dims = (40, 36, 32)
+    st = _state(dims, seed=9)
+    ctx = oracle.Context(dims, nbc)
+    stp = Stepper(dims, nbc, True, st["umf"], st)
+    stp.num.set_precision(1)
+    q = dev(st["qa"])
+    case(f"mixed rk4 step nbc={nbc}")
+    dt = stp.rk4_step(q)
+    qo = st["qa"].copy()
+    ctx.rk4_step(1, qo, st["ssk"], st["xim"], st["etm"], st["zem"], st["yaco"], st["cm"], st["sbcc"], st["umf"])
+    lmx = dims[0] * dims[1] * dims[2]
+    qh = host(q)
+    assert dt > 0
+    for v in range(5):
+        d_gpu = qh[v * lmx:(v + 1) * lmx] - st["qa"][v * lmx:(v + 1) * lmx]
+        d_ref = qo[v * lmx:(v + 1) * lmx] - st["qa"][v * lmx:(v + 1) * lmx]
+        assert nrel(d_gpu, d_ref) < 1e-10, v
+
+
+@pytest.mark.gpu
 def test_walls_match_oracle(oracle):
     """Wall faces: gcbc setup from de (flag 0) and the wall update, then
     wall_condition_update (gcbc.hpp:188-228)."""
