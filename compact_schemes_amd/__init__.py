@@ -34,8 +34,8 @@ EXPORTS = [
     "cs_physics_create", "cs_physics_destroy", "cs_physics_set_frame", "cs_physics_buffers",
     "cs_primitives", "cs_calc_fluxes", "cs_calc_viscous_shear_stress", "cs_calc_time_step",
     "cs_gcbc_go", "cs_wall_condition_update", "cs_rk_stage",
-    "cs_set_precision", "cs_solver_create", "cs_solver_destroy", "cs_solver_rhs", "cs_solver_sweep", "cs_solver_tune",
-    "cs_solver_set_order",
+    "cs_set_precision", "cs_deriv_adjoint", "cs_filter_adjoint", "cs_solver_create", "cs_solver_destroy", "cs_solver_rhs", "cs_solver_sweep", "cs_solver_tune",
+    "cs_solver_set_order", "cs_solver_rhs_vjp", "cs_solver_rk4_vjp",
     "cs_batch_create", "cs_batch_destroy", "cs_batch_bind", "cs_batch_deriv", "cs_batch_filter", "cs_batch_step",
     "cs_batch_capture", "cs_batch_replay", "cs_batch_set_persistent", "cs_batch_info", "cs_batch_autotune", "cs_tune_set", "cs_tune_get",
     "cs_tune_clear", "cs_tune_save", "cs_tune_load",
@@ -105,6 +105,8 @@ def lib():
             "cs_filter": (I, [P, I, P, P, P]),
             "cs_set_filter_alpha": (I, [P, D]),
             "cs_set_precision": (I, [P, I]),
+            "cs_deriv_adjoint": (I, [P, I, P, P, P]),
+            "cs_filter_adjoint": (I, [P, I, P, P, P]),
             "cs_physics_create": (I, [ctypes.POINTER(Dims), I, Point, PP]),
             "cs_physics_destroy": (I, [P]),
             "cs_physics_set_frame": (I, [P, Point, Point]),
@@ -122,6 +124,8 @@ def lib():
             "cs_solver_sweep": (I, [P, I, P, P, P, P]),
             "cs_solver_tune": (I, [P, I, I, I, I]),
             "cs_solver_set_order": (I, [P, I]),
+            "cs_solver_rhs_vjp": (I, [P, P, P, P, P, P]),
+            "cs_solver_rk4_vjp": (I, [P, P, P, D, P, P]),
             "cs_batch_create": (I, [I, P, P, P]),
             "cs_batch_destroy": (I, [P]),
             "cs_batch_bind": (I, [P, P, P]),
@@ -201,6 +205,14 @@ class Numerics:
     def set_precision(self, mode):
         """0: FP64 line solves; 1: FP32 solve + one FP64-residual refinement sweep."""
         check(lib().cs_set_precision(self.h, int(mode)))
+
+    def deriv_adjoint(self, axis, fin, fout, stream=None):
+        """fout = D^T fin, the transpose of deriv along axis."""
+        check(lib().cs_deriv_adjoint(self.h, int(axis), _p(fin), _p(fout), _stream(stream)))
+
+    def filter_adjoint(self, axis, fin, fout, stream=None):
+        """fout = F^T fin, the transpose of filter along axis."""
+        check(lib().cs_filter_adjoint(self.h, int(axis), _p(fin), _p(fout), _stream(stream)))
 
     def deriv_setup(self):
         check(lib().cs_deriv_setup(self.h))

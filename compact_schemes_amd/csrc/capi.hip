@@ -167,6 +167,7 @@ int cs_numerics_destroy(cs_numerics* num) {
         if (num->recv[a]) (void)hipFree(num->recv[a]);
     }
     if (num->pb) (void)hipFree(num->pb);
+    if (num->work) (void)hipFree(num->work);
     for (auto& kv : num->ops) free_line_op(kv.second);
     delete num;
     return CS_OK;
@@ -178,6 +179,11 @@ int cs_set_interface_table(cs_numerics* num, const double* pbco, const double* p
     std::memcpy(num->pb_host + 16, pbci, 16 * sizeof(double));
     CS_HIP(hipMemcpy(num->pb, num->pb_host, sizeof num->pb_host, hipMemcpyHostToDevice));
     num->table_set = true;
+    // transposed derivative tables hold the interface coefficients: rebuild on use
+    for (auto it = num->ops.begin(); it != num->ops.end();) {
+        if (it->first.op == OP_DERIV_T) { free_line_op(it->second); it = num->ops.erase(it); }
+        else ++it;
+    }
     return CS_OK;
 }
 
@@ -215,7 +221,7 @@ int cs_set_filter_alpha(cs_numerics* num, double af) {
     if (!(af > -0.5 && af < 0.5)) return fail(CS_ERR_ARG, "filter alpha must lie in (-0.5, 0.5)");
     num->alpha_f = af;
     for (auto it = num->ops.begin(); it != num->ops.end();) {
-        if (it->first.op == OP_FILTER) { free_line_op(it->second); it = num->ops.erase(it); }
+        if (it->first.op == OP_FILTER || it->first.op == OP_FILTER_T) { free_line_op(it->second); it = num->ops.erase(it); }
         else ++it;
     }
     return CS_OK;

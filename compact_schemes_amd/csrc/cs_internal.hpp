@@ -70,7 +70,9 @@ __host__ __device__ inline long line_base(int axis, long f, int lxi, long plane)
 }
 
 // ---- line operators ------------------------------------------------------------
-enum LineOp { OP_DERIV = 0, OP_FILTER = 1 };
+// OP_SOLVE: the solve alone (right-hand side = the input) with the table of
+// a transposed operator, keyed as OP_DERIV_T / OP_FILTER_T
+enum LineOp { OP_DERIV = 0, OP_FILTER = 1, OP_SOLVE = 2, OP_DERIV_T = 3, OP_FILTER_T = 4 };
 
 // A line operator is the constant tridiagonal system of one line length and
 // one pair of end conditions, stored as the coefficients of two first-order
@@ -109,6 +111,10 @@ inline void free_line_op(LineOpTable& t) {
 }
 
 int choose_chunk(int n, int* K, int* P);
+// tables of the transposed operators (adjoint.hip): LU of A^T in the layout
+// of build_line_table, then the transposed explicit stencil B^T
+int build_adjoint_table(int op, int n, int start, int end, double alpha_f, const double* pb, int K,
+                        std::vector<double>& h);
 int build_line_table(int op, int n, int start, int end, double alpha_f, int K,
                      std::vector<double>& host);
 void filter_coefficients(int M, double af, double* an);
@@ -165,6 +171,7 @@ struct cs_numerics {
     bool table_set = false;
     double alpha_f = 0.49;
     int precision = 0;           // 0: FP64 line solves; 1: FP32 solves + FP64 residual refinement
+    double* work = nullptr;      // [lmx] scratch of the adjoint line operators (allocated on first use)
     std::map<cs::LineOpKey, cs::LineOpTable> ops;
 };
 
@@ -186,6 +193,9 @@ struct cs_physics {
 namespace cs {
 int get_line_op(cs_numerics* num, int op, int axis, int start, int end, LineOpTable** t);
 int make_line_op(const cs_numerics* num, int op, int n, int start, int end, int K, LineOpTable* t);
+int adjoint_workspace(cs_numerics* num, double** w);
+int deriv_adjoint(cs_numerics* num, int axis, const double* in, double* out, hipStream_t s);
+int filter_adjoint(cs_numerics* num, int axis, const double* in, double* out, hipStream_t s);
 int line_apply(cs_numerics* num, int op, int axis, const double* in, double* out, int start, int end,
                const double* recv, int var, bool write_drva, hipStream_t s);
 int check_dims(const cs_dims* d, Geo* g);

@@ -53,6 +53,7 @@ struct cs_solver {
     int red_blocks = 0;
     double* dt = nullptr;      // [0] dt, [1..2] maxima
     double* dt_host = nullptr; // pinned
+    double* vjp = nullptr;     // [41][lmx] adjoint scratch (allocated on the first adjoint call)
 };
 
 namespace cs {

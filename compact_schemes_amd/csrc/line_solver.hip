@@ -235,7 +235,7 @@ __device__ __forceinline__ void chunk_solve(T (&r)[K], const T* tab, int N, int 
 // depend on TL, so any TL gives the same results bit for bit.
 template <int K, int OP, int AX, int TL, bool MIX = false>
 __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* smem) {
-    constexpr int H = (OP == OP_DERIV) ? 2 : 5;     // stencil half-width
+    constexpr int H = (OP == OP_DERIV) ? 2 : (OP == OP_FILTER ? 5 : 0);   // stencil half-width
     constexpr int W = K + 2 * H;
     constexpr int TP = TL + 1;                      // padded LDS row (x tile)
 
@@ -288,7 +288,10 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
             if (OP == OP_FILTER && a.cyclic) row = (row + N) % N;
             w[t] = (row >= 0 && row < N) ? fetch(row) : 0.0;
         }
-        if (OP == OP_DERIV) {
+        if constexpr (OP == OP_SOLVE) {
+#pragma unroll
+            for (int t = 0; t < K; t++) r[t] = w[t];
+        } else if constexpr (OP == OP_DERIV) {
 #pragma unroll
             for (int t = 0; t < K; t++)
                 r[t] = kAa * (w[t + H + 1] - w[t + H - 1]) + kBb * (w[t + H + 2] - w[t + H - 2]);
@@ -491,7 +494,9 @@ static int launch_mix(const LineArgs& a, hipStream_t s) {
 
 template <int K, int OP, int AX>
 static int launch_one(const LineArgs& a, hipStream_t s) {
-    if (a.mixed) return launch_mix<K, OP, AX, true>(a, s);
+    if constexpr (OP != OP_SOLVE) {
+        if (a.mixed) return launch_mix<K, OP, AX, true>(a, s);
+    }
     return launch_mix<K, OP, AX, false>(a, s);
 }
 
@@ -512,10 +517,15 @@ int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s) {
         if (K == 8) return launch_axis<8, OP_DERIV>(axis, a, s);
         if (K == 16) return launch_axis<16, OP_DERIV>(axis, a, s);
         if (K == 32) return launch_axis<32, OP_DERIV>(axis, a, s);
-    } else {
+    } else if (op == OP_FILTER) {
         if (K == 8) return launch_axis<8, OP_FILTER>(axis, a, s);
         if (K == 16) return launch_axis<16, OP_FILTER>(axis, a, s);
         if (K == 32) return launch_axis<32, OP_FILTER>(axis, a, s);
+    } else if (op == OP_SOLVE) {
+        if (a.mixed) return fail(CS_ERR_UNSUPPORTED, "transposed solves run in FP64");
+        if (K == 8) return launch_axis<8, OP_SOLVE>(axis, a, s);
+        if (K == 16) return launch_axis<16, OP_SOLVE>(axis, a, s);
+        if (K == 32) return launch_axis<32, OP_SOLVE>(axis, a, s);
     }
     return fail(CS_ERR_SHAPE, "unsupported chunk length");
 }
@@ -644,7 +654,9 @@ int make_line_op(const cs_numerics* num, int op, int n, int start, int end, int 
         if (rc) return rc;
     }
     std::vector<double> h;
-    int rc = build_line_table(op, n, start, end, num->alpha_f, t->K, h);
+    int rc = (op == OP_DERIV_T || op == OP_FILTER_T)
+                 ? build_adjoint_table(op, n, start, end, num->alpha_f, num->pb_host, t->K, h)
+                 : build_line_table(op, n, start, end, num->alpha_f, t->K, h);
     if (rc) return rc;
     CS_HIP(hipMalloc(&t->dev, h.size() * sizeof(double)));
     CS_HIP(hipMemcpy(t->dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));

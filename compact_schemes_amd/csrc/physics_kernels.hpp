@@ -5,6 +5,7 @@
 #pragma once
 
 #include "cs_internal.hpp"
+#include "dual.hpp"
 
 namespace cs {
 
@@ -13,13 +14,17 @@ __device__ __forceinline__ void load_q(const double* qa, long t, long lmx, doubl
     for (int v = 0; v < 5; v++) q[v] = qa[t + (long)v * lmx];
 }
 
-struct Prim {
-    double rhoi, u, v, w, p, c2;
+template <class T>
+struct PrimT {
+    T rhoi, u, v, w, p, c2;
 };
+using Prim = PrimT<double>;
 
-// primitive variables from the conserved ones (absent upstream)
-__device__ __forceinline__ Prim primitive(const double* q) {
-    Prim r;
+// primitive variables from the conserved ones (absent upstream); T = double,
+// or a dual number for Jacobian products
+template <class T>
+__device__ __forceinline__ PrimT<T> primitive(const T* q) {
+    PrimT<T> r;
     r.rhoi = 1.0 / q[0];
     r.u = q[1] * r.rhoi;
     r.v = q[2] * r.rhoi;
@@ -82,11 +87,14 @@ __device__ __forceinline__ void viscous_closure(double mu, double yaco, const do
 
 // Eigen-decomposition of the normal-flux Jacobian at a face point
 // (gcbc_vgpr, src/cuda/kernels/gcbc.hpp:38-158; left vectors D10-corrected).
-struct Eig {
-    double ao, aoi, hv2, vn, vs, ve[3];
+// T = double, or a dual number for Jacobian products; the vectors the
+// matrices apply to may be dual numbers of their own (type X).
+template <class T>
+struct EigT {
+    T ao, aoi, hv2, vn, vs, ve[3];
     // gcbc_vgpr::eleme, src/cuda/kernels/gcbc.hpp:44-61
-    __device__ void eleme(const double* cm, double qa1, const double* qa24, double pp, cs_point umf) {
-        double rhoi = 1.0 / qa1;
+    __device__ void eleme(const double* cm, T qa1, const T* qa24, T pp, cs_point umf) {
+        T rhoi = 1.0 / qa1;
         ao = sqrt(kGam * rhoi * pp);
         aoi = 1.0 / ao;
         ve[0] = rhoi * qa24[0];
@@ -97,11 +105,12 @@ struct Eig {
         vs = cm[0] * umf.x + cm[1] * umf.y + cm[2] * umf.z;
     }
     // characteristic -> conservative, gcbc_vgpr::xtr2q, src/cuda/kernels/gcbc.hpp:63-105
-    __device__ void r_apply(const double* cm, const double* x, double* y) const {
-        const double bo = hv2 + kHamm1 * ao * ao, co = ao * vn;
-        const double dm0 = ao * cm[0], dm1 = ao * cm[1], dm2 = ao * cm[2];
-        const double a13 = 0.5 * (ve[0] + dm0), a23 = 0.5 * (ve[1] + dm1), a33 = 0.5 * (ve[2] + dm2);
-        const double a43 = 0.5 * (bo + co);
+    template <class X>
+    __device__ void r_apply(const double* cm, const X* x, X* y) const {
+        const T bo = hv2 + kHamm1 * ao * ao, co = ao * vn;
+        const T dm0 = ao * cm[0], dm1 = ao * cm[1], dm2 = ao * cm[2];
+        const T a13 = 0.5 * (ve[0] + dm0), a23 = 0.5 * (ve[1] + dm1), a33 = 0.5 * (ve[2] + dm2);
+        const T a43 = 0.5 * (bo + co);
         y[0] = cm[0] * x[0] + cm[1] * x[1] + cm[2] * x[2] + 0.5 * x[3] + 0.5 * x[4];
         y[1] = cm[0] * ve[0] * x[0] + (cm[1] * ve[0] - dm2) * x[1] + (cm[2] * ve[0] + dm1) * x[2] + a13 * x[3] +
                (a13 - dm0) * x[4];
@@ -113,10 +122,11 @@ struct Eig {
                (hv2 * cm[2] + dm1 * ve[0] - dm0 * ve[1]) * x[2] + a43 * x[3] + (a43 - co) * x[4];
     }
     // conservative -> characteristic, gcbc_vgpr::xtq2r, src/cuda/kernels/gcbc.hpp:107-150 (D10 corrected)
-    __device__ void l_apply(const double* cm, const double* x, double* y) const {
-        const double ho = kGamm1 * aoi * aoi, bo = 1.0 - ho * hv2, co = aoi * vn;
-        const double dm0 = aoi * cm[0], dm1 = aoi * cm[1], dm2 = aoi * cm[2];
-        const double rv0 = ho * ve[0], rv1 = ho * ve[1], rv2 = ho * ve[2];
+    template <class X>
+    __device__ void l_apply(const double* cm, const X* x, X* y) const {
+        const T ho = kGamm1 * aoi * aoi, bo = 1.0 - ho * hv2, co = aoi * vn;
+        const T dm0 = aoi * cm[0], dm1 = aoi * cm[1], dm2 = aoi * cm[2];
+        const T rv0 = ho * ve[0], rv1 = ho * ve[1], rv2 = ho * ve[2];
         y[0] = (bo * cm[0] + dm1 * ve[2] - dm2 * ve[1]) * x[0] + cm[0] * rv0 * x[1] + (cm[0] * rv1 + dm2) * x[2] +
                (cm[0] * rv2 - dm1) * x[3] - ho * cm[0] * x[4];
         y[1] = (bo * cm[1] + dm2 * ve[0] - dm0 * ve[2]) * x[0] + (cm[1] * rv0 - dm2) * x[1] + cm[1] * rv1 * x[2] +
@@ -127,6 +137,7 @@ struct Eig {
         y[4] = (1.0 - bo + co) * x[0] + (-dm0 - rv0) * x[1] + (-dm1 - rv1) * x[2] + (-dm2 - rv2) * x[3] + ho * x[4];
     }
 };
+using Eig = EigT<double>;
 
 int launch_primitives(const double* qa, double* de, double* p, long lmx, hipStream_t s);
 int launch_fluxes_pre(bool visc, double* buffer, const double* qa, const double* p, double* de, const double* xim,

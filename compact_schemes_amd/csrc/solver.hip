@@ -69,7 +69,7 @@ extern "C" {
 
 int cs_solver_destroy(cs_solver* sv) {
     if (!sv) return CS_OK;
-    double* bufs[] = {sv->ga, sv->gb, sv->tau, sv->hf, sv->dpart, sv->qb, sv->qc, sv->red, sv->dt,
+    double* bufs[] = {sv->ga, sv->gb, sv->tau, sv->hf, sv->dpart, sv->qb, sv->qc, sv->red, sv->dt, sv->vjp,
                       sv->tabs[0], sv->tabs[1], sv->tabs[2]};
     for (double* p : bufs)
         if (p) (void)hipFree(p);

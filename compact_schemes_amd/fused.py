@@ -143,6 +143,18 @@ class FusedSolver:
                                        ctypes.byref(out) if report_dt else None, _stream(stream)))
         return out.value if report_dt else None
 
+    def rhs_vjp(self, q, lam, g=None, stream=None):
+        """g = (dR/dq)^T lam at q (inviscid solvers)."""
+        import torch
+        g = torch.empty_like(lam) if g is None else g
+        check(lib().cs_solver_rhs_vjp(self.h, _p(q), _p(lam), _p(g), _p(self.sbcc), _stream(stream)))
+        return g
+
+    def rk4_vjp(self, q0, lam, dt, stream=None):
+        """lam <- (d step(q0) / d q0)^T lam for one RK4 step with fixed dt (in place)."""
+        check(lib().cs_solver_rk4_vjp(self.h, _p(q0), _p(lam), float(dt), _p(self.sbcc), _stream(stream)))
+        return lam
+
     def time_sweeps(self, q, stream, reps=10, sweeps=None):
         """Average duration (ms) of each sweep, HIP events on the launch stream
         (default: the sweeps of one RK stage)."""

@@ -107,6 +107,14 @@ int cs_set_filter_alpha(cs_numerics* num, double alpha_f);
  * refinement sweep -- FP64 residual of the FP64 system, FP32 solve of it,
  * FP64 sum.  Right-hand sides, boundary rows and results stay FP64. */
 int cs_set_precision(cs_numerics* num, int mode);
+/* transposed line operators (no reference counterpart): out = D^T in and
+ * out = F^T in for the derivative D = A^-1 B (cs_deriv1d/cs_deriv with this
+ * context's closure or seam rows) and the filter F of cs_filter along
+ * `axis`: a solve with A^T, then the transposed explicit stencil B^T.  Lines
+ * of at least 16 points; out != in; the first call allocates one field of
+ * scratch in the context. */
+int cs_deriv_adjoint(cs_numerics* num, int axis, const double* in, double* out, cs_stream stream);
+int cs_filter_adjoint(cs_numerics* num, int axis, const double* in, double* out, cs_stream stream);
 
 /* ---- physics ----------------------------------------------------------- */
 
@@ -218,6 +226,18 @@ int cs_solver_time_step(cs_solver* sv, const double* qa, double* dte, cs_stream 
  * receives the step used; passing it synchronises the stream. */
 int cs_solver_rk4_step(cs_solver* sv, double* qa, const double* sbcc, double dt, double* dt_out,
                        cs_stream stream);
+/* adjoint of the inviscid right-hand side (no reference counterpart):
+ * g = (dR/dq at qa)^T lam, R = cs_solver_rhs, characteristic corrections
+ * included.  The first adjoint call allocates 41 fields of scratch. */
+int cs_solver_rhs_vjp(cs_solver* sv, const double* qa, const double* lam, double* g, const double* sbcc,
+                      cs_stream stream);
+/* adjoint of one RK4 step of cs_solver_rk4_step with the given dt (held
+ * fixed; the filter included when the solver filters): lam (the gradient of
+ * a loss with respect to the state after the step) is replaced by the
+ * gradient with respect to q0, the state before it.  q0 is not modified;
+ * the three inner stages are recomputed from it. */
+int cs_solver_rk4_vjp(cs_solver* sv, const double* q0, double* lam, double dt, const double* sbcc,
+                      cs_stream stream);
 
 /* ---- batched subdomains (no reference counterpart) ----------------------
  * The plain method -- derivative of a field along x, y and z
