@@ -38,10 +38,11 @@ def parse():
                     help="only launch the dominant sweep and a plain device copy --reps times each "
                          "(the program rocprofv3 --pmc runs for the traffic figures)")
     ap.add_argument("--reps", type=int, default=5)
-    ap.add_argument("--workload", default="rhs", choices=["rhs", "batch96", "precision"],
+    ap.add_argument("--workload", default="rhs", choices=["rhs", "batch96", "precision", "adjoint"],
                     help="rhs: the 256^3 viscous RK4 step (default); batch96: derivative x/y/z + filter "
                          "x/y/z of 96 periodic subdomains with seeded sizes in [48, 320]; precision: 50 RK4 "
-                         "steps at 512x512x256 with FP64 and with FP32+refinement line solves")
+                         "steps at 512x512x256 with FP64 and with FP32+refinement line solves; adjoint: gradient "
+                         "of an acoustic-energy loss after 40 Euler RK4 steps at 128^3 (checkpointed adjoint)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-reps", type=int, default=3)
     return ap.parse_args()
@@ -82,6 +83,8 @@ def main():
         return batch_bench(args)
     if args.workload == "precision":
         return precision_bench(args)
+    if args.workload == "adjoint":
+        return adjoint_bench(args)
     n = args.n
     dims = (n, n, n)
     lmx = n ** 3
@@ -286,11 +289,12 @@ def batch_bench(args):
 def precision_bench(args, dims=(512, 512, 256), nsteps=50):
     """Config 4: the flow of the RHS workload on 512x512x256 with periodic
     faces, 50 RK4 steps (reference-ordered operators, viscous, filter every
-    step, dt fixed from the CFL limit of the initial state) once with FP64
-    line solves and once with FP32 solves + one FP64-residual refinement
-    sweep (fluxes, boundary rows, stage update in FP64).  Reports the error
-    of the mixed run against the FP64 run per conserved variable, relative
-    to the state and to its change over the run."""
+    step, dt fixed from the CFL limit of the initial state) with FP64 line
+    solves, then with FP32 solves + one and + two FP64-residual refinement
+    sweeps (fluxes, boundary rows, stage update in FP64).  Reports the error
+    of the mixed runs against the FP64 run per conserved variable, relative
+    to the state and to its change over the run; value = the largest
+    state-relative error with one sweep."""
     import numpy as np
     import torch
     from compact_schemes_amd import workloads as wl
@@ -302,7 +306,7 @@ def precision_bench(args, dims=(512, 512, 256), nsteps=50):
     del st["qa"]
     res, times = {}, {}
     dt = None
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         stepper = Stepper(dims, per, True, st["umf"], st)
         stepper.num.set_precision(mode)
         q = q0.clone()
@@ -322,28 +326,138 @@ def precision_bench(args, dims=(512, 512, 256), nsteps=50):
         res[mode] = q
         del stepper
         torch.cuda.empty_cache()
-    err = []
-    for v in range(5):
-        sl = slice(v * lmx, (v + 1) * lmx)
-        a, b, c0 = res[1][sl], res[0][sl], q0[sl]
-        d = (a - b)
-        ch = (b - c0)
-        err.append({"var": ["rho", "rho_u", "rho_v", "rho_w", "E"][v],
-                    "max_rel_state": float(d.abs().max() / b.abs().max()),
-                    "l2_rel_state": float(d.norm() / b.norm()),
-                    "max_rel_change": float(d.abs().max() / ch.abs().max()),
-                    "l2_rel_change": float(d.norm() / ch.norm())})
-    worst = max(e["max_rel_state"] for e in err)
+
+    def errors(mode):
+        err = []
+        for v in range(5):
+            sl = slice(v * lmx, (v + 1) * lmx)
+            a, b, c0 = res[mode][sl], res[0][sl], q0[sl]
+            d = a - b
+            ch = b - c0
+            err.append({"var": ["rho", "rho_u", "rho_v", "rho_w", "E"][v],
+                        "max_rel_state": float(d.abs().max() / b.abs().max()),
+                        "l2_rel_state": float(d.norm() / b.norm()),
+                        "max_rel_change": float(d.abs().max() / ch.abs().max()),
+                        "l2_rel_change": float(d.norm() / ch.norm())})
+        return err
+    e1, e2 = errors(1), errors(2)
     line = {
         "metric": "FP32+refinement vs FP64: max relative error after 50 RK4 steps",
-        "value": worst, "unit": "relative", "n_gpus": 1, "steps": nsteps, "warmup": 1,
+        "value": max(e["max_rel_state"] for e in e1), "unit": "relative", "n_gpus": 1, "steps": nsteps, "warmup": 1,
         "ms_per_step": round(1e3 * times[1], 3), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 solves + f64 refinement",
         "data": "synthetic (seeded acoustic pulse on uniform Mach 0.5 flow, periodic faces)",
         "config": {"workload": "viscous RHS, 512x512x256, periodic, 50 RK4 steps with GV10 filter, "
                                "reference-ordered operators", "dt": dt,
-                   "ms_per_step_fp64": round(1e3 * times[0], 3), "ms_per_step_mixed": round(1e3 * times[1], 3),
-                   "errors": err},
+                   "ms_per_step_fp64": round(1e3 * times[0], 3), "ms_per_step_one_sweep": round(1e3 * times[1], 3),
+                   "ms_per_step_two_sweeps": round(1e3 * times[2], 3),
+                   "two_sweeps_max_rel_state": max(e["max_rel_state"] for e in e2),
+                   "errors_one_sweep": e1, "errors_two_sweeps": e2},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def adjoint_bench(args, n=128, nsteps=40, every=8, plane=96, ndir=32):
+    """Config 5: Euler at 128^3, periodic x, non-reflective y and z; the
+    initial state is uniform Mach 0.5 flow plus an isentropic density
+    perturbation rho' (seeded Gaussian, amplitude 1e-4, p' = c0^2 rho');
+    40 RK4 steps with the filter and dt fixed from the initial CFL limit
+    (CFL 0.5);
+    loss = acoustic energy sum(p'^2/(2 rho0 c0^2) + rho0 |u'|^2/2) on the
+    plane z = 96.  The gradient with respect to rho' comes from the adjoint
+    with checkpoints every 8 steps (states recomputed per segment); it is
+    checked by central finite differences on 32 seeded directions."""
+    import numpy as np
+    import torch
+    from compact_schemes_amd import workloads as wl
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (n, n, n)
+    lmx = n ** 3
+    nbc = [[45, 10, 10], [45, 10, 10]]
+    st = wl.flow_state(dims, seed=5, amp=1e-4)
+    st0 = wl.flow_state(dims, seed=5, amp=0.0)
+    base = torch.from_numpy(st0["qa"]).cuda()
+    rho1 = torch.from_numpy(st["qa"][:lmx] - st0["qa"][:lmx]).cuda()       # rho' of the pulse
+    gam = wl.GAM
+    p0, rho0, c0 = 1.0 / gam, 1.0, 1.0
+    # CFL 0.5: with non-reflective faces the step is unstable at 0.95 once the
+    # pulse reaches the boundaries (about 25 steps at this size)
+    sv = FusedSolver(dims, nbc, st, viscous=False, cfl=0.5)
+
+    def initial(r):
+        q = base.clone()
+        q[:lmx] += r
+        q[4 * lmx:] += (c0 * c0 / (gam - 1.0)) * r            # p' = c0^2 rho', u' = 0
+        return q
+
+    def loss_and_grad(q):
+        qp = q.view(5, n, n, n)[:, plane].detach().clone().requires_grad_(True)
+        r, m, E = qp[0], qp[1:4], qp[4]
+        pr = (gam - 1.0) * (E - 0.5 * (m * m).sum(0) / r)
+        u2 = ((m / r) ** 2).sum(0)
+        L = ((pr - p0) ** 2 / (2 * rho0 * c0 * c0) + 0.5 * rho0 * u2).sum()
+        L.backward()
+        lam = torch.zeros(5 * lmx, dtype=torch.float64, device="cuda")
+        lam.view(5, n, n, n)[:, plane] = qp.grad
+        return float(L), lam
+
+    q = initial(rho1)
+    dt = sv.time_step(q)
+
+    def forward(r, keep=False):
+        q = initial(r)
+        ck = {}
+        for k in range(nsteps):
+            if keep and k % every == 0:
+                ck[k] = q.clone()
+            sv.rk4_step(q, dt=dt)
+        return q, ck
+
+    # forward with checkpoints, then the adjoint sweep
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    qT, ck = forward(rho1, keep=True)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    L0, lam = loss_and_grad(qT)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k0 in sorted(ck, reverse=True):
+        states = [ck[k0]]
+        for k in range(k0, min(k0 + every, nsteps) - 1):
+            x = states[-1].clone()
+            sv.rk4_step(x, dt=dt)
+            states.append(x)
+        for x in reversed(states):
+            sv.rk4_vjp(x, lam, dt)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    grad = lam[:lmx] + (c0 * c0 / (gam - 1.0)) * lam[4 * lmx:]
+    # central finite differences on seeded directions
+    eps = 1e-5
+    errs = []
+    for d in range(ndir):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(500 + d)
+        v = torch.randn(lmx, generator=g, device="cuda", dtype=torch.float64)
+        v /= v.abs().max()
+        Lp, _ = loss_and_grad(forward(rho1 + eps * v)[0])
+        Lm, _ = loss_and_grad(forward(rho1 - eps * v)[0])
+        fd = (Lp - Lm) / (2 * eps)
+        ad = float(torch.dot(grad, v))
+        errs.append(abs(fd - ad) / abs(ad))
+    line = {
+        "metric": "adjoint/forward cost ratio, 40 Euler RK4 steps at 128^3 (checkpoints every 8)",
+        "value": round(t_adj / t_fwd, 3), "unit": "ratio", "n_gpus": 1, "steps": nsteps, "warmup": 0,
+        "ms_per_step": round(1e3 * t_fwd / nsteps, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (uniform Mach 0.5 flow + seeded isentropic density pulse, amplitude 1e-4)",
+        "config": {"workload": "Euler 128^3, periodic x, non-reflective y/z, GV10 filter, 40 RK4 steps, loss = "
+                               "acoustic energy on the plane z=96, gradient w.r.t. the initial density perturbation",
+                   "loss": L0, "dt": dt, "forward_s": round(t_fwd, 4), "adjoint_s_incl_recompute": round(t_adj, 4),
+                   "fd_directions": ndir, "fd_eps": eps, "fd_rel_err_max": max(errs),
+                   "fd_rel_err_median": float(np.median(errs)),
+                   "checkpoint_fields": 5 * len(ck) + 5 * every},
     }
     print(json.dumps(line), flush=True)
 

@@ -203,7 +203,7 @@ class Numerics:
             pass
 
     def set_precision(self, mode):
-        """0: FP64 line solves; 1: FP32 solve + one FP64-residual refinement sweep."""
+        """0: FP64 line solves; k = 1..4: FP32 solve + k FP64-residual refinement sweeps."""
         check(lib().cs_set_precision(self.h, int(mode)))
 
     def deriv_adjoint(self, axis, fin, fout, stream=None):

@@ -5,10 +5,11 @@
 // single periodic rank) followed by the compact filter along x, y, z, applied
 // to a set of subdomains of unrelated shapes.  Every (subdomain, axis) line
 // operator is cut into workgroup-sized items, and the items of all
-// subdomains that share a kernel variant (chunk length K, axis) run in one
-// persistent launch: the derivative phase has one launch per variant present
-// (the three derivatives are independent), the filter one per variant and
-// axis, x, then y, then z (each filters the previous result in place).  Items
+// subdomains that share a kernel variant (chunk length K, axis) and a chunk
+// count P (so a workgroup has exactly the threads its lines need) run in one
+// launch: the derivative phase has one launch per such class present (the
+// three derivatives are independent), the filter one per class and axis, x,
+// then y, then z (each filters the previous result in place).  Items
 // are ordered longest first and dealt round-robin to a grid sized to fill the
 // device, so the uneven subdomains share the CUs instead of each paying the
 // tail of a small launch.
@@ -28,8 +29,8 @@ using namespace cs;
 
 namespace {
 
-struct Launch {                // one kernel variant (K, axis) of a phase
-    int K = 8, axis = 0;
+struct Launch {                // one kernel variant (K, axis) and chunk count P of a phase
+    int K = 8, axis = 0, P = 1;
     int first = 0, nitems = 0;  // range in the phase's item list
     int threads = 0, grid = 0;
     size_t lds = 0;
@@ -117,11 +118,12 @@ static int build_phase(cs_batch* b, Phase& p, int op, int axis_lo, int axis_hi) 
             if (rc) return rc;
             int v = -1;
             for (size_t k = 0; k < var.size(); k++)
-                if (var[k].K == K && var[k].axis == axis) v = (int)k;
+                if (var[k].K == K && var[k].axis == axis && var[k].P == a.P) v = (int)k;
             if (v < 0) {
                 Launch l;
                 l.K = K;
                 l.axis = axis;
+                l.P = a.P;
                 var.push_back(l);
                 v = (int)var.size() - 1;
             }

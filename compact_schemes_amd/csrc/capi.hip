@@ -229,7 +229,7 @@ int cs_set_filter_alpha(cs_numerics* num, double af) {
 
 int cs_set_precision(cs_numerics* num, int mode) {
     if (!num) return fail(CS_ERR_ARG, "null context");
-    if (mode != 0 && mode != 1) return fail(CS_ERR_ARG, "precision mode is 0 (FP64) or 1 (FP32 + FP64 refinement)");
+    if (mode < 0 || mode > 4) return fail(CS_ERR_ARG, "precision mode is 0 (FP64) or 1..4 (FP32 + that many FP64 refinement sweeps)");
     num->precision = mode;
     return CS_OK;
 }

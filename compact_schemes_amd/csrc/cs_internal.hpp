@@ -136,7 +136,7 @@ struct LineArgs {
     int seam0, seam1;     // deriv: seam rows at the start / end of the line
     int cyclic;           // filter: cyclic system
     const float* tab32;   // FP32 factorisation (mixed precision)
-    int mixed;            // 1: FP32 solve + one FP64-residual refinement sweep
+    int mixed;            // k > 0: FP32 solve + k FP64-residual refinement sweeps
 };
 
 int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s);
@@ -170,7 +170,7 @@ struct cs_numerics {
     double pb_host[32];
     bool table_set = false;
     double alpha_f = 0.49;
-    int precision = 0;           // 0: FP64 line solves; 1: FP32 solves + FP64 residual refinement
+    int precision = 0;           // 0: FP64 line solves; k > 0: FP32 solves + k FP64-residual refinement sweeps
     double* work = nullptr;      // [lmx] scratch of the adjoint line operators (allocated on first use)
     std::map<cs::LineOpKey, cs::LineOpTable> ops;
 };

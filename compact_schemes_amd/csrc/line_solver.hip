@@ -393,47 +393,56 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
     if constexpr (!MIX) {
         chunk_solve<K, double>(r, tab, N, P, s, c, cvalid, cyc, TL, lloc, gend, xbeg);
     } else {
-        // FP32 solve, FP64 residual of the FP64 system, FP32 solve of the
-        // residual, FP64 sum: one refinement sweep
-        float x[K], dx[K];
+        // FP32 solve, then `mixed` refinement sweeps: FP64 residual of the
+        // FP64 system, FP32 solve of it, FP64 update
+        float x32[K];
 #pragma unroll
-        for (int t = 0; t < K; t++) x[t] = (float)r[t];
-        chunk_solve<K, float>(x, a.tab32, N, P, s, c, cvalid, cyc, TL, lloc, (float*)gend, (float*)xbeg);
+        for (int t = 0; t < K; t++) x32[t] = (float)r[t];
+        chunk_solve<K, float>(x32, a.tab32, N, P, s, c, cvalid, cyc, TL, lloc, (float*)gend, (float*)xbeg);
+        double x[K];
+#pragma unroll
+        for (int t = 0; t < K; t++) x[t] = (double)x32[t];
         double* first = smem + 2 * kMaxP * TL + (AX == 0 ? (size_t)N * TP : 0);   // [kMaxP][TL]
         double* last = first + kMaxP * TL;
         const int L = min(K, N - s);
-        if (cvalid) {
-            double xl = 0.0;
-#pragma unroll
-            for (int t = 0; t < K; t++)
-                if (t == L - 1) xl = x[t];
-            first[c * TL + lloc] = x[0];
-            last[c * TL + lloc] = xl;
-        }
-        __syncthreads();
         const double* Am = tab + 6 * N + 44;
         const double* Bm = Am + N;
         const double* Cm = Am + 2 * N;
         const int cl = c > 0 ? c - 1 : P - 1, cr = c + 1 < P ? c + 1 : 0;    // wraps: cyclic corners
-        const double xprev = cvalid ? last[cl * TL + lloc] : 0.0;
-        const double xnext = cvalid ? first[cr * TL + lloc] : 0.0;
+        for (int sweep = 0; sweep < a.mixed; sweep++) {
+            __syncthreads();          // the previous sweep's reads of first/last are done
+            if (cvalid) {
+                double xl = 0.0;
 #pragma unroll
-        for (int t = 0; t < K; t++) {
-            const int j = s + t;
-            double res = 0.0;
-            if (j < N) {
-                const double xm = t > 0 ? (double)x[t > 0 ? t - 1 : 0] : xprev;
-                double xp = xnext;
-#pragma unroll
-                for (int u = 0; u < K; u++)
-                    if (u == t + 1 && u < L) xp = x[u];
-                res = r[t] - (Am[j] * xm + Bm[j] * (double)x[t] + Cm[j] * xp);
+                for (int t = 0; t < K; t++)
+                    if (t == L - 1) xl = x[t];
+                first[c * TL + lloc] = x[0];
+                last[c * TL + lloc] = xl;
             }
-            dx[t] = (float)res;
-        }
-        chunk_solve<K, float>(dx, a.tab32, N, P, s, c, cvalid, cyc, TL, lloc, (float*)gend, (float*)xbeg);
+            __syncthreads();
+            const double xprev = cvalid ? last[cl * TL + lloc] : 0.0;
+            const double xnext = cvalid ? first[cr * TL + lloc] : 0.0;
+            float dx[K];
 #pragma unroll
-        for (int t = 0; t < K; t++) r[t] = (double)x[t] + (double)dx[t];
+            for (int t = 0; t < K; t++) {
+                const int j = s + t;
+                double res = 0.0;
+                if (j < N) {
+                    const double xm = t > 0 ? x[t > 0 ? t - 1 : 0] : xprev;
+                    double xp = xnext;
+#pragma unroll
+                    for (int u = 0; u < K; u++)
+                        if (u == t + 1 && u < L) xp = x[u];
+                    res = r[t] - (Am[j] * xm + Bm[j] * x[t] + Cm[j] * xp);
+                }
+                dx[t] = (float)res;
+            }
+            chunk_solve<K, float>(dx, a.tab32, N, P, s, c, cvalid, cyc, TL, lloc, (float*)gend, (float*)xbeg);
+#pragma unroll
+            for (int t = 0; t < K; t++) x[t] += (double)dx[t];
+        }
+#pragma unroll
+        for (int t = 0; t < K; t++) r[t] = x[t];
     }
 
     // ---- boundary values for the characteristic BCs ----

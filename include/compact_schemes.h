@@ -103,9 +103,9 @@ int cs_filter(cs_numerics* num, int axis, const double* in, double* out, cs_stre
 int cs_set_filter_alpha(cs_numerics* num, double alpha_f);
 /* arithmetic of the line solves of this context's derivatives and filters
  * (no reference counterpart; the reference templates everything on one
- * Type): 0 = FP64 (default); 1 = the solve in FP32 followed by one
- * refinement sweep -- FP64 residual of the FP64 system, FP32 solve of it,
- * FP64 sum.  Right-hand sides, boundary rows and results stay FP64. */
+ * Type): 0 = FP64 (default); k = 1..4: the solve in FP32 followed by k
+ * refinement sweeps -- FP64 residual of the FP64 system, FP32 solve of it,
+ * FP64 update.  Right-hand sides, boundary rows and results stay FP64. */
 int cs_set_precision(cs_numerics* num, int mode);
 /* transposed line operators (no reference counterpart): out = D^T in and
  * out = F^T in for the derivative D = A^-1 B (cs_deriv1d/cs_deriv with this
@@ -243,10 +243,10 @@ int cs_solver_rk4_vjp(cs_solver* sv, const double* q0, double* lam, double dt, c
  * The plain method -- derivative of a field along x, y and z
  * (numerics_pc::deriv1d, src/numerics_pc.hpp:74-119, seam faces wrapped on
  * the subdomain itself) and the compact filter along x, y, z -- on many
- * independent subdomains of unrelated shapes: one persistent launch per kernel
- * variant (chunk length, axis) serves the items of all subdomains -- at most
- * 9 launches for the three derivatives, 3 per filter axis.  Each subdomain's
- * results are bit-identical to cs_deriv / cs_filter on it alone. */
+ * independent subdomains of unrelated shapes: one launch per kernel variant
+ * (chunk length, axis, chunk count) serves the items of all subdomains.
+ * Each subdomain's results are bit-identical to cs_deriv / cs_filter on it
+ * alone. */
 typedef struct cs_batch cs_batch;
 /* count subdomains; dims[count]; nbc6 = [count][2 face][3 axis] */
 int cs_batch_create(int count, const cs_dims* dims, const int* nbc6, cs_batch** out);

@@ -171,6 +171,8 @@ def test_mixed_precision_solves_match_oracle(oracle, dims, nbc):
         num.deriv(axis, fin, out)
         err = nrel(host(out), ctx.deriv_field(axis, f))
         assert err < 1e-11, err
+        if nbc is NBC_PER and dims[axis] < 11:
+            continue                                  # the cyclic filter needs 11 points
         case(f"mixed filter dims={dims} nbc={nbc[0][0]} axis={axis}")
         num.filter(axis, fin, out)
         err = nrel(host(out), ctx.filter(axis, f))
