@@ -137,6 +137,8 @@ struct LineArgs {
     int cyclic;           // filter: cyclic system
     const float* tab32;   // FP32 factorisation (mixed precision)
     int mixed;            // k > 0: FP32 solve + k FP64-residual refinement sweeps
+    int nfields;          // fields processed by one launch (grid y), 0 or 1 = one
+    long fstride;         // distance between those fields
 };
 
 int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s);
@@ -197,6 +199,6 @@ int adjoint_workspace(cs_numerics* num, double** w);
 int deriv_adjoint(cs_numerics* num, int axis, const double* in, double* out, hipStream_t s);
 int filter_adjoint(cs_numerics* num, int axis, const double* in, double* out, hipStream_t s);
 int line_apply(cs_numerics* num, int op, int axis, const double* in, double* out, int start, int end,
-               const double* recv, int var, bool write_drva, hipStream_t s);
+               const double* recv, int var, bool write_drva, hipStream_t s, int nfields = 1);
 int check_dims(const cs_dims* d, Geo* g);
 }  // namespace cs

@@ -479,9 +479,14 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
     }
 }
 
+// grid y runs a.nfields fields a.fstride apart with the same operator
 template <int K, int OP, int AX, bool MIX>
 __global__ void __launch_bounds__(max_threads(K)) line_kernel(LineArgs a) {
     extern __shared__ double smem[];
+    if (blockIdx.y > 0) {
+        a.in += (long)blockIdx.y * a.fstride;
+        a.out += (long)blockIdx.y * a.fstride;
+    }
     line_body<K, OP, AX, lines_per_block(K, AX), MIX>(a, blockIdx.x, smem);
 }
 
@@ -489,7 +494,7 @@ template <int K, int OP, int AX, bool MIX>
 static int launch_mix(const LineArgs& a, hipStream_t s) {
     constexpr int TL = lines_per_block(K, AX);
     dim3 block(TL * a.P);
-    dim3 grid((unsigned)((a.nlines + TL - 1) / TL));
+    dim3 grid((unsigned)((a.nlines + TL - 1) / TL), (unsigned)(a.nfields > 1 ? a.nfields : 1));
     size_t lds = sizeof(double) * 2 * kMaxP * TL;
     if (AX == 0) lds += sizeof(double) * (size_t)a.N * (TL + 1);
     if (MIX) lds += sizeof(double) * 2 * kMaxP * TL;
@@ -518,6 +523,7 @@ static int launch_axis(int axis, const LineArgs& a, hipStream_t s) {
 
 int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s) {
     if (a.P < 1 || a.P > kMaxP) return fail(CS_ERR_SHAPE, "chunk count out of range");
+    if (a.nfields > 1 && (a.drva || a.recv)) return fail(CS_ERR_ARG, "multi-field launches write no boundary data");
     if ((long)a.P * K < a.N) return fail(CS_ERR_SHAPE, "chunks do not cover the line");
     if (axis == 0 && (size_t)a.N * (kXLines + 1) * 8 + 4 * kMaxP * kXLines * 8 > 160 * 1024)
         return fail(CS_ERR_SHAPE, "x line too long for the LDS tile");
@@ -693,7 +699,7 @@ int get_line_op(cs_numerics* num, int op, int axis, int start, int end, LineOpTa
 }
 
 int line_apply(cs_numerics* num, int op, int axis, const double* in, double* out, int start, int end,
-               const double* recv, int var, bool write_drva, hipStream_t s) {
+               const double* recv, int var, bool write_drva, hipStream_t s, int nfields) {
     if (axis < 0 || axis > 2) return fail(CS_ERR_ARG, "axis must be 0, 1 or 2");
     if (!in || !out) return fail(CS_ERR_ARG, "null field pointer");
     if (op == OP_DERIV && (start == 1 || end == 1) && !num->table_set)
@@ -720,6 +726,8 @@ int line_apply(cs_numerics* num, int op, int axis, const double* in, double* out
     a.cyclic = (op == OP_FILTER) ? (start == 1 && end == 1) : 0;
     a.tab32 = t->dev32;
     a.mixed = num->precision;
+    a.nfields = nfields;
+    a.fstride = num->g.lmx;
     return launch_line(op, axis, t->K, a, s);
 }
 

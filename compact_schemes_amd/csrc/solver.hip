@@ -202,12 +202,10 @@ int cs_solver_rk4_step(cs_solver* sv, double* qa, const double* sbcc, double dt,
     if ((rc = rhs_sweeps(sv, sv->qc, sbcc, sv->qb, qa, 2, s))) return rc;
     if ((rc = rhs_sweeps(sv, sv->qb, sbcc, qa, qa, 1, s))) return rc;
     if (sv->cfg.filter) {
-        for (int v = 0; v < 5; v++) {
-            double* f = qa + (long)v * sv->g.lmx;
-            for (int a = 0; a < 3; a++) {
-                const int sp = sv->ax[a].seam;
-                if ((rc = line_apply(sv->num, OP_FILTER, a, f, f, sp, sp, nullptr, 0, false, s))) return rc;
-            }
+        // the five conserved variables in one launch per axis
+        for (int a = 0; a < 3; a++) {
+            const int sp = sv->ax[a].seam;
+            if ((rc = line_apply(sv->num, OP_FILTER, a, qa, qa, sp, sp, nullptr, 0, false, s, 5))) return rc;
         }
     }
     if (dt_out) {
