@@ -67,6 +67,8 @@ struct S1 {
     static constexpr int NF = 4;
     static constexpr bool kGcbc = false;   // gradients only; BC updates act on the flux sweeps
     static constexpr bool kPv = CLOSE;     // the closure needs u, v, w, c^2 at its rows
+    static constexpr int kH = 2;           // derivative stencil half-width
+    static constexpr bool kFilter = false;
     const double* q[5];
     double* gout[4];          // !CLOSE: d/d(axis AX) of u, v, w, c^2
     const double* gin[3][4];  // CLOSE: stored gradients along the other two axes
@@ -118,6 +120,8 @@ struct S2 {
     static constexpr int NF = 5;
     static constexpr bool kGcbc = true;
     static constexpr bool kPv = false;
+    static constexpr int kH = 2;
+    static constexpr bool kFilter = false;
     const double* q[5];
     const double* tr[3];      // row AX of the stress tensor
     const double* hx;         // heat flux component AX
@@ -188,6 +192,28 @@ struct S2 {
     }
 };
 
+// SF: the compact filter of the five conserved variables along AX, in place
+// (the filter of cs_filter: 10th-order interior rows, reduced-order centred
+// rows near closure ends, identity end rows; cyclic on seam axes)
+template <int AX>
+struct SF {
+    static constexpr int NF = 5;
+    static constexpr bool kGcbc = false;
+    static constexpr bool kPv = false;
+    static constexpr int kH = 5;
+    static constexpr bool kFilter = true;
+    double* q[5];
+    __device__ __forceinline__ void point(uoff bo, double* v) const {
+#pragma unroll
+        for (int f = 0; f < 5; f++) v[f] = ldg(q[f], bo);
+    }
+    __device__ __forceinline__ void store(uoff bo, const double* d, const double*) const {
+#pragma unroll
+        for (int f = 0; f < 5; f++) stg(q[f], bo, d[f]);
+    }
+    __device__ __forceinline__ void gcbc(uoff, int, const double*, double*) const {}
+};
+
 // ---------------------------------------------------------------------------
 // pieces shared by the sweep kernels
 // ---------------------------------------------------------------------------
@@ -234,6 +260,58 @@ __device__ __forceinline__ void window_rhs(const double (&w)[NF][W], double (&r)
                 if (t == L - 2) r[f][t] = r2;
             }
         }
+    }
+}
+
+// filter rows from a window w[f][u] holding rows s-5+u: 10th order inside,
+// order 2*dist at distance dist < 5 from a closure end, identity at the end
+// (an = the order-2M coefficient sets, an[6M..6M+M]); cyclic lines are all
+// inside rows
+template <int K, int NF, int W>
+__device__ __forceinline__ void filter_rhs(const double (&w)[NF][W], double (&r)[NF][K], int s, int N, bool cyclic,
+                                           const double* an) {
+    constexpr int H = 5;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        const int j = s + t;
+        const int dist = cyclic ? H : min(j, N - 1 - j);
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            double v = an[30] * w[f][t + H];
+#pragma unroll
+            for (int m = 1; m <= 5; m++) v += 0.5 * an[30 + m] * (w[f][t + H + m] + w[f][t + H - m]);
+            r[f][t] = v;
+        }
+        if (dist < H) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                double v = w[f][t + H];
+                if (dist > 0) {
+                    const double* cf = an + 6 * dist;
+                    v = cf[0] * w[f][t + H];
+#pragma unroll
+                    for (int m = 1; m < H; m++)
+                        if (m <= dist) v += 0.5 * cf[m] * (w[f][t + H + m] + w[f][t + H - m]);
+                }
+                r[f][t] = v;
+            }
+        }
+    }
+}
+
+// Sherman-Morrison correction of a cyclic filter solve: y0 and yN1 are the
+// solution of the modified system at rows 0 and N-1 (tab layout of
+// build_line_table: z at 5N, 1/(1 + z0 + kappa zN1) at 6N, kappa at 6N+1)
+template <int K, int NF>
+__device__ __forceinline__ void cyclic_correction(const double* tab, int N, int s, const double (&y0)[NF],
+                                                  const double (&yN1)[NF], double (&r)[NF][K]) {
+    const double* z = tab + 5 * N;
+    const double inv = tab[6 * N], kappa = tab[6 * N + 1];
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        const double beta = (y0[f] + kappa * yN1[f]) * inv;
+#pragma unroll
+        for (int t = 0; t < K; t++) r[f][t] -= beta * z[s + t];
     }
 }
 
@@ -519,7 +597,9 @@ template <int K, int AX, int TL, class SW, bool GCBC>
 __global__ void __launch_bounds__(AX == 0 ? 256 : TL * kMaxP) __attribute__((amdgpu_waves_per_eu(AX == 0 ? 1 : 3)))
 direct_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
     constexpr int NF = SW::NF;
-    constexpr int W = K + 4;
+    constexpr int H = SW::kH;
+    constexpr int W = K + 2 * H;
+    static_assert(H <= K, "halo rows come from the neighbouring chunks only");
     extern __shared__ double lds[];
 
     const int N = ax.N, P = ax.P;
@@ -542,7 +622,8 @@ direct_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
     const int s = c * K;
     const uoff st = (uoff)ax.st * 8u;
     const uoff base = line_off(AX, act ? (uoff)line : 0u, (uoff)cm.lxi, (uoff)cm.plane);
-    const double* tab = ax.tab;
+    const double* tab = SW::kFilter ? ax.ftab : ax.tab;
+    const bool cyc = SW::kFilter && ax.seam;
 
     double w[NF][W];
 #pragma unroll
@@ -555,7 +636,7 @@ direct_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
             for (int f = 0; f < NF; f++) x[f] = 0.0;
         }
 #pragma unroll
-        for (int f = 0; f < NF; f++) w[f][t + 2] = x[f];
+        for (int f = 0; f < NF; f++) w[f][t + H] = x[f];
     }
     if (AX == 0) {
 #pragma unroll
@@ -564,6 +645,25 @@ direct_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
             w[f][1] = __shfl_up(w[f][K + 1], 1, GS);
             w[f][K + 2] = __shfl_down(w[f][2], 1, GS);
             w[f][K + 3] = __shfl_down(w[f][3], 1, GS);
+        }
+    } else if constexpr (H != 2) {
+        // halo rows through LDS one field at a time: hb[k*P + c][TL], k = the
+        // first H then the last H rows of chunk c (neighbours wrap on cyclic lines)
+        const int cl = c > 0 ? c - 1 : (cyc ? P - 1 : 0), cr = c + 1 < P ? c + 1 : (cyc ? 0 : P - 1);
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            if (f > 0) __syncthreads();
+#pragma unroll
+            for (int k = 0; k < H; k++) {
+                lds[(k * P + c) * TL + lloc] = w[f][H + k];
+                lds[((H + k) * P + c) * TL + lloc] = w[f][K + k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < H; k++) {
+                w[f][k] = lds[((H + k) * P + cl) * TL + lloc];
+                w[f][K + H + k] = lds[(k * P + cr) * TL + lloc];
+            }
         }
     } else {
         // halo rows through LDS: hb[(f*4 + k)*P + c][TL], k = chunk rows 0, 1, K-2, K-1
@@ -592,8 +692,13 @@ direct_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
             for (int t = 0; t < K; t++) pv[f][t] = w[f][t + 2];
     }
     double r[NF][K];
-    window_rhs<K, NF, W>(w, r, K, work && c == 0 && !ax.seam, work && c == P - 1 && !ax.seam);
-    if (work && ax.seam && (c == 0 || c == P - 1)) seam_rows<K, NF>(sw, base, st, N, K, c == 0, c == P - 1, cm.pb, r);
+    if constexpr (SW::kFilter) {
+        filter_rhs<K, NF, W>(w, r, s, N, cyc, ax.ftab + 6 * N + 8);
+    } else {
+        window_rhs<K, NF, W>(w, r, K, work && c == 0 && !ax.seam, work && c == P - 1 && !ax.seam);
+        if (work && ax.seam && (c == 0 || c == P - 1))
+            seam_rows<K, NF>(sw, base, st, N, K, c == 0, c == P - 1, cm.pb, r);
+    }
 
     double carry[NF], G[NF], X[NF];
     forward_local<K, NF, true>(tab, N, s, K, r, carry);
@@ -618,6 +723,26 @@ direct_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
         fold_backward<NF, true, K>(xb, tab, N, P, TL, c, lloc, X);
     }
     apply_carry<K, NF, true>(tab, 4 * N, s, K, r, X);
+    if (AX != 0 && cyc) {
+        // rows 0 and N-1 of the modified system through LDS (after the folds' reads)
+        double* ends = lds + 2 * NF * P * TL;
+        if (c == 0) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) ends[f * TL + lloc] = r[f][0];
+        }
+        if (c == P - 1) {
+#pragma unroll
+            for (int f = 0; f < NF; f++) ends[(NF + f) * TL + lloc] = r[f][K - 1];
+        }
+        __syncthreads();
+        double y0[NF], yN1[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            y0[f] = ends[f * TL + lloc];
+            yN1[f] = ends[(NF + f) * TL + lloc];
+        }
+        cyclic_correction<K, NF>(tab, N, s, y0, yN1, r);
+    }
 
     if (!work) return;
     face_corrections<K, NF, SW, GCBC, true>(sw, base, st, N, s, K, c == 0, c == P - 1, r, cm.sbcc);
@@ -646,7 +771,9 @@ template <int K, class SW, bool GCBC>
 __global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
     constexpr int NF = SW::NF;
     constexpr int J = K;                  // points per lane: (64/GS)*P*K <= 64*K
-    constexpr int W = K + 4;
+    constexpr int H = SW::kH;
+    constexpr int W = K + 2 * H;
+    static_assert(H <= K, "halo rows come from the neighbouring chunks only");
     extern __shared__ double lds[];       // [64/GS lines][P chunks][K+1]
 
     const int N = ax.N, P = ax.P, GS = ax.GS;
@@ -691,10 +818,14 @@ __global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW 
     const bool work = li < nl && c < P;
     const int lr = li < nl ? li : 0;
     const int cc = c < P ? c : P - 1;
-    const int cl = cc > 0 ? cc - 1 : 0, cr = cc + 1 < P ? cc + 1 : P - 1;
     const int s = cc * K;
-    const double* tab = ax.tab;
+    const double* tab = SW::kFilter ? ax.ftab : ax.tab;
     const uoff base = run + (uoff)lr * (uoff)N * st;
+    const bool cyc = SW::kFilter && ax.seam;
+    // halo chunks: the neighbours, wrapped on a cyclic line (other lines clamp;
+    // their values there are not used)
+    const int hl = cc > 0 ? cc - 1 : (cyc ? P - 1 : 0);
+    const int hr = cc + 1 < P ? cc + 1 : (cyc ? 0 : P - 1);
 
     double r[NF][K];
 #pragma unroll
@@ -705,18 +836,23 @@ __global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW 
             if (lane + 64 * j < npts) lds[slot[j]] = xq[f][j];
         __syncthreads();
         double w[1][W];
-        w[0][0] = lds[xt_slot<K>(lr, P, cl, K - 2)];
-        w[0][1] = lds[xt_slot<K>(lr, P, cl, K - 1)];
 #pragma unroll
-        for (int t = 0; t < K; t++) w[0][t + 2] = lds[xt_slot<K>(lr, P, cc, t)];
-        w[0][K + 2] = lds[xt_slot<K>(lr, P, cr, 0)];
-        w[0][K + 3] = lds[xt_slot<K>(lr, P, cr, 1)];
+        for (int u = 0; u < H; u++) {
+            w[0][u] = lds[xt_slot<K>(lr, P, hl, K - H + u)];
+            w[0][K + H + u] = lds[xt_slot<K>(lr, P, hr, u)];
+        }
+#pragma unroll
+        for (int t = 0; t < K; t++) w[0][t + H] = lds[xt_slot<K>(lr, P, cc, t)];
         double rf[1][K];
-        window_rhs<K, 1, W>(w, rf, K, work && c == 0 && !ax.seam, work && c == P - 1 && !ax.seam);
+        if constexpr (SW::kFilter)
+            filter_rhs<K, 1, W>(w, rf, s, N, cyc, ax.ftab + 6 * N + 8);
+        else
+            window_rhs<K, 1, W>(w, rf, K, work && c == 0 && !ax.seam, work && c == P - 1 && !ax.seam);
 #pragma unroll
         for (int t = 0; t < K; t++) r[f][t] = rf[0][t];
     }
-    if (work && ax.seam && (c == 0 || c == P - 1)) seam_rows<K, NF>(sw, base, st, N, K, c == 0, c == P - 1, cm.pb, r);
+    if constexpr (!SW::kFilter)
+        if (work && ax.seam && (c == 0 || c == P - 1)) seam_rows<K, NF>(sw, base, st, N, K, c == 0, c == P - 1, cm.pb, r);
 
     const bool cv = c < P;
     double carry[NF], G[NF], X[NF];
@@ -726,6 +862,15 @@ __global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW 
     backward_local<K, NF, true>(tab, N, s, K, r, carry);
     scan_backward<NF>(cv ? tab[4 * N + s] : 0.0, carry, cv, c, P, GS, X);
     apply_carry<K, NF, true>(tab, 4 * N, s, K, r, X);
+    if (cyc) {
+        double y0[NF], yN1[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            y0[f] = __shfl(r[f][0], 0, GS);
+            yN1[f] = __shfl(r[f][K - 1], P - 1, GS);
+        }
+        cyclic_correction<K, NF>(tab, N, s, y0, yN1, r);
+    }
     if (work) face_corrections<K, NF, SW, GCBC, true>(sw, base, st, N, s, K, c == 0, c == P - 1, r, cm.sbcc);
 
     // back to the coalesced layout, one field at a time (the point values
@@ -1066,6 +1211,32 @@ static int launch_sweep(const AxisDesc& ax, const SweepCommon& cm, const SW& sw,
     return CS_OK;
 }
 
+// the five-field filter sweep: equal 8-point chunks only (filter_fusable)
+template <int AX>
+static int launch_filter(const AxisDesc& ax, const SweepCommon& cm, const SF<AX>& sw, hipStream_t s) {
+    constexpr int K = 8, NF = 5, H = SF<AX>::kH;
+    if (ax.N != ax.P * K || !ax.ftab) return fail(CS_ERR_STATE, "fused filter needs equal 8-point chunks");
+    if constexpr (AX == 0) {
+        const int lpw = 64 / ax.GS;
+        const size_t lds = sizeof(double) * lpw * ax.P * (K + 1);
+        dim3 block(64), grid((unsigned)((ax.nlines + lpw - 1) / lpw));
+        hipLaunchKernelGGL((xt_kernel<K, SF<AX>, false>), grid, block, lds, s, ax, cm, sw);
+    } else {
+        const int TL = ax.tl == 8 ? 8 : 16;
+        const size_t halo = (size_t)2 * H * ax.P * TL;
+        const size_t carries = (size_t)2 * NF * ax.P * TL + 2 * NF * TL;
+        const size_t lds = sizeof(double) * (halo > carries ? halo : carries);
+        dim3 block(TL * ax.P), grid((unsigned)((ax.nlines + TL - 1) / TL));
+        const void* k = TL == 8 ? (const void*)direct_kernel<K, AX, 8, SF<AX>, false>
+                                : (const void*)direct_kernel<K, AX, 16, SF<AX>, false>;
+        if (lds > 65536) CS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (TL == 8) hipLaunchKernelGGL((direct_kernel<K, AX, 8, SF<AX>, false>), grid, block, lds, s, ax, cm, sw);
+        else hipLaunchKernelGGL((direct_kernel<K, AX, 16, SF<AX>, false>), grid, block, lds, s, ax, cm, sw);
+    }
+    CS_HIP(hipGetLastError());
+    return CS_OK;
+}
+
 template <int AX, class SW>
 static int dispatch(const AxisDesc& ax, const SweepCommon& cm, const SW& sw, hipStream_t s) {
     if constexpr (SW::kGcbc) {
@@ -1267,13 +1438,42 @@ int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, 
 // One sweep of the right-hand side on the solver's scratch buffers (0..5 =
 // the six sweeps in launch order, the last in RHS mode; 6 = the last sweep with
 // the RK stage update out = q - dt*J*RHS of the last RK4 stage, dt from the
-// device time-step slot); for timing single kernels.
+// device time-step slot; 7 = the filter of the five variables along x, y, z
+// into out); for timing single kernels.
 int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, double* out, hipStream_t s) {
     const SweepCommon cm = common_of(sv, sbcc);
-    if (which < 0 || which > 6) return fail(CS_ERR_ARG, "sweep index out of 0..6");
+    if (which < 0 || which > 7) return fail(CS_ERR_ARG, "sweep index out of 0..7");
+    if (which == 7) {
+        if (!filter_fusable(sv)) return fail(CS_ERR_UNSUPPORTED, "the fused filter needs equal 8-point chunks");
+        if (out != q) CS_HIP(hipMemcpyAsync(out, q, sizeof(double) * 5 * sv->g.lmx, hipMemcpyDeviceToDevice, s));
+        return filter_sweeps(sv, out, s);
+    }
     if (which < 3 && !sv->cfg.viscous) return fail(CS_ERR_STATE, "gradient sweeps exist only for the viscous RHS");
     if (which == 6) return sweep_at(sv, 5, cm, q, out, q, 1, s);
     return sweep_at(sv, which, cm, q, out, nullptr, 0, s);
+}
+
+bool filter_fusable(const cs_solver* sv) {
+    for (int a = 0; a < 3; a++)
+        if (sv->ax[a].K != 8 || sv->ax[a].N != sv->ax[a].P * 8 || !sv->ax[a].ftab) return false;
+    return true;
+}
+
+// the compact filter of the five conserved variables along x, y, z in place,
+// one launch per axis
+int filter_sweeps(cs_solver* sv, double* q, hipStream_t s) {
+    const SweepCommon cm = common_of(sv, nullptr);
+    const long lmx = sv->g.lmx;
+    int rc;
+    SF<0> f0{};
+    SF<1> f1{};
+    SF<2> f2{};
+    fields(q, lmx, f0.q);
+    fields(q, lmx, f1.q);
+    fields(q, lmx, f2.q);
+    if ((rc = launch_filter<0>(sv->ax[0], cm, f0, s))) return rc;
+    if ((rc = launch_filter<1>(sv->ax[1], cm, f1, s))) return rc;
+    return launch_filter<2>(sv->ax[2], cm, f2, s);
 }
 
 int time_step(cs_solver* sv, const double* q, double dt_fixed, hipStream_t s) {

@@ -14,6 +14,7 @@ constexpr int kMaxP = 32;     // chunks per line
 // one axis: derivative operator with balanced chunks
 struct AxisDesc {
     const double* tab;   // [5][N]: nu, mu, rho, Mf, Rb
+    const double* ftab;  // filter operator (build_line_table layout, 8-point chunks), or null
     int N, P, K, GS;     // GS: lanes per line in the x layout (pow2 >= P)
     long nlines;
     long st;
@@ -41,6 +42,7 @@ struct cs_solver {
     cs_numerics* num = nullptr;         // interface table, filter operators
     cs::fused::AxisDesc ax[3];
     double* tabs[3] = {nullptr, nullptr, nullptr};
+    double* ftabs[3] = {nullptr, nullptr, nullptr};
     int order = 1;             // sweep order (cs_solver_set_order)
     double* ga = nullptr;      // [4][lmx] gradients of the first S1 sweep
     double* gb = nullptr;      // [4][lmx] gradients of the second S1 sweep
@@ -63,5 +65,7 @@ int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, 
 int time_step(cs_solver* sv, const double* q, double dt_fixed, hipStream_t s);
 int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, double* out, hipStream_t s);
 int build_axis_table(int n, int seam, int K, int P, std::vector<double>& h);
+bool filter_fusable(const cs_solver* sv);
+int filter_sweeps(cs_solver* sv, double* q, hipStream_t s);
 }  // namespace fused
 }  // namespace cs

@@ -70,7 +70,7 @@ extern "C" {
 int cs_solver_destroy(cs_solver* sv) {
     if (!sv) return CS_OK;
     double* bufs[] = {sv->ga, sv->gb, sv->tau, sv->hf, sv->dpart, sv->qb, sv->qc, sv->red, sv->dt, sv->vjp,
-                      sv->tabs[0], sv->tabs[1], sv->tabs[2]};
+                      sv->tabs[0], sv->tabs[1], sv->tabs[2], sv->ftabs[0], sv->ftabs[1], sv->ftabs[2]};
     for (double* p : bufs)
         if (p) (void)hipFree(p);
     if (sv->dt_host) (void)hipHostFree(sv->dt_host);
@@ -126,6 +126,22 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
             return fail(CS_ERR_HIP, "operator table upload failed");
         }
         ax.tab = sv->tabs[a];
+        ax.ftab = nullptr;
+        if (cfg->filter && ax.K == 8 && ax.N == ax.P * 8) {
+            // the filter of the five variables in one sweep per axis (8-point chunks)
+            std::vector<double> hf;
+            const int sp = ax.seam;
+            if ((rc = build_line_table(OP_FILTER, ax.N, sp, sp, cfg->alpha_f, 8, hf))) {
+                cs_solver_destroy(sv);
+                return rc;
+            }
+            if (hipMalloc(&sv->ftabs[a], hf.size() * sizeof(double)) != hipSuccess ||
+                hipMemcpy(sv->ftabs[a], hf.data(), hf.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+                cs_solver_destroy(sv);
+                return fail(CS_ERR_HIP, "filter table upload failed");
+            }
+            ax.ftab = sv->ftabs[a];
+        }
     }
     sv->red_blocks = (int)std::min<long>(2048, (lmx + 255) / 256);
     bool ok = hipMalloc(&sv->dpart, sizeof(double) * 5 * lmx) == hipSuccess &&
@@ -201,7 +217,9 @@ int cs_solver_rk4_step(cs_solver* sv, double* qa, const double* sbcc, double dt,
     if ((rc = rhs_sweeps(sv, sv->qb, sbcc, sv->qc, qa, 3, s))) return rc;
     if ((rc = rhs_sweeps(sv, sv->qc, sbcc, sv->qb, qa, 2, s))) return rc;
     if ((rc = rhs_sweeps(sv, sv->qb, sbcc, qa, qa, 1, s))) return rc;
-    if (sv->cfg.filter) {
+    if (sv->cfg.filter && filter_fusable(sv)) {
+        if ((rc = filter_sweeps(sv, qa, s))) return rc;
+    } else if (sv->cfg.filter) {
         // the five conserved variables in one launch per axis
         for (int a = 0; a < 3; a++) {
             const int sp = sv->ax[a].seam;

@@ -197,7 +197,9 @@ int cs_solver_destroy(cs_solver* sv);
 int cs_solver_rhs(cs_solver* sv, const double* qa, double* de, const double* sbcc, cs_stream stream);
 /* one of the six sweeps of cs_solver_rhs (0..5 = gradients x, y, z, flux
  * divergence x, y, z; 6 = flux divergence z fused with the last RK4 stage
- * update de = qa - dt*yaco*RHS, dt from the last time step, 0 before any)
+ * update de = qa - dt*yaco*RHS, dt from the last time step, 0 before any;
+ * 7 = the compact filter of the five variables of qa along x, y, z into de,
+ * three launches, when every axis has 8-point chunks)
  * on the solver's scratch state; no reference counterpart -- it exists to
  * time and profile single kernels.  Sweeps depend on the outputs of the
  * earlier ones of the same RHS. */
