@@ -103,7 +103,7 @@ def main():
         rhs = lambda: stepper.rhs(q)
         num = stepper.num
     else:
-        from compact_schemes_amd.fused import FusedSolver
+        from compact_schemes_amd.fused import FILTER_SWEEPS, FusedSolver
         solver = FusedSolver(dims, wl.NBC_NON_REFLECTIVE, st)
         step = lambda: solver.rk4_step(q)
         rhs = lambda: solver.rhs(q)
@@ -154,6 +154,10 @@ def main():
         q.copy_(q_init)
         kern = solver.time_dominant_kernel(q, stream)
         kern["traffic"] = traffic_from_profile(kern["kernel"], n)
+        if solver.filter_fused:                   # the three filter sweeps that close each step
+            kern["all"] += [{"sweep": r["name"], "kernel": r["kernel"], "ms": round(r["ms"], 5),
+                             "GBs": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 1)}
+                            for r in solver.time_sweeps(q, stream, reps=5, sweeps=FILTER_SWEEPS)]
 
     achieved = kern["bytes"] / (kern["ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -552,7 +556,7 @@ def pmc_probe(args):
     launches each, so that per-dispatch counters are easy to attribute."""
     import torch
     from compact_schemes_amd import workloads as wl
-    from compact_schemes_amd.fused import FusedSolver, RK_STAGE_SWEEPS, sweep_names
+    from compact_schemes_amd.fused import FILTER_SWEEPS, FusedSolver, RK_STAGE_SWEEPS, sweep_names
     n = args.n
     dims = (n, n, n)
     lmx = n ** 3
@@ -566,6 +570,10 @@ def pmc_probe(args):
     for w in RK_STAGE_SWEEPS:
         for _ in range(args.reps):
             solver.sweep(w, q, stream=stream)
+    solver.de.copy_(q)
+    for w in FILTER_SWEEPS:                   # the filter sweeps, in place on the scratch field
+        for _ in range(args.reps):
+            solver.sweep(w, solver.de, solver.de, stream=stream)
     src = torch.ones(5 * lmx, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(args.reps):

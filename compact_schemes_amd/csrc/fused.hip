@@ -214,6 +214,23 @@ struct SF {
     __device__ __forceinline__ void gcbc(uoff, int, const double*, double*) const {}
 };
 
+// SF1: the same filter, one conserved variable per workgroup (blockIdx.y picks
+// the variable of the five stored lmx points apart); a fifth of SF's registers
+template <int AX>
+struct SF1 {
+    static constexpr int NF = 1;
+    static constexpr bool kGcbc = false;
+    static constexpr bool kPv = false;
+    static constexpr int kH = 5;
+    static constexpr bool kFilter = true;
+    double* q;
+    long lmx;
+    __device__ __forceinline__ double* field() const { return q + (long)blockIdx.y * lmx; }
+    __device__ __forceinline__ void point(uoff bo, double* v) const { v[0] = ldg(field(), bo); }
+    __device__ __forceinline__ void store(uoff bo, const double* d, const double*) const { stg(field(), bo, d[0]); }
+    __device__ __forceinline__ void gcbc(uoff, int, const double*, double*) const {}
+};
+
 // ---------------------------------------------------------------------------
 // pieces shared by the sweep kernels
 // ---------------------------------------------------------------------------
@@ -1211,27 +1228,30 @@ static int launch_sweep(const AxisDesc& ax, const SweepCommon& cm, const SW& sw,
     return CS_OK;
 }
 
-// the five-field filter sweep: equal 8-point chunks only (filter_fusable)
-template <int AX>
-static int launch_filter(const AxisDesc& ax, const SweepCommon& cm, const SF<AX>& sw, hipStream_t s) {
-    constexpr int K = 8, NF = 5, H = SF<AX>::kH;
+// the filter sweep: equal 8-point chunks only (filter_fusable); SW = SF<AX>
+// (five variables per workgroup) or SF1<AX> (one variable per workgroup,
+// grid.y = 5)
+template <int AX, class SW>
+static int launch_filter(const AxisDesc& ax, const SweepCommon& cm, const SW& sw, hipStream_t s) {
+    constexpr int K = 8, NF = SW::NF, H = SW::kH;
+    const unsigned gy = NF == 1 ? 5u : 1u;
     if (ax.N != ax.P * K || !ax.ftab) return fail(CS_ERR_STATE, "fused filter needs equal 8-point chunks");
     if constexpr (AX == 0) {
         const int lpw = 64 / ax.GS;
         const size_t lds = sizeof(double) * lpw * ax.P * (K + 1);
-        dim3 block(64), grid((unsigned)((ax.nlines + lpw - 1) / lpw));
-        hipLaunchKernelGGL((xt_kernel<K, SF<AX>, false>), grid, block, lds, s, ax, cm, sw);
+        dim3 block(64), grid((unsigned)((ax.nlines + lpw - 1) / lpw), gy);
+        hipLaunchKernelGGL((xt_kernel<K, SW, false>), grid, block, lds, s, ax, cm, sw);
     } else {
-        const int TL = ax.tl == 8 ? 8 : 16;
+        const int TL = ax.ftl == 8 ? 8 : 16;
         const size_t halo = (size_t)2 * H * ax.P * TL;
         const size_t carries = (size_t)2 * NF * ax.P * TL + 2 * NF * TL;
         const size_t lds = sizeof(double) * (halo > carries ? halo : carries);
-        dim3 block(TL * ax.P), grid((unsigned)((ax.nlines + TL - 1) / TL));
-        const void* k = TL == 8 ? (const void*)direct_kernel<K, AX, 8, SF<AX>, false>
-                                : (const void*)direct_kernel<K, AX, 16, SF<AX>, false>;
+        dim3 block(TL * ax.P), grid((unsigned)((ax.nlines + TL - 1) / TL), gy);
+        const void* k = TL == 8 ? (const void*)direct_kernel<K, AX, 8, SW, false>
+                                : (const void*)direct_kernel<K, AX, 16, SW, false>;
         if (lds > 65536) CS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        if (TL == 8) hipLaunchKernelGGL((direct_kernel<K, AX, 8, SF<AX>, false>), grid, block, lds, s, ax, cm, sw);
-        else hipLaunchKernelGGL((direct_kernel<K, AX, 16, SF<AX>, false>), grid, block, lds, s, ax, cm, sw);
+        if (TL == 8) hipLaunchKernelGGL((direct_kernel<K, AX, 8, SW, false>), grid, block, lds, s, ax, cm, sw);
+        else hipLaunchKernelGGL((direct_kernel<K, AX, 16, SW, false>), grid, block, lds, s, ax, cm, sw);
     }
     CS_HIP(hipGetLastError());
     return CS_OK;
@@ -1439,14 +1459,14 @@ int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, 
 // the six sweeps in launch order, the last in RHS mode; 6 = the last sweep with
 // the RK stage update out = q - dt*J*RHS of the last RK4 stage, dt from the
 // device time-step slot; 7 = the filter of the five variables along x, y, z
-// into out); for timing single kernels.
+// into out, 8/9/10 = that filter along x/y/z only); for timing single kernels.
 int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, double* out, hipStream_t s) {
     const SweepCommon cm = common_of(sv, sbcc);
-    if (which < 0 || which > 7) return fail(CS_ERR_ARG, "sweep index out of 0..7");
-    if (which == 7) {
+    if (which < 0 || which > 10) return fail(CS_ERR_ARG, "sweep index out of 0..10");
+    if (which >= 7) {
         if (!filter_fusable(sv)) return fail(CS_ERR_UNSUPPORTED, "the fused filter needs equal 8-point chunks");
         if (out != q) CS_HIP(hipMemcpyAsync(out, q, sizeof(double) * 5 * sv->g.lmx, hipMemcpyDeviceToDevice, s));
-        return filter_sweeps(sv, out, s);
+        return filter_sweeps(sv, out, s, which == 7 ? -1 : which - 8);
     }
     if (which < 3 && !sv->cfg.viscous) return fail(CS_ERR_STATE, "gradient sweeps exist only for the viscous RHS");
     if (which == 6) return sweep_at(sv, 5, cm, q, out, q, 1, s);
@@ -1459,21 +1479,27 @@ bool filter_fusable(const cs_solver* sv) {
     return true;
 }
 
-// the compact filter of the five conserved variables along x, y, z in place,
-// one launch per axis
-int filter_sweeps(cs_solver* sv, double* q, hipStream_t s) {
+template <int AX>
+static int filter_axis(cs_solver* sv, double* q, const SweepCommon& cm, hipStream_t s) {
+    const AxisDesc& ax = sv->ax[AX];
+    if (ax.ffields == 1) {
+        SF1<AX> f{q, sv->g.lmx};
+        return launch_filter<AX>(ax, cm, f, s);
+    }
+    SF<AX> f{};
+    fields(q, sv->g.lmx, f.q);
+    return launch_filter<AX>(ax, cm, f, s);
+}
+
+// the compact filter of the five conserved variables in place: along x, y, z
+// (axis < 0) or along one axis, one launch per axis
+int filter_sweeps(cs_solver* sv, double* q, hipStream_t s, int axis) {
     const SweepCommon cm = common_of(sv, nullptr);
-    const long lmx = sv->g.lmx;
-    int rc;
-    SF<0> f0{};
-    SF<1> f1{};
-    SF<2> f2{};
-    fields(q, lmx, f0.q);
-    fields(q, lmx, f1.q);
-    fields(q, lmx, f2.q);
-    if ((rc = launch_filter<0>(sv->ax[0], cm, f0, s))) return rc;
-    if ((rc = launch_filter<1>(sv->ax[1], cm, f1, s))) return rc;
-    return launch_filter<2>(sv->ax[2], cm, f2, s);
+    int rc = CS_OK;
+    if ((axis < 0 || axis == 0) && (rc = filter_axis<0>(sv, q, cm, s))) return rc;
+    if ((axis < 0 || axis == 1) && (rc = filter_axis<1>(sv, q, cm, s))) return rc;
+    if (axis < 0 || axis == 2) rc = filter_axis<2>(sv, q, cm, s);
+    return rc;
 }
 
 int time_step(cs_solver* sv, const double* q, double dt_fixed, hipStream_t s) {

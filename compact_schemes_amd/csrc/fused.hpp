@@ -24,6 +24,11 @@ struct AxisDesc {
     int variant;         // 0: direct (ragged when N != P*K); 1: LDS row-block tiles
     int tl;              // lines per workgroup (y/z lines of variant 0, all of variant 1): 8 or 16
     int xcd;             // 1: neighbouring line groups go to workgroups on the same XCD
+    // filter sweep along this axis: 1 = one field per workgroup (grid.y over
+    // the five fields), 5 = all five fields per workgroup; lines per workgroup
+    // of the y/z filter sweep (8 or 16)
+    int ffields;
+    int ftl;
 };
 
 struct SweepCommon {
@@ -66,6 +71,6 @@ int time_step(cs_solver* sv, const double* q, double dt_fixed, hipStream_t s);
 int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, double* out, hipStream_t s);
 int build_axis_table(int n, int seam, int K, int P, std::vector<double>& h);
 bool filter_fusable(const cs_solver* sv);
-int filter_sweeps(cs_solver* sv, double* q, hipStream_t s);
+int filter_sweeps(cs_solver* sv, double* q, hipStream_t s, int axis = -1);
 }  // namespace fused
 }  // namespace cs

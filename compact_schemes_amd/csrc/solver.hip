@@ -118,6 +118,9 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
         ax.variant = 0;
         ax.tl = a == 1 ? 8 : 16;
         ax.xcd = a == 1 ? 1 : 0;
+        // filter sweeps: x all five variables per wave, y/z one per workgroup
+        ax.ffields = a == 0 ? 5 : 1;
+        ax.ftl = 16;
         std::vector<double> h;
         build_axis_table(ax.N, ax.seam, ax.K, ax.P, h);
         if (hipMalloc(&sv->tabs[a], h.size() * sizeof(double)) != hipSuccess ||
@@ -186,6 +189,15 @@ int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd) {
     ax.variant = variant;
     ax.tl = lines;
     ax.xcd = xcd ? 1 : 0;
+    return CS_OK;
+}
+
+int cs_solver_tune_filter(cs_solver* sv, int axis, int fields, int lines) {
+    if (!sv || axis < 0 || axis > 2) return fail(CS_ERR_ARG, "bad solver or axis");
+    if ((fields != 1 && fields != 5) || (lines != 8 && lines != 16))
+        return fail(CS_ERR_ARG, "fields 1/5, lines 8/16");
+    sv->ax[axis].ffields = fields;
+    sv->ax[axis].ftl = lines;
     return CS_OK;
 }
 

@@ -22,13 +22,17 @@ def sweep_names(order):
     a1, a2 = S1_AXES[order], S2_AXES[order]
     x = "xyz"
     return [f"S1{x[a1[0]]} grad", f"S1{x[a1[1]]} grad", f"S1{x[a1[2]]} grad+stress", f"S2{x[a2[0]]} flux",
-            f"S2{x[a2[1]]} flux", f"S2{x[a2[2]]} flux", f"S2{x[a2[2]]} flux+RK stage"]
+            f"S2{x[a2[1]]} flux", f"S2{x[a2[2]]} flux", f"S2{x[a2[2]]} flux+RK stage", "filter x+y+z (3 launches)",
+            "filter x", "filter y", "filter z"]
 
 
 SWEEP_NAMES = sweep_names(1)
 SWEEP_BYTES_VISC = [8 * (5 + 4), 8 * (5 + 4), 8 * (5 + 8 + 9), 8 * (5 + 4 + 5), 8 * (5 + 4 + 5 + 5),
-                    8 * (5 + 4 + 5 + 5), 8 * (5 + 4 + 5 + 5 + 5)]
-SWEEP_BYTES_EULER = [0, 0, 0, 8 * (5 + 5), 8 * (5 + 5 + 5), 8 * (5 + 5 + 5), 8 * (5 + 5 + 5 + 5)]
+                    8 * (5 + 4 + 5 + 5), 8 * (5 + 4 + 5 + 5 + 5), 3 * 8 * (5 + 5)] + [8 * (5 + 5)] * 3
+SWEEP_BYTES_EULER = [0, 0, 0, 8 * (5 + 5), 8 * (5 + 5 + 5), 8 * (5 + 5 + 5), 8 * (5 + 5 + 5 + 5),
+                     3 * 8 * (5 + 5)] + [8 * (5 + 5)] * 3
+# the filter sweeps (one per axis) that close an RK4 step
+FILTER_SWEEPS = [8, 9, 10]
 # the sweeps of one RK stage, in launch order
 RK_STAGE_SWEEPS = [0, 1, 2, 3, 4, 6]
 
@@ -77,7 +81,11 @@ class FusedSolver:
         self.num = None
         self.nbc_flat = flat
         self._tuned = {0: (0, 16, 0), 1: (0, 8, 1), 2: (0, 16, 0)}
+        self._ftuned = {0: (5, 16), 1: (1, 16), 2: (1, 16)}
         self.order = 1
+        # the filter of cs_solver_rk4_step runs as one sweep per axis when every
+        # axis splits into 8-point chunks (else one line-solver launch per axis)
+        self.filter_fused = bool(filt) and all(n <= 256 and n % 8 == 0 for n in self.dims)
 
     def close(self):
         if getattr(self, "h", None):
@@ -103,6 +111,10 @@ class FusedSolver:
         check(lib().cs_solver_tune(self.h, int(axis), int(variant), int(lines), int(xcd)))
         self._tuned[int(axis)] = (int(variant), int(lines), int(xcd))
 
+    def tune_filter(self, axis, fields, lines=16):
+        check(lib().cs_solver_tune_filter(self.h, int(axis), int(fields), int(lines)))
+        self._ftuned[int(axis)] = (int(fields), int(lines))
+
     def set_order(self, order):
         check(lib().cs_solver_set_order(self.h, int(order)))
         self.order = int(order)
@@ -110,6 +122,15 @@ class FusedSolver:
     def kernel_name(self, which):
         """Name of the HIP kernel instantiation a sweep launches (as profilers
         print it, without the argument list)."""
+        if which == 7:
+            return " + ".join(self.kernel_name(w) for w in FILTER_SWEEPS)
+        if which >= 8:
+            ax = which - 8
+            fields, lines = self._ftuned[ax]
+            sf = f"cs::fused::SF{'1' if fields == 1 else ''}<{ax}>"
+            if ax == 0:
+                return f"cs::fused::xt_kernel<8, {sf}, false>"
+            return f"cs::fused::direct_kernel<8, {ax}, {lines}, {sf}, false>"
         ax = S1_AXES[self.order][which] if which < 3 else S2_AXES[self.order][min(which, 5) - 3]
         n = self.dims[ax]
         K = 8 if n <= 256 else 16
@@ -168,10 +189,13 @@ class FusedSolver:
                 continue
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
-            self.sweep(w, q, stream=stream)
+            src = self.de if w >= 7 else q           # the filter runs in place on the scratch RHS
+            if w >= 7:
+                self.de.copy_(q)
+            self.sweep(w, src, stream=stream)
             e0.record(stream)
             for _ in range(reps):
-                self.sweep(w, q, stream=stream)
+                self.sweep(w, src, stream=stream)
             e1.record(stream)
             torch.cuda.synchronize()
             res.append({"which": w, "name": sweep_names(self.order)[w], "kernel": self.kernel_name(w),

@@ -195,11 +195,13 @@ int cs_solver_destroy(cs_solver* sv);
  * sbcc: [lmx] characteristic-update weights (read on the 4 rows next to each
  * non-reflective face). */
 int cs_solver_rhs(cs_solver* sv, const double* qa, double* de, const double* sbcc, cs_stream stream);
-/* one of the six sweeps of cs_solver_rhs (0..5 = gradients x, y, z, flux
- * divergence x, y, z; 6 = flux divergence z fused with the last RK4 stage
- * update de = qa - dt*yaco*RHS, dt from the last time step, 0 before any;
- * 7 = the compact filter of the five variables of qa along x, y, z into de,
- * three launches, when every axis has 8-point chunks)
+/* one of the six sweeps of cs_solver_rhs (0..5 = the six sweeps in launch
+ * order of cs_solver_set_order, three gradient sweeps then three flux
+ * divergence sweeps; 6 = the last sweep fused with the last RK4 stage update
+ * de = qa - dt*yaco*RHS, dt from the last time step, 0 before any; 7 = the
+ * compact filter of the five variables of qa along x, y, z into de, three
+ * launches, when every axis has 8-point chunks; 8, 9, 10 = that filter along
+ * x, y or z only)
  * on the solver's scratch state; no reference counterpart -- it exists to
  * time and profile single kernels.  Sweeps depend on the outputs of the
  * earlier ones of the same RHS. */
@@ -213,6 +215,12 @@ int cs_solver_sweep(cs_solver* sv, int which, const double* qa, double* de, cons
  * variant 0, all lines of variant 1); xcd = 1 keeps neighbouring line groups
  * on one XCD.  Results agree to rounding whatever the choice. */
 int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd);
+/* launch shape of the filter sweep along axis (no reference counterpart):
+ * fields = 1 filters one conserved variable per workgroup (the five variables
+ * spread over the grid; default along y and z), 5 all five per workgroup
+ * (default along x); lines = 8 or 16 lines per workgroup of the y/z sweeps
+ * (default 16).  Results agree to rounding whatever the choice. */
+int cs_solver_tune_filter(cs_solver* sv, int axis, int fields, int lines);
 /* order of the six sweeps (no reference counterpart): 0 = gradients along
  * x, y, then z with the stress closure, fluxes along x, y, then z with the
  * RHS/stage write; 1 (default) = gradients y, z, x (closure in the x sweep),

@@ -146,18 +146,32 @@ def test_fused_errors():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("dims,nbc", [((32, 24, 40), CL), ((40, 32, 48), MIX), ((24, 32, 16), PER)])
-def test_fused_filter_matches_oracle(oracle, dims, nbc):
-    """The five-field filter sweeps (x, y, z) against the oracle's filter of
-    each variable along x, then y, then z."""
+@pytest.mark.parametrize("shape", [(1, 16), (1, 8), (5, 16), (5, 8)], ids=lambda s: f"fields{s[0]}-lines{s[1]}")
+def test_fused_filter_matches_oracle(oracle, dims, nbc, shape):
+    """The filter sweeps (x, y, z) against the oracle's filter of each
+    variable along x, then y, then z, for each launch shape; the shapes agree
+    to rounding, and the three single-axis sweeps run in turn bit for bit."""
+    import torch
     from compact_schemes_amd.fused import FusedSolver
     st = _state(dims, seed=8)
     lmx = dims[0] * dims[1] * dims[2]
     sv = FusedSolver(dims, nbc, st)
     q = dev(st["qa"])
-    case(f"fused filter dims={dims} nbc={nbc[0]}")
-    out = torch_zeros_like(q)
+    case(f"fused filter dims={dims} nbc={nbc[0]} shape={shape}")
+    ref_sv = FusedSolver(dims, nbc, st)                # default launch shapes
+    want = torch.zeros_like(q)
+    ref_sv.sweep(7, q, want)
+    for a in range(3):
+        sv.tune_filter(a, *shape)
+    out = torch.zeros_like(q)
     sv.sweep(7, q, out)
     got = host(out)
+    f0 = st["qa"]
+    assert nrel(got - f0, host(want) - f0) < 5e-12     # the solve's FMA contraction differs by shape
+    one = q.clone()
+    for w in (8, 9, 10):
+        sv.sweep(w, one, one)
+    assert np.array_equal(host(one), got)
     ctx = oracle.Context(dims, nbc)
     for v in range(5):
         f = st["qa"][v * lmx:(v + 1) * lmx]
