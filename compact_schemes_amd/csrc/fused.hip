@@ -30,35 +30,14 @@
 //
 // Addressing: a field is at most 4 GiB, so point offsets are 32-bit byte
 // offsets added to the per-field base pointers the kernel arguments hold.
-#include "fused.hpp"
+#include "sweep_parts.hpp"
 
 namespace cs {
 namespace fused {
 
-typedef unsigned int uoff;    // byte offset inside one field
-
-__device__ __forceinline__ double ldg(const double* p, uoff bo) {
-    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(p) + bo);
-}
-__device__ __forceinline__ void stg(double* p, uoff bo, double v) {
-    *reinterpret_cast<double*>(reinterpret_cast<char*>(p) + bo) = v;
-}
-
 // ---------------------------------------------------------------------------
 // sweep functors
 // ---------------------------------------------------------------------------
-struct Phys {
-    cs_point umf;
-    double mu, J;
-    double m[3];   // metric scale per axis: xi_x, eta_y, zeta_z
-};
-
-__device__ __forceinline__ Prim prim_at(const double* const* q, uoff bo, double* qq) {
-#pragma unroll
-    for (int v = 0; v < 5; v++) qq[v] = ldg(q[v], bo);
-    return primitive(qq);
-}
-
 // S1: velocity and c^2 gradients along AX.  The first two S1 sweeps of an
 // RHS store their index-space gradients; the third (CLOSE) combines its own
 // gradients with the stored ones into the stress tensor and heat flux.
@@ -131,6 +110,10 @@ struct S2 {
     const double* dtp;
     int div;
     Phys ph;
+    double* rec;              // [2 face][5 var][face] boundary derivatives of this sweep, or null
+    long nface;
+    int lxi;
+    long plane;
 
     // the AX row of calc_fluxes_pre_compute_kernel with metric row m*e_AX
     // (src/cuda/kernels/physics.hpp:98-191); zero metric terms dropped
@@ -175,6 +158,13 @@ struct S2 {
     // boundary derivatives of this sweep (gcbc_setup_kernel +
     // gcbc_update_non_reflective_kernel, src/cuda/kernels/gcbc.hpp:251-454)
     __device__ __forceinline__ void gcbc(uoff bo, int face, const double* drv, double* corr) const {
+        if (rec) {
+            // face index of the face point (x: k*let+j, y: k*lxi+i, z: j*lxi+i)
+            const long o = bo / 8u;
+            const long fi = AX == 0 ? o / lxi : (AX == 1 ? (o / plane) * lxi + o % lxi : o % plane);
+#pragma unroll
+            for (int v = 0; v < 5; v++) rec[(face * 5 + v) * nface + fi] = drv[v];
+        }
         double qq[5];
         Prim p = prim_at(q, bo, qq);
         const double cm[3] = {AX == 0 ? 1.0 : 0.0, AX == 1 ? 1.0 : 0.0, AX == 2 ? 1.0 : 0.0};
@@ -234,14 +224,6 @@ struct SF1 {
 // ---------------------------------------------------------------------------
 // pieces shared by the sweep kernels
 // ---------------------------------------------------------------------------
-// byte offset of row 0 of line f of an axis (f runs over the face's points
-// with the first remaining index fastest)
-__device__ __forceinline__ uoff line_off(int axis, uoff f, uoff lxi, uoff plane) {
-    if (axis == 0) return f * lxi * 8u;
-    if (axis == 1) return ((f / lxi) * plane + (f % lxi)) * 8u;
-    return f * 8u;
-}
-
 // interior rows and closure rows from a window w[f][u] holding rows s-2+u
 // (build_system, numerics_device_pc.hpp:176-293)
 template <int K, int NF, int W>
@@ -431,136 +413,6 @@ __device__ __forceinline__ void seam_rows(const SW& sw, uoff base, uoff st, int 
     }
 }
 
-// recurrences of the solve (EX: every chunk has K rows)
-template <int K, int NF, bool EX>
-__device__ __forceinline__ void forward_local(const double* tab, int N, int s, int L, double (&r)[NF][K],
-                                              double (&carry)[NF]) {
-#pragma clang fp contract(fast)
-#pragma unroll
-    for (int f = 0; f < NF; f++) carry[f] = 0.0;
-#pragma unroll
-    for (int t = 0; t < K; t++) {
-        if (EX || t < L) {
-            const double nu = tab[s + t], mu = tab[N + s + t];
-#pragma unroll
-            for (int f = 0; f < NF; f++) { carry[f] = nu * r[f][t] + mu * carry[f]; r[f][t] = carry[f]; }
-        }
-    }
-}
-
-template <int K, int NF, bool EX>
-__device__ __forceinline__ void backward_local(const double* tab, int N, int s, int L, double (&r)[NF][K],
-                                               double (&carry)[NF]) {
-#pragma clang fp contract(fast)
-#pragma unroll
-    for (int f = 0; f < NF; f++) carry[f] = 0.0;
-#pragma unroll
-    for (int t = K - 1; t >= 0; t--) {
-        if (EX || t < L) {
-            const double rh = tab[2 * N + s + t];
-#pragma unroll
-            for (int f = 0; f < NF; f++) { carry[f] = r[f][t] + rh * carry[f]; r[f][t] = carry[f]; }
-        }
-    }
-}
-
-template <int K, int NF, bool EX>
-__device__ __forceinline__ void apply_carry(const double* tab, int off, int s, int L, double (&r)[NF][K],
-                                            const double (&G)[NF]) {
-#pragma clang fp contract(fast)
-#pragma unroll
-    for (int t = 0; t < K; t++) {
-        if (EX || t < L) {
-            const double M = tab[off + s + t];
-#pragma unroll
-            for (int f = 0; f < NF; f++) r[f][t] += M * G[f];
-        }
-    }
-}
-
-// carries across the chunks of a line through LDS: buf = [NF][P][TL]
-template <int NF, bool EX, int K>
-__device__ __forceinline__ void fold_forward(const double* buf, const double* tab, int N, int P, int TL, int c,
-                                             int lloc, double (&G)[NF]) {
-#pragma clang fp contract(fast)
-#pragma unroll
-    for (int f = 0; f < NF; f++) G[f] = 0.0;
-    for (int cc = 0; cc < c; cc++) {
-        const int e = EX ? cc * K + K - 1 : (int)(((long)(cc + 1) * N) / P) - 1;
-        const double M = tab[3 * N + e];
-#pragma unroll
-        for (int f = 0; f < NF; f++) G[f] = buf[(f * P + cc) * TL + lloc] + M * G[f];
-    }
-}
-
-template <int NF, bool EX, int K>
-__device__ __forceinline__ void fold_backward(const double* buf, const double* tab, int N, int P, int TL, int c,
-                                              int lloc, double (&X)[NF]) {
-#pragma clang fp contract(fast)
-#pragma unroll
-    for (int f = 0; f < NF; f++) X[f] = 0.0;
-    for (int cc = P - 1; cc > c; cc--) {
-        const int sc = EX ? cc * K : (int)(((long)cc * N) / P);
-        const double R = tab[4 * N + sc];
-#pragma unroll
-        for (int f = 0; f < NF; f++) X[f] = buf[(f * P + cc) * TL + lloc] + R * X[f];
-    }
-}
-
-// carries across the chunks of a line held by GS consecutive lanes of a wave:
-// segmented Kogge-Stone scans of the affine maps x -> A*x + B
-template <int NF>
-__device__ __forceinline__ void scan_forward(double A, const double (&carry)[NF], bool cv, int c, int GS,
-                                             double (&G)[NF]) {
-#pragma clang fp contract(fast)
-    double B[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) B[f] = cv ? carry[f] : 0.0;
-    if (!cv) A = 1.0;
-    for (int d = 1; d < GS; d <<= 1) {
-        const double Au = __shfl_up(A, d, GS);
-        double Bu[NF];
-#pragma unroll
-        for (int f = 0; f < NF; f++) Bu[f] = __shfl_up(B[f], d, GS);
-        if (c >= d) {
-#pragma unroll
-            for (int f = 0; f < NF; f++) B[f] += A * Bu[f];
-            A *= Au;
-        }
-    }
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-        const double g = __shfl_up(B[f], 1, GS);
-        G[f] = (c == 0) ? 0.0 : g;
-    }
-}
-
-template <int NF>
-__device__ __forceinline__ void scan_backward(double A, const double (&carry)[NF], bool cv, int c, int P, int GS,
-                                              double (&X)[NF]) {
-#pragma clang fp contract(fast)
-    double B[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) B[f] = cv ? carry[f] : 0.0;
-    if (!cv) A = 0.0;
-    for (int d = 1; d < GS; d <<= 1) {
-        const double Ad = __shfl_down(A, d, GS);
-        double Bd[NF];
-#pragma unroll
-        for (int f = 0; f < NF; f++) Bd[f] = __shfl_down(B[f], d, GS);
-        if (c + d < GS) {
-#pragma unroll
-            for (int f = 0; f < NF; f++) B[f] += A * Bd[f];
-            A *= Ad;
-        }
-    }
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-        const double x = __shfl_down(B[f], 1, GS);
-        X[f] = (c + 1 < P) ? x : 0.0;
-    }
-}
-
 // characteristic correction of a face (first or last chunk of a line, never
 // both: lines have at least two chunks), weighted by sbcc on the kMbci rows
 // next to the face; the face derivative is the uncorrected solve result
@@ -592,16 +444,6 @@ __device__ __forceinline__ void face_corrections(const SW& sw, uoff base, uoff s
             }
         }
     }
-}
-
-// workgroup -> line group, neighbouring groups on the same XCD (workgroups
-// are dispatched round-robin over the 8 XCDs, each with its own L2)
-__device__ __forceinline__ long group_of(int xcd) {
-    const long w = blockIdx.x;
-    if (!xcd) return w;
-    const long G8 = ((long)gridDim.x / 8) * 8;
-    if (w >= G8) return w;
-    return (w % 8) * (G8 / 8) + w / 8;
 }
 
 // ---------------------------------------------------------------------------
@@ -781,9 +623,6 @@ direct_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
 // line is solved with wave scans, and the results go back through the buffer
 // for coalesced stores.  No workgroup barrier: a workgroup is one wave.
 // ---------------------------------------------------------------------------
-template <int K>
-__device__ __forceinline__ int xt_slot(int li, int P, int c, int t) { return (li * P + c) * (K + 1) + t; }
-
 template <int K, class SW, bool GCBC>
 __global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
     constexpr int NF = SW::NF;
@@ -1335,29 +1174,6 @@ __global__ void set_dt_kernel(double dt, double* out) {
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
-static Phys phys_of(const cs_solver* sv) {
-    Phys ph;
-    ph.umf = sv->cfg.umf;
-    ph.mu = sv->cfg.mu;
-    ph.J = sv->cfg.yaco;
-    for (int a = 0; a < 3; a++) ph.m[a] = sv->cfg.metric[a];
-    return ph;
-}
-
-static SweepCommon common_of(const cs_solver* sv, const double* sbcc) {
-    SweepCommon cm;
-    cm.lxi = sv->g.lxi;
-    cm.plane = (long)sv->g.lxi * sv->g.let;
-    cm.pb = sv->num->pb;
-    cm.sbcc = sbcc;
-    return cm;
-}
-
-template <class T, int N>
-static void fields(T* base, long lmx, T* (&out)[N]) {
-    for (int k = 0; k < N; k++) out[k] = base ? base + (long)k * lmx : nullptr;
-}
-
 // axes of the S1 sweeps (the third one closes) and of the S2 sweeps (the
 // third one finishes the RHS) for each sweep order
 static const int kS1Axes[2][3] = {{0, 1, 2}, {1, 2, 0}};
@@ -1404,6 +1220,10 @@ static S2<AX, MODE, V> make_s2(const cs_solver* sv, const double* q, const doubl
     a.dtp = sv->dt;
     a.div = div;
     a.ph = phys_of(sv);
+    a.nface = sv->g.face_size(AX);
+    a.rec = sv->rec ? sv->rec + face_offset(sv->g, AX) : nullptr;
+    a.lxi = sv->g.lxi;
+    a.plane = (long)sv->g.lxi * sv->g.let;
     return a;
 }
 

@@ -61,6 +61,12 @@ struct cs_solver {
     double* dt = nullptr;      // [0] dt, [1..2] maxima
     double* dt_host = nullptr; // pinned
     double* vjp = nullptr;     // [41][lmx] adjoint scratch (allocated on the first adjoint call)
+    // adjoint sweeps (tape.hip, set up on the first call that needs them)
+    double* ttabs[3] = {nullptr, nullptr, nullptr};   // LU of A^T of the derivative, 8-point chunks
+    double* fttabs[3] = {nullptr, nullptr, nullptr};  // LU of A^T of the closure-line filter
+    double* adj = nullptr;     // seed, accumulator (5 fields each), face buffers, internal tape
+    double* rec = nullptr;     // faces of the RHS being evaluated go here when set ([3 axes][2][5][face])
+    int adj_mode = 1;          // 1: fused transposed sweeps where the lines allow; 0: per-operator products
 };
 
 namespace cs {
@@ -72,5 +78,14 @@ int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, dou
 int build_axis_table(int n, int seam, int K, int P, std::vector<double>& h);
 bool filter_fusable(const cs_solver* sv);
 int filter_sweeps(cs_solver* sv, double* q, hipStream_t s, int axis = -1);
+// adjoint (vjp.hip: per-operator products; tape.hip: fused transposed sweeps)
+bool adjoint_fused(const cs_solver* sv);
+long tape_doubles(const cs_solver* sv);
+int rk4_vjp_lines(cs_solver* sv, const double* q0, const double* Q, double* lam, double dt, const double* sbcc,
+                  hipStream_t s);
+int rk4_step_tape(cs_solver* sv, double* qa, const double* sbcc, double dt, double* tape, hipStream_t s);
+int rk4_vjp_tape(cs_solver* sv, const double* tape, double* lam, double dt, const double* sbcc, hipStream_t s);
+int rk4_vjp_fused(cs_solver* sv, const double* q0, double* lam, double dt, const double* sbcc, hipStream_t s);
+int rhs_vjp_fused(cs_solver* sv, const double* q, const double* lam, double* g, const double* sbcc, hipStream_t s);
 }  // namespace fused
 }  // namespace cs

@@ -70,7 +70,9 @@ extern "C" {
 int cs_solver_destroy(cs_solver* sv) {
     if (!sv) return CS_OK;
     double* bufs[] = {sv->ga, sv->gb, sv->tau, sv->hf, sv->dpart, sv->qb, sv->qc, sv->red, sv->dt, sv->vjp,
-                      sv->tabs[0], sv->tabs[1], sv->tabs[2], sv->ftabs[0], sv->ftabs[1], sv->ftabs[2]};
+                      sv->tabs[0], sv->tabs[1], sv->tabs[2], sv->ftabs[0], sv->ftabs[1], sv->ftabs[2],
+                      sv->ttabs[0], sv->ttabs[1], sv->ttabs[2], sv->fttabs[0], sv->fttabs[1], sv->fttabs[2],
+                      sv->adj};
     for (double* p : bufs)
         if (p) (void)hipFree(p);
     if (sv->dt_host) (void)hipHostFree(sv->dt_host);

@@ -17,33 +17,14 @@
 // products come from forward-mode dual numbers on the same formulas; D_a^T
 // is cs_deriv_adjoint.  The RK4 step adjoint recomputes the three inner
 // stages from q0 and runs the stages backwards (see cs_solver_rk4_vjp).
-#include "fused.hpp"
+// These per-operator products serve lines that do not split into 8-point
+// chunks and adjoint mode 0; tape.hip holds the fused transposed sweeps.
+#include "vjp_parts.hpp"
 
 namespace cs {
 namespace fused {
 
 namespace {
-
-struct EulerPhys {
-    cs_point umf;
-    double J;
-    double m[3];
-};
-
-// flux row of axis ax (the S2 sweep's point(), Euler part)
-template <class T>
-__device__ __forceinline__ void flux_row(int ax, const T* q, double m, cs_point umf, T* F) {
-    PrimT<T> p = primitive(q);
-    const T vel = ax == 0 ? p.u : (ax == 1 ? p.v : p.w);
-    const double uf = ax == 0 ? umf.x : (ax == 1 ? umf.y : umf.z);
-    const T ss = m * (vel + uf);
-    F[0] = q[0] * ss;
-    F[1] = q[1] * ss;
-    F[2] = q[2] * ss;
-    F[3] = q[3] * ss;
-    F[1 + ax] = F[1 + ax] + m * p.p;
-    F[4] = (q[4] + p.p) * ss - p.p * (uf * m);
-}
 
 // F[v] = flux row of axis ax at every point
 __global__ void __launch_bounds__(256) flux_fields_kernel(const double* q, double* F, int ax, EulerPhys ph,
@@ -72,24 +53,6 @@ __global__ void __launch_bounds__(256) flux_vjp_kernel(const double* q, const do
     for (int v = 0; v < 5; v++) phi = phi + f[v] * nu[t + v * lmx];
 #pragma unroll
     for (int v = 0; v < 5; v++) g[t + v * lmx] += phi.d[v];
-}
-
-// characteristic correction of a face point (the S2 sweep's gcbc())
-template <class T, class X>
-__device__ __forceinline__ void face_corr(int ax, int face, const T* q, const X* drv, const EulerPhys& ph, X* corr) {
-    PrimT<T> p = primitive(q);
-    const double cm[3] = {ax == 0 ? 1.0 : 0.0, ax == 1 ? 1.0 : 0.0, ax == 2 ? 1.0 : 0.0};
-    EigT<T> e;
-    e.eleme(cm, q[0], q + 1, p.p, ph.umf);
-    X cha[5], ch[5];
-#pragma unroll
-    for (int v = 0; v < 5; v++) cha[v] = ph.J * drv[v];
-    e.l_apply(cm, cha, ch);
-    const double ra0 = 1 - 2 * face;
-    X dl[5] = {X(0.0), X(0.0), X(0.0), X(0.0), X(0.0)};
-    if (ra0 * value_of(e.vn + e.vs + e.ao) > 0) dl[3] = -ch[3];
-    if (ra0 * value_of(e.vn + e.vs - e.ao) > 0) dl[4] = -ch[4];
-    e.r_apply(cm, dl, corr);
 }
 
 // face points of axis ax: lam_f = sum_ii sbcc*lambda over the 4 rows next to
@@ -238,28 +201,17 @@ int solver_rhs_vjp(cs_solver* sv, const double* q, const double* lam, double* g,
 }
 
 // lam <- (d q_next / d q0)^T lam for the RK4 step of cs_solver_rk4_step with
-// the time step dt (held fixed), including the filter when enabled.
-int solver_rk4_vjp(cs_solver* sv, const double* q0, double* lam, double dt, const double* sbcc, hipStream_t s) {
-    if (sv->cfg.viscous) return fail(CS_ERR_UNSUPPORTED, "the adjoint covers the inviscid right-hand side");
-    if (!(dt > 0.0)) return fail(CS_ERR_ARG, "the adjoint needs the step's dt > 0");
+// the time step dt (held fixed), including the filter when enabled, from the
+// stage states q0 and Q = q1, q2, q3 (per-operator line solves)
+int rk4_vjp_lines(cs_solver* sv, const double* q0, const double* Q, double* lam, double dt, const double* sbcc,
+                  hipStream_t s) {
     int rc = vjp_buffers(sv);
     if (rc) return rc;
     const long lmx = sv->g.lmx, n5 = 5 * lmx;
-    double* Q = sv->vjp + 16 * lmx;          // q1, q2, q3
-    double* t = Q + 15 * lmx;                // adjoint of the current stage
+    double* t = sv->vjp + 31 * lmx;          // adjoint of the current stage
     double* gr = t + 5 * lmx;                // R'(q)^T t
-    double* de = sv->dpart;                  // forward RHS (scratch of the solver)
     const double J = sv->cfg.yaco;
     const double ck[5] = {0.0, dt / 4.0 * J, dt / 3.0 * J, dt / 2.0 * J, dt * J};
-    // stage states: q_k = q0 - c_k R(q_{k-1})
-    const double* prev = q0;
-    for (int k = 1; k <= 3; k++) {
-        if ((rc = rhs_sweeps(sv, prev, sbcc, de, nullptr, 0, s))) return rc;
-        double* qk = Q + (long)(k - 1) * n5;
-        hipLaunchKernelGGL(stage_kernel, grid_of(n5), dim3(256), 0, s, qk, q0, de, ck[k], n5);
-        CS_HIP(hipGetLastError());
-        prev = qk;
-    }
     // through the filter (applied x, then y, then z after the last stage)
     if (sv->cfg.filter) {
         double* w1 = gr;
@@ -286,6 +238,28 @@ int solver_rk4_vjp(cs_solver* sv, const double* q0, double* lam, double dt, cons
     return CS_OK;
 }
 
+// the per-operator form of cs_solver_rk4_vjp: the three inner stage states
+// recomputed from q0, then rk4_vjp_lines
+int solver_rk4_vjp(cs_solver* sv, const double* q0, double* lam, double dt, const double* sbcc, hipStream_t s) {
+    int rc = vjp_buffers(sv);
+    if (rc) return rc;
+    const long lmx = sv->g.lmx, n5 = 5 * lmx;
+    double* Q = sv->vjp + 16 * lmx;          // q1, q2, q3
+    double* de = sv->dpart;                  // forward RHS (scratch of the solver)
+    const double J = sv->cfg.yaco;
+    const double ck[4] = {0.0, dt / 4.0 * J, dt / 3.0 * J, dt / 2.0 * J};
+    // stage states: q_k = q0 - c_k R(q_{k-1})
+    const double* prev = q0;
+    for (int k = 1; k <= 3; k++) {
+        if ((rc = rhs_sweeps(sv, prev, sbcc, de, nullptr, 0, s))) return rc;
+        double* qk = Q + (long)(k - 1) * n5;
+        hipLaunchKernelGGL(stage_kernel, grid_of(n5), dim3(256), 0, s, qk, q0, de, ck[k], n5);
+        CS_HIP(hipGetLastError());
+        prev = qk;
+    }
+    return rk4_vjp_lines(sv, q0, Q, lam, dt, sbcc, s);
+}
+
 }  // namespace fused
 }  // namespace cs
 
@@ -296,6 +270,8 @@ int cs_solver_rhs_vjp(cs_solver* sv, const double* qa, const double* lam, double
     if (!sv || !qa || !lam || !g) return cs::fail(CS_ERR_ARG, "null argument");
     if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
         return cs::fail(CS_ERR_ARG, "non-reflective faces need sbcc");
+    if (sv->cfg.viscous) return cs::fail(CS_ERR_UNSUPPORTED, "the adjoint covers the inviscid right-hand side");
+    if (cs::fused::adjoint_fused(sv)) return cs::fused::rhs_vjp_fused(sv, qa, lam, g, sbcc, (hipStream_t)stream);
     return cs::fused::solver_rhs_vjp(sv, qa, lam, g, sbcc, (hipStream_t)stream);
 }
 
@@ -303,6 +279,9 @@ int cs_solver_rk4_vjp(cs_solver* sv, const double* q0, double* lam, double dt, c
     if (!sv || !q0 || !lam) return cs::fail(CS_ERR_ARG, "null argument");
     if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
         return cs::fail(CS_ERR_ARG, "non-reflective faces need sbcc");
+    if (sv->cfg.viscous) return cs::fail(CS_ERR_UNSUPPORTED, "the adjoint covers the inviscid right-hand side");
+    if (!(dt > 0.0)) return cs::fail(CS_ERR_ARG, "the adjoint needs the step's dt > 0");
+    if (cs::fused::adjoint_fused(sv)) return cs::fused::rk4_vjp_fused(sv, q0, lam, dt, sbcc, (hipStream_t)stream);
     return cs::fused::solver_rk4_vjp(sv, q0, lam, dt, sbcc, (hipStream_t)stream);
 }
 

@@ -176,6 +176,32 @@ class FusedSolver:
         check(lib().cs_solver_rk4_vjp(self.h, _p(q0), _p(lam), float(dt), _p(self.sbcc), _stream(stream)))
         return lam
 
+    def tape_size(self):
+        """Doubles of the tape of one RK4 step (cs_solver_tape_size)."""
+        n = ctypes.c_size_t()
+        check(lib().cs_solver_tape_size(self.h, ctypes.byref(n)))
+        return n.value
+
+    def new_tape(self):
+        import torch
+        return torch.empty(self.tape_size(), dtype=torch.float64, device="cuda")
+
+    def rk4_step_tape(self, q, tape, dt=0.0, report_dt=False, stream=None):
+        """rk4_step that records the step's tape (inviscid solvers)."""
+        out = ctypes.c_double()
+        check(lib().cs_solver_rk4_step_tape(self.h, _p(q), _p(self.sbcc), float(dt),
+                                            ctypes.byref(out) if report_dt else None, _p(tape), _stream(stream)))
+        return out.value if report_dt else None
+
+    def rk4_vjp_tape(self, tape, lam, dt, stream=None):
+        """lam <- (d step / d q0)^T lam for the step that recorded `tape` (in place)."""
+        check(lib().cs_solver_rk4_vjp_tape(self.h, _p(tape), _p(lam), float(dt), _p(self.sbcc), _stream(stream)))
+        return lam
+
+    def set_adjoint_mode(self, mode):
+        """1: fused transposed sweeps (default, lines of 8-point chunks); 0: per-operator products."""
+        check(lib().cs_solver_adjoint_mode(self.h, int(mode)))
+
     def time_sweeps(self, q, stream, reps=10, sweeps=None):
         """Average duration (ms) of each sweep, HIP events on the launch stream
         (default: the sweeps of one RK stage)."""

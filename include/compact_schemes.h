@@ -248,6 +248,25 @@ int cs_solver_rhs_vjp(cs_solver* sv, const double* qa, const double* lam, double
  * the three inner stages are recomputed from it. */
 int cs_solver_rk4_vjp(cs_solver* sv, const double* q0, double* lam, double dt, const double* sbcc,
                       cs_stream stream);
+/* size, in doubles, of the tape of one RK4 step (no reference counterpart):
+ * the stage states q0..q3 (20 fields) and, per stage, the boundary
+ * derivatives of the flux divergence along each axis ([3][2][5][face]). */
+int cs_solver_tape_size(const cs_solver* sv, size_t* doubles);
+/* cs_solver_rk4_step that also records the tape of the step into `tape`
+ * (device, cs_solver_tape_size doubles) for cs_solver_rk4_vjp_tape; inviscid
+ * solvers.  Same state and dt semantics as cs_solver_rk4_step. */
+int cs_solver_rk4_step_tape(cs_solver* sv, double* qa, const double* sbcc, double dt, double* dt_out, double* tape,
+                            cs_stream stream);
+/* lam <- (d step / d q0)^T lam for the step that recorded `tape`, with the
+ * step's dt: the adjoint of cs_solver_rk4_vjp without recomputing the stages. */
+int cs_solver_rk4_vjp_tape(cs_solver* sv, const double* tape, double* lam, double dt, const double* sbcc,
+                           cs_stream stream);
+/* adjoint implementation (no reference counterpart): 1 (default) = transposed
+ * line sweeps fused with the pointwise Jacobian products, used when every
+ * axis splits into 8-point chunks (N % 8 == 0, N <= 256); 0 = per-operator
+ * transposed line solves (cs_deriv_adjoint, cs_filter_adjoint).  Results
+ * agree to rounding. */
+int cs_solver_adjoint_mode(cs_solver* sv, int mode);
 
 /* ---- batched subdomains (no reference counterpart) ----------------------
  * The plain method -- derivative of a field along x, y and z

@@ -1,0 +1,126 @@
+"""Fused adjoint of the inviscid RHS and RK4 step (transposed line sweeps,
+cs_solver_adjoint_mode 1) against the per-operator adjoint (mode 0, itself
+checked against finite differences in test_gpu_vjp.py), the recorded-tape
+step and adjoint (cs_solver_rk4_step_tape / cs_solver_rk4_vjp_tape), and
+central finite differences on lines that split into 8-point chunks.
+
+Bars: fused vs per-operator products agree to 1e-12 norm-wise (the same
+operators, summed in another order); the tape step reproduces
+cs_solver_rk4_step bit for bit and the tape adjoint cs_solver_rk4_vjp bit for
+bit (same kernels on the same stage states); finite differences to 1e-7.
+"""
+import numpy as np
+import pytest
+import torch
+
+from gpu_util import case, dev, host, nrel
+
+CONFIG5 = [[45, 10, 10], [45, 10, 10]]     # periodic x, characteristic y and z
+CL = [[10, 10, 10], [10, 10, 10]]
+PER = [[45, 45, 45], [45, 45, 45]]
+MIXT = [[10, 45, 10], [10, 45, 10]]        # characteristic x (transposing x sweep with faces)
+NBC = [CONFIG5, CL, PER, MIXT]
+IDS = ["config5", "closure", "periodic", "charx"]
+
+
+def _setup(dims, nbc, seed, filt=True):
+    from compact_schemes_amd import workloads as wl
+    from compact_schemes_amd.fused import FusedSolver
+    st = wl.flow_state(dims, seed=seed, noise=1e-2, width=3.0)
+    sv = FusedSolver(dims, nbc, st, viscous=False, filt=filt)
+    return st, sv
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims", [(32, 24, 16), (16, 40, 24)])
+@pytest.mark.parametrize("nbc", NBC, ids=IDS)
+def test_fused_rhs_vjp_matches_per_operator(dims, nbc):
+    st, sv = _setup(dims, nbc, 31)
+    rng = np.random.default_rng(7)
+    q = dev(st["qa"])
+    lam = dev(rng.standard_normal(st["qa"].size))
+    case(f"fused rhs vjp dims={dims} nbc={nbc}")
+    g1 = sv.rhs_vjp(q, lam).clone()
+    sv.set_adjoint_mode(0)
+    g0 = sv.rhs_vjp(q, lam).clone()
+    assert nrel(host(g1), host(g0)) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims", [(32, 24, 16), (16, 40, 24)])
+@pytest.mark.parametrize("nbc", NBC, ids=IDS)
+@pytest.mark.parametrize("filt", [True, False], ids=["filter", "nofilter"])
+def test_fused_rk4_vjp_matches_per_operator(dims, nbc, filt):
+    st, sv = _setup(dims, nbc, 32, filt)
+    rng = np.random.default_rng(8)
+    q0 = dev(st["qa"])
+    dt = sv.time_step(q0)
+    lam = dev(rng.standard_normal(st["qa"].size))
+    case(f"fused rk4 vjp dims={dims} nbc={nbc} filter={filt}")
+    a1 = sv.rk4_vjp(q0, lam.clone(), dt).clone()
+    sv.set_adjoint_mode(0)
+    a0 = sv.rk4_vjp(q0, lam.clone(), dt).clone()
+    assert nrel(host(a1), host(a0)) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nbc", NBC, ids=IDS)
+def test_tape_step_and_adjoint(nbc):
+    dims = (24, 32, 16)
+    st, sv = _setup(dims, nbc, 33)
+    rng = np.random.default_rng(9)
+    q0 = dev(st["qa"])
+    case(f"tape step nbc={nbc}")
+    ref = q0.clone()
+    dt = sv.rk4_step(ref, report_dt=True)
+    tape = sv.new_tape()
+    q = q0.clone()
+    dt2 = sv.rk4_step_tape(q, tape, report_dt=True)
+    assert dt2 == dt
+    assert torch.equal(q, ref)
+    assert torch.equal(tape[:q0.numel()], q0)
+    lam = dev(rng.standard_normal(st["qa"].size))
+    a_tape = sv.rk4_vjp_tape(tape, lam.clone(), dt).clone()
+    a_rec = sv.rk4_vjp(q0, lam.clone(), dt).clone()
+    assert torch.equal(a_tape, a_rec)
+    # the per-operator adjoint reads the same tape
+    sv.set_adjoint_mode(0)
+    a_lines = sv.rk4_vjp_tape(tape, lam.clone(), dt).clone()
+    assert nrel(host(a_lines), host(a_tape)) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nbc", [CONFIG5, CL, MIXT], ids=["config5", "closure", "charx"])
+def test_fused_rk4_vjp_matches_finite_differences(nbc):
+    dims = (24, 16, 32)
+    st, sv = _setup(dims, nbc, 34)
+    rng = np.random.default_rng(10)
+    n = st["qa"].size
+    q0 = dev(st["qa"])
+    dt = sv.time_step(q0)
+    lam = dev(rng.standard_normal(n))
+    v = dev(rng.standard_normal(n))
+
+    def step(q):
+        x = q.clone()
+        sv.rk4_step(x, dt=dt)
+        return x
+    case(f"fused rk4 vjp fd nbc={nbc}")
+    g = sv.rk4_vjp(q0, lam.clone(), dt)
+    eps = 1e-6
+    fd = float(torch.dot(lam, (step(q0 + eps * v) - step(q0 - eps * v)) / (2 * eps)))
+    ad = float(torch.dot(g, v))
+    assert abs(fd - ad) <= 1e-7 * abs(ad), (fd, ad)
+
+
+@pytest.mark.gpu
+def test_tape_errors():
+    import compact_schemes_amd as cs
+    st, sv = _setup((16, 16, 16), CONFIG5, 35)
+    q = dev(st["qa"])
+    tape = sv.new_tape()
+    sv.rk4_step_tape(q, tape, dt=0.1)
+    with pytest.raises(cs.CsError):
+        sv.rk4_vjp_tape(tape, q.clone(), 0.0)            # the adjoint needs the step's dt
+    with pytest.raises(cs.CsError):
+        sv.set_adjoint_mode(2)
