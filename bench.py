@@ -367,7 +367,9 @@ def precision_bench(args, dims=(512, 512, 256), nsteps=50):
     faces, 50 RK4 steps (reference-ordered operators, viscous, filter every
     step, dt fixed from the CFL limit of the initial state) with FP64 line
     solves, then with FP32 solves + one and + two FP64-residual refinement
-    sweeps (fluxes, boundary rows, stage update in FP64).  Reports the error
+    sweeps (fluxes, boundary rows, stage update in FP64; the mixed filter
+    solves for its correction x - f, so the FP32 rounding scales with the
+    correction, not with the field).  Reports the error
     of the mixed runs against the FP64 run per conserved variable, relative
     to the state and to its change over the run; value = the largest
     state-relative error with one sweep."""

@@ -47,6 +47,7 @@ struct S1 {
     static constexpr bool kGcbc = false;   // gradients only; BC updates act on the flux sweeps
     static constexpr bool kPv = CLOSE;     // the closure needs u, v, w, c^2 at its rows
     static constexpr int kH = 2;           // derivative stencil half-width
+    static constexpr int kWpe = 3;         // waves per SIMD of the y/z kernels (168 VGPRs)
     static constexpr bool kFilter = false;
     const double* q[5];
     double* gout[4];          // !CLOSE: d/d(axis AX) of u, v, w, c^2
@@ -100,6 +101,7 @@ struct S2 {
     static constexpr bool kGcbc = true;
     static constexpr bool kPv = false;
     static constexpr int kH = 2;
+    static constexpr int kWpe = 3;
     static constexpr bool kFilter = false;
     const double* q[5];
     const double* tr[3];      // row AX of the stress tensor
@@ -191,6 +193,7 @@ struct SF {
     static constexpr bool kGcbc = false;
     static constexpr bool kPv = false;
     static constexpr int kH = 5;
+    static constexpr int kWpe = 3;
     static constexpr bool kFilter = true;
     double* q[5];
     __device__ __forceinline__ void point(uoff bo, double* v) const {
@@ -212,6 +215,7 @@ struct SF1 {
     static constexpr bool kGcbc = false;
     static constexpr bool kPv = false;
     static constexpr int kH = 5;
+    static constexpr int kWpe = 3;
     static constexpr bool kFilter = true;
     double* q;
     long lmx;
@@ -453,7 +457,7 @@ __device__ __forceinline__ void face_corrections(const SW& sw, uoff base, uoff s
 // LDS for y/z lines)
 // ---------------------------------------------------------------------------
 template <int K, int AX, int TL, class SW, bool GCBC>
-__global__ void __launch_bounds__(AX == 0 ? 256 : TL * kMaxP) __attribute__((amdgpu_waves_per_eu(AX == 0 ? 1 : 3)))
+__global__ void __launch_bounds__(AX == 0 ? 256 : TL * kMaxP) __attribute__((amdgpu_waves_per_eu(AX == 0 ? 1 : SW::kWpe)))
 direct_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
     constexpr int NF = SW::NF;
     constexpr int H = SW::kH;

@@ -280,6 +280,7 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
 
     // ---- right-hand side of the chunk rows ----
     double r[K];
+    double f0[MIX && OP == OP_FILTER ? K : 1];      // the filtered values (mixed filter: x = f + d)
     if (cvalid) {
         double w[W];
 #pragma unroll
@@ -384,9 +385,26 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
                 }
             }
         }
+        if constexpr (MIX && OP == OP_FILTER) {
+            // the mixed filter solves for the correction d = x - f, A d = B f - A f:
+            // its FP32 rounding is relative to the (small) correction, not to f
+            const double* Am = tab + 6 * N + 44;
+            const double* Bm = Am + N;
+            const double* Cm = Am + 2 * N;
+#pragma unroll
+            for (int t = 0; t < K; t++) {
+                const int j = s + t;
+                f0[t] = w[t + H];
+                if (j < N) r[t] -= Am[j] * w[t + H - 1] + Bm[j] * w[t + H] + Cm[j] * w[t + H + 1];
+            }
+        }
     } else {
 #pragma unroll
         for (int t = 0; t < K; t++) r[t] = 0.0;
+        if constexpr (MIX && OP == OP_FILTER) {
+#pragma unroll
+            for (int t = 0; t < K; t++) f0[t] = 0.0;
+        }
     }
 
     const bool cyc = OP == OP_FILTER && a.cyclic;
@@ -441,8 +459,13 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
 #pragma unroll
             for (int t = 0; t < K; t++) x[t] += (double)dx[t];
         }
+        if constexpr (OP == OP_FILTER) {
 #pragma unroll
-        for (int t = 0; t < K; t++) r[t] = x[t];
+            for (int t = 0; t < K; t++) r[t] = f0[t] + x[t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < K; t++) r[t] = x[t];
+        }
     }
 
     // ---- boundary values for the characteristic BCs ----
