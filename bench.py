@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only launch the dominant sweep and a plain device copy --reps times each "
                          "(the program rocprofv3 --pmc runs for the traffic figures)")
+    ap.add_argument("--adjoint-probe", action="store_true",
+                    help="only run --reps RK4 steps recording tapes and their adjoints on the 128^3 Euler "
+                         "workload of --workload adjoint (the program rocprofv3 profiles the adjoint kernels with)")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--workload", default="rhs", choices=["rhs", "plain64", "batch96", "precision", "adjoint"],
                     help="rhs: the 256^3 viscous RK4 step (default); plain64: derivative x/y/z + filter x/y/z "
@@ -81,6 +84,8 @@ def main():
     torch.cuda.set_device(0)
     if args.pmc_probe:
         return pmc_probe(args)
+    if args.adjoint_probe:
+        return adjoint_probe(args)
     if args.workload == "plain64":
         return plain_bench(args)
     if args.workload == "batch96":
@@ -612,6 +617,32 @@ def traffic_from_profile(kernel, n):
         if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
             return float(parts[2])
     return None
+
+
+def adjoint_probe(args, n=128):
+    """--reps RK4 steps of the config-5 solver recording tapes, then their
+    adjoints backwards, twice; for kernel traces of the adjoint sweeps."""
+    import torch
+    from compact_schemes_amd import workloads as wl
+    from compact_schemes_amd.fused import FusedSolver
+    st = wl.flow_state((n, n, n), seed=5, amp=1e-4)
+    sv = FusedSolver((n, n, n), [[45, 10, 10], [45, 10, 10]], st, viscous=False, cfl=0.5)
+    q = torch.from_numpy(st["qa"]).cuda()
+    dt = sv.time_step(q)
+    tapes = [sv.new_tape() for _ in range(args.reps)]
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    lam = torch.randn(q.numel(), generator=g, dtype=torch.float64, device="cuda")
+    for _ in range(2):
+        x = q.clone()
+        for i in range(args.reps):
+            sv.rk4_step(x.clone(), dt=dt)
+        for i in range(args.reps):
+            sv.rk4_step_tape(x, tapes[i], dt=dt)
+        for i in reversed(range(args.reps)):
+            sv.rk4_vjp_tape(tapes[i], lam, dt)
+    torch.cuda.synchronize()
+    print(json.dumps({"adjoint_probe": "done", "steps": args.reps, "n": n}), flush=True)
 
 
 def pmc_probe(args):
