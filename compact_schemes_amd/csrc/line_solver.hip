@@ -377,9 +377,12 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
                     } else if (dist <= 0) {
                         r[t] = w[t + H];
                     } else {
+                        // unrolled with a bound test so that w stays in registers
                         const double* cf = an + 6 * dist;
                         double v = cf[0] * w[t + H];
-                        for (int m = 1; m <= dist; m++) v += 0.5 * cf[m] * (w[t + H + m] + w[t + H - m]);
+#pragma unroll
+                        for (int m = 1; m < H; m++)
+                            if (m <= dist) v += 0.5 * cf[m] * (w[t + H + m] + w[t + H - m]);
                         r[t] = v;
                     }
                 }
