@@ -302,9 +302,10 @@ def batch_bench(args):
     """Config 3: the plain method on 96 uneven periodic subdomains: one step =
     derivative along x, y, z + filter along x, y, z, replayed from a hipGraph
     of the batched launches, after the autotuner has picked chunk lengths and
-    the grid mode.  The untuned batch, the persistent grid and the same work
-    done subdomain by subdomain with the single-subdomain operators are timed
-    beside it.  value = grid points of all subdomains per second per step."""
+    the grid mode (one workgroup per item, persistent, or one launch per
+    subdomain and operator).  The untuned batch in each grid mode and the same
+    work done subdomain by subdomain through cs_deriv / cs_filter (not
+    captured) are timed beside it.  value = grid points of all subdomains per second per step."""
     import torch
     import compact_schemes_amd as cs
     from compact_schemes_amd import batch as bm
@@ -333,6 +334,8 @@ def batch_bench(args):
     ms_default = timed(lambda: b.step(st), args.steps)
     b.set_persistent(1)
     ms_persistent = timed(lambda: b.step(st), args.steps)
+    b.set_persistent(2)
+    ms_per_subdomain = timed(lambda: b.step(st), args.steps)
     b.set_persistent(0)
     nums = [cs.Numerics(dm, per) for dm in dims]
 
@@ -348,6 +351,7 @@ def batch_bench(args):
     torch.cuda.synchronize()
     ms_graph = timed(lambda: b.replay(st), args.steps)
     items, launches = b.info()
+    mode = b.get_persistent()
     bm.tune_clear()
     line = {
         "metric": "batched plain method Gpts/s (96 uneven FP64 subdomains)",
@@ -360,7 +364,10 @@ def batch_bench(args):
         "config": {"workload": "derivative x, y, z + GV10 filter x, y, z on 96 periodic subdomains, sizes in [48, 320]",
                    "points": npts, "tuned_graph_ms": round(ms_graph, 3),
                    "untuned_batched_ms": round(ms_default, 3), "untuned_persistent_ms": round(ms_persistent, 3),
+                   "untuned_per_subdomain_launches_ms": round(ms_per_subdomain, 3),
                    "one_subdomain_at_a_time_ms": round(ms_single, 3),
+                   "tuned_grid_mode": {0: "one workgroup per item", 1: "persistent",
+                                       2: "one launch per subdomain and operator"}[mode],
                    "items": items, "launches": launches,
                    "algorithmic_GBs": round(npts * 96 / (ms_graph * 1e-3) / 1e9, 1)},
     }

@@ -38,7 +38,7 @@ EXPORTS = [
     "cs_solver_set_order", "cs_solver_rhs_vjp", "cs_solver_rk4_vjp", "cs_solver_tape_size",
     "cs_solver_rk4_step_tape", "cs_solver_rk4_vjp_tape", "cs_solver_adjoint_mode",
     "cs_batch_create", "cs_batch_destroy", "cs_batch_bind", "cs_batch_deriv", "cs_batch_filter", "cs_batch_step",
-    "cs_batch_capture", "cs_batch_replay", "cs_batch_set_persistent", "cs_batch_info", "cs_batch_autotune", "cs_tune_set", "cs_tune_get",
+    "cs_batch_capture", "cs_batch_replay", "cs_batch_get_persistent", "cs_batch_set_persistent", "cs_batch_info", "cs_batch_autotune", "cs_tune_set", "cs_tune_get",
     "cs_tune_clear", "cs_tune_save", "cs_tune_load",
     "cs_solver_time_step", "cs_solver_rk4_step",
 ]
@@ -140,6 +140,7 @@ def lib():
             "cs_batch_step": (I, [P, P]),
             "cs_batch_capture": (I, [P, P]),
             "cs_batch_set_persistent": (I, [P, I]),
+            "cs_batch_get_persistent": (I, [P, ctypes.POINTER(ctypes.c_int)]),
             "cs_batch_replay": (I, [P, P]),
             "cs_batch_info": (I, [P, P, P]),
             "cs_batch_autotune": (I, [P, I, P]),

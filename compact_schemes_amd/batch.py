@@ -54,8 +54,15 @@ class Batch:
     def step(self, stream=None):
         check(lib().cs_batch_step(self.h, _stream(stream)))
 
-    def set_persistent(self, on):
-        check(lib().cs_batch_set_persistent(self.h, int(bool(on))))
+    def set_persistent(self, mode):
+        """Grid mode: 0 one workgroup per item, 1 persistent grid, 2 one launch
+        per subdomain and operator (the single-subdomain kernels)."""
+        check(lib().cs_batch_set_persistent(self.h, int(mode)))
+
+    def get_persistent(self):
+        m = ctypes.c_int()
+        check(lib().cs_batch_get_persistent(self.h, ctypes.byref(m)))
+        return m.value
 
     def capture(self, stream):
         check(lib().cs_batch_capture(self.h, _stream(stream)))

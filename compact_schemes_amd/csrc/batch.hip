@@ -54,7 +54,8 @@ struct cs_batch {
     Phase ph[4];                // derivative (all axes), filter x, y, z
     bool bound = false;
     int persistent = 0;         // 1: grid = resident workgroups, items round-robin;
-                                // 0: one workgroup per item (the dispatcher balances)
+                                // 0: one workgroup per item (the dispatcher balances);
+                                // 2: one launch per subdomain and operator (line_kernel)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -167,7 +168,41 @@ static int build_phase(cs_batch* b, Phase& p, int op, int axis_lo, int axis_hi) 
     return CS_OK;
 }
 
-static int run_phase(const Phase& p, int persistent, hipStream_t s) {
+// grid mode 2: the single-subdomain kernels (cs_deriv / cs_filter), one launch
+// per subdomain and operator -- shape-specialised, and cheap to launch once
+// recorded into the batch's hipGraph
+static int run_direct(cs_batch* b, int k, hipStream_t s) {
+    for (int sd = 0; sd < b->count; sd++) {
+        cs_numerics* num = b->num[sd];
+        const int a0 = k == 0 ? 0 : k - 1, a1 = k == 0 ? 2 : k - 1;
+        for (int axis = a0; axis <= a1; axis++) {
+            const int start = num->ndf[0][axis], end = num->ndf[1][axis];
+            const int rc = k == 0 ? line_apply(num, OP_DERIV, axis, b->f[sd], b->d[3 * sd + axis], start, end,
+                                               nullptr, 0, false, s)
+                                  : line_apply(num, OP_FILTER, axis, b->f[sd], b->f[sd], start, end, nullptr, 0,
+                                               false, s);
+            if (rc) return rc;
+        }
+    }
+    return CS_OK;
+}
+
+// the operator tables mode 2 launches with, built now so that nothing
+// allocates while a graph is being captured
+static int warm_tables(cs_batch* b) {
+    for (cs_numerics* num : b->num)
+        for (int axis = 0; axis < 3; axis++)
+            for (int op : {OP_DERIV, OP_FILTER}) {
+                LineOpTable* t = nullptr;
+                const int rc = get_line_op(num, op, axis, num->ndf[0][axis], num->ndf[1][axis], &t);
+                if (rc) return rc;
+            }
+    return CS_OK;
+}
+
+static int run_phase(cs_batch* b, int k, int persistent, hipStream_t s) {
+    if (persistent == 2) return run_direct(b, k, s);
+    const Phase& p = b->ph[k];
     for (const Launch& l : p.launches) {
         const int grid = persistent ? l.grid : l.nitems;
         int rc = launch_batch_lines(p.op, l.K, l.axis, p.args, p.items + l.first, l.nitems, grid, l.threads, l.lds, s);
@@ -230,13 +265,13 @@ int cs_batch_bind(cs_batch* b, double* const* f, double* const* d) {
 
 int cs_batch_deriv(cs_batch* b, cs_stream stream) {
     if (!b || !b->bound) return fail(CS_ERR_STATE, "batch has no bound fields");
-    return run_phase(b->ph[0], b->persistent, (hipStream_t)stream);
+    return run_phase(b, 0, b->persistent, (hipStream_t)stream);
 }
 
 int cs_batch_filter(cs_batch* b, cs_stream stream) {
     if (!b || !b->bound) return fail(CS_ERR_STATE, "batch has no bound fields");
     for (int axis = 0; axis < 3; axis++) {
-        int rc = run_phase(b->ph[1 + axis], b->persistent, (hipStream_t)stream);
+        int rc = run_phase(b, 1 + axis, b->persistent, (hipStream_t)stream);
         if (rc) return rc;
     }
     return CS_OK;
@@ -250,8 +285,19 @@ int cs_batch_step(cs_batch* b, cs_stream stream) {
 
 int cs_batch_set_persistent(cs_batch* b, int persistent) {
     if (!b) return fail(CS_ERR_ARG, "null batch");
-    b->persistent = persistent ? 1 : 0;
+    if (persistent < 0 || persistent > 2) return fail(CS_ERR_ARG, "grid mode is 0, 1 or 2");
+    if (persistent == 2) {
+        const int rc = warm_tables(b);
+        if (rc) return rc;
+    }
+    b->persistent = persistent;
     drop_graph(b);      // a captured graph holds the old grid
+    return CS_OK;
+}
+
+int cs_batch_get_persistent(const cs_batch* b, int* mode) {
+    if (!b || !mode) return fail(CS_ERR_ARG, "null argument");
+    *mode = b->persistent;
     return CS_OK;
 }
 
@@ -355,13 +401,13 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
     std::vector<double*> f = b->f, d = b->d;
     if ((rc = cs_batch_bind(b, f.data(), d.data()))) return rc;
     // then the grid mode, on the derivative phase (it leaves f unchanged)
-    float t[2] = {0.f, 0.f};
+    float t[3] = {0.f, 0.f, 0.f};
     CS_HIP(hipEventCreate(&e0));
     CS_HIP(hipEventCreate(&e1));
-    for (int mode = 0; mode < 2 && !rc; mode++) {
-        rc = run_phase(b->ph[0], mode, s);
+    for (int mode = 0; mode < 3 && !rc; mode++) {
+        rc = run_phase(b, 0, mode, s);
         if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = fail(CS_ERR_HIP, "event record failed");
-        for (int r = 0; r < reps && !rc; r++) rc = run_phase(b->ph[0], mode, s);
+        for (int r = 0; r < reps && !rc; r++) rc = run_phase(b, 0, mode, s);
         if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                     hipEventElapsedTime(&t[mode], e0, e1) != hipSuccess))
             rc = fail(CS_ERR_HIP, "event timing failed");
@@ -369,8 +415,11 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
-    b->persistent = t[1] < t[0] ? 1 : 0;
-    return CS_OK;
+    int best = 0;
+    for (int mode = 1; mode < 3; mode++)
+        if (t[mode] < t[best]) best = mode;
+    b->persistent = best;
+    return best == 2 ? warm_tables(b) : CS_OK;
 }
 
 int cs_tune_set(int op, int n, int K) {

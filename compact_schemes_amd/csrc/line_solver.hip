@@ -581,9 +581,21 @@ template <int K, int OP, int AX>
 __global__ void __launch_bounds__(max_threads(K)) batch_line_kernel(const LineArgs* args, const int2* items,
                                                                     int nitems) {
     extern __shared__ double smem[];
+    static_assert(sizeof(LineArgs) % 4 == 0, "LineArgs moves as 32-bit words");
+    constexpr int kWords = sizeof(LineArgs) / 4;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const int2 w = items[it];
-        line_body<K, OP, AX, lines_per_block(K, AX)>(args[w.x], w.y, smem);
+        // the item and its operator are the same for the whole workgroup:
+        // move them into scalar registers word by word, as the by-value
+        // kernel argument of line_kernel is
+        const int op = __builtin_amdgcn_readfirstlane(items[it].x);
+        const int blk = __builtin_amdgcn_readfirstlane(items[it].y);
+        const unsigned* src = reinterpret_cast<const unsigned*>(args + op);
+        unsigned words[kWords];
+#pragma unroll
+        for (int i = 0; i < kWords; i++) words[i] = __builtin_amdgcn_readfirstlane(src[i]);
+        LineArgs a;
+        __builtin_memcpy(&a, words, sizeof(LineArgs));
+        line_body<K, OP, AX, lines_per_block(K, AX)>(a, blk, smem);
         __syncthreads();   // the LDS of this item is free for the next
     }
 }

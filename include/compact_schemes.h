@@ -289,8 +289,12 @@ int cs_batch_filter(cs_batch* b, cs_stream stream);     /* filter x, then y, the
 int cs_batch_step(cs_batch* b, cs_stream stream);       /* deriv, then filter */
 /* grid of the batched launches: 0 (default) = one workgroup per work item,
  * the hardware dispatcher balances; 1 = persistent, as many workgroups as
- * fit on the device at once, items dealt round-robin.  Same results. */
+ * fit on the device at once, items dealt round-robin; 2 = one launch per
+ * subdomain and operator of the single-subdomain kernels (cs_deriv /
+ * cs_filter), cheap to launch once captured.  Same results. */
 int cs_batch_set_persistent(cs_batch* b, int persistent);
+/* the grid mode in use (set, or picked by cs_batch_autotune) */
+int cs_batch_get_persistent(const cs_batch* b, int* mode);
 /* records cs_batch_step into a hipGraph on `stream` (not the null stream);
  * cs_batch_replay launches it */
 int cs_batch_capture(cs_batch* b, cs_stream stream);
@@ -299,7 +303,8 @@ int cs_batch_replay(cs_batch* b, cs_stream stream);
 int cs_batch_info(const cs_batch* b, int* items4, int* launches4);
 /* autotuner: times the admissible chunk lengths (8, 16, 32 points) of every
  * (operator, line length) of the batch, records the fastest, rebuilds the
- * batch and picks the faster grid mode (cs_batch_set_persistent).  The choices hold for every operator table built afterwards in
+ * batch and picks the fastest grid mode (cs_batch_set_persistent).  The
+ * choices hold for every operator table built afterwards in
  * the process (the numerics contexts created later included), so batched and
  * single-subdomain results stay identical. */
 int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream);

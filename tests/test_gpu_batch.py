@@ -78,16 +78,19 @@ def test_batch_bit_identical_to_single_subdomain(bcs):
 
 
 @pytest.mark.gpu
-def test_batch_grid_modes_agree():
-    """Persistent round-robin grid and one workgroup per item: same bits."""
+@pytest.mark.parametrize("mode", [1, 2], ids=["persistent", "per-subdomain"])
+def test_batch_grid_modes_agree(mode):
+    """Persistent round-robin grid, one launch per subdomain and operator, and
+    one workgroup per item: same bits."""
     import torch
     dims = _dims(6, 6)
     nbc = [PER] * len(dims)
     fields = _fields(dims, 400)
     b1, f1, d1 = _batch(dims, nbc, fields)
     b2, f2, d2 = _batch(dims, nbc, fields)
-    b2.set_persistent(1)
-    case("batch grid modes")
+    b2.set_persistent(mode)
+    assert b2.get_persistent() == mode
+    case(f"batch grid mode {mode}")
     b1.step()
     b2.step()
     torch.cuda.synchronize()
