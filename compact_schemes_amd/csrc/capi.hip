@@ -274,6 +274,27 @@ int cs_mpigo(cs_numerics* num, int var, cs_stream stream) {
     return CS_OK;
 }
 
+int cs_exchange(cs_numerics* a, int face_a, cs_numerics* b, int face_b, int axis, int var, cs_stream stream) {
+    if (!a || !b || axis < 0 || axis > 2 || var < 0 || var >= kNV) return fail(CS_ERR_ARG, "bad context, axis or variable");
+    if ((face_a != 0 && face_a != 1) || (face_b != 0 && face_b != 1)) return fail(CS_ERR_ARG, "face is 0 or 1");
+    if (a->ndf[face_a][axis] != 1 || b->ndf[face_b][axis] != 1)
+        return fail(CS_ERR_STATE, "both faces must be seam faces (BC 35, 40 or 45)");
+    const long fs = a->g.face_size(axis);
+    if (b->g.face_size(axis) != fs || (axis != 0 && a->g.lxi != b->g.lxi) || (axis != 1 && a->g.let != b->g.let) ||
+        (axis != 2 && a->g.lze != b->g.lze))
+        return fail(CS_ERR_SHAPE, "the two faces do not match point for point");
+    if (a == b && face_a == face_b) return fail(CS_ERR_ARG, "a face cannot adjoin itself");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t bytes = sizeof(double) * 2 * fs;
+    double* sa = a->send[axis] + ((long)var * 2 + face_a) * 2 * fs;
+    double* ra = a->recv[axis] + ((long)var * 2 + face_a) * 2 * fs;
+    double* sb = b->send[axis] + ((long)var * 2 + face_b) * 2 * fs;
+    double* rb = b->recv[axis] + ((long)var * 2 + face_b) * 2 * fs;
+    CS_HIP(hipMemcpyAsync(rb, sa, bytes, hipMemcpyDeviceToDevice, s));
+    CS_HIP(hipMemcpyAsync(ra, sb, bytes, hipMemcpyDeviceToDevice, s));
+    return CS_OK;
+}
+
 int cs_deriv1d(cs_numerics* num, int axis, const double* in, double* out, int nstart, int nend, int var,
                cs_stream stream) {
     if (!num || axis < 0 || axis > 2) return fail(CS_ERR_ARG, "bad context or axis");

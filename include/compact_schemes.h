@@ -83,6 +83,14 @@ int cs_fill_buffers(cs_numerics* num, const double* infield, int var_lo, int var
 /* numerics_base::mpigo (2D), src/numerics_base.hpp:114-180, one subdomain:
  * recv[var][face] = send[var][1-face] on every seam face. */
 int cs_mpigo(cs_numerics* num, int var, cs_stream stream);
+/* numerics_base::mpigo between two subdomains on the same GPU (the neighbour
+ * rank mcd[face][axis] of the reference, src/numerics_base.hpp:114-180, is
+ * another context here): for the adjoining faces face_a of `a` and face_b of
+ * `b` along `axis`, recv_b[var][face_b] = send_a[var][face_a] and
+ * recv_a[var][face_a] = send_b[var][face_b] (device copies, asynchronous on
+ * `stream`).  Both faces must be seam faces with the same points; a == b with
+ * face_a != face_b is the periodic self-wrap of cs_mpigo. */
+int cs_exchange(cs_numerics* a, int face_a, cs_numerics* b, int face_b, int axis, int var, cs_stream stream);
 /* numerics_pc::deriv1d<Axis>, src/numerics_pc.hpp:74-119: out = d(in)/d(xi_axis),
  * closure rows where nstart/nend == 0, seam rows (reading recv) where 1. */
 int cs_deriv1d(cs_numerics* num, int axis, const double* in, double* out, int nstart, int nend,
