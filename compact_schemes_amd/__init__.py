@@ -29,7 +29,7 @@ ERRORS = {1: "CS_ERR_ARG", 2: "CS_ERR_SHAPE", 3: "CS_ERR_HIP", 4: "CS_ERR_UNSUPP
 EXPORTS = [
     "cs_last_error", "cs_version",
     "cs_numerics_create", "cs_numerics_destroy", "cs_deriv_setup", "cs_set_interface_table",
-    "cs_get_interface_table", "cs_numerics_buffers", "cs_numerics_ndf", "cs_fill_buffers", "cs_mpigo", "cs_exchange",
+    "cs_get_interface_table", "cs_numerics_buffers", "cs_numerics_ndf", "cs_fill_buffers", "cs_mpigo", "cs_exchange", "cs_metrics",
     "cs_deriv1d", "cs_deriv2d", "cs_deriv", "cs_filter", "cs_set_filter_alpha",
     "cs_physics_create", "cs_physics_destroy", "cs_physics_set_frame", "cs_physics_buffers",
     "cs_primitives", "cs_calc_fluxes", "cs_calc_viscous_shear_stress", "cs_calc_time_step",
@@ -101,6 +101,7 @@ def lib():
             "cs_fill_buffers": (I, [P, P, I, I, I, P]),
             "cs_mpigo": (I, [P, I, P]),
             "cs_exchange": (I, [P, I, P, I, I, I, P]),
+            "cs_metrics": (I, [P, P, P, P, P, P, P, P]),
             "cs_deriv1d": (I, [P, I, P, P, I, I, I, P]),
             "cs_deriv2d": (I, [P, I, P, P, I, I, I, I, P]),
             "cs_deriv": (I, [P, I, P, P, I, P]),
@@ -268,6 +269,20 @@ class Numerics:
     def exchange(self, face, other, other_face, axis, var, stream=None):
         """Seam exchange with another context on this GPU (cs_exchange)."""
         check(lib().cs_exchange(self.h, int(face), other.h, int(other_face), int(axis), int(var), _stream(stream)))
+
+    def metrics(self, xyz, period=None, stream=None):
+        """(xim, etm, zem, yaco) from [3][lmx] coordinates (cs_metrics);
+        period: 3x3 coordinate jumps over one period of each seam axis."""
+        import torch
+        lmx = self.lmx
+        xim, etm, zem = (torch.empty(3 * lmx, dtype=torch.float64, device="cuda") for _ in range(3))
+        yaco = torch.empty(lmx, dtype=torch.float64, device="cuda")
+        per = None
+        if period is not None:
+            per = (ctypes.c_double * 9)(*[float(v) for row in period for v in row])
+            per = ctypes.cast(per, ctypes.c_void_p)
+        check(lib().cs_metrics(self.h, _p(xyz), per, _p(xim), _p(etm), _p(zem), _p(yaco), _stream(stream)))
+        return xim, etm, zem, yaco
 
     def deriv1d(self, axis, fin, fout, nstart, nend, var=0, stream=None):
         check(lib().cs_deriv1d(self.h, axis, _p(fin), _p(fout), nstart, nend, var, _stream(stream)))

@@ -99,6 +99,15 @@ int cs_deriv1d(cs_numerics* num, int axis, const double* in, double* out, int ns
  * `comp` of in ([3][lmx] per variable), writes component `axis` of out. */
 int cs_deriv2d(cs_numerics* num, int axis, const double* in, double* out, int nstart, int nend,
                int var, int comp, cs_stream stream);
+/* grid metrics from point coordinates (no reference counterpart: the
+ * reference's grid allocates xim/etm/zem and never fills them,
+ * src/grid.hpp:66-72): xyz = [3][lmx] coordinates x, y, z; period9 = [axis][c]
+ * the coordinate jump over one period of each seam axis (NULL = none).
+ * Writes xim, etm, zem ([3][lmx] each: grad(xi)/J, grad(eta)/J, grad(zeta)/J
+ * as cofactors of d x_c / d xi_a, each from compact derivatives with the
+ * context's BCs) and yaco = J = 1 / det(d x_c / d xi_a). */
+int cs_metrics(cs_numerics* num, const double* xyz, const double* period9, double* xim, double* etm, double* zem,
+               double* yaco, cs_stream stream);
 /* fill + mpigo + deriv1d in one launch: the seam sums are formed from the
  * opposite end of each line inside the kernel (no send/recv traffic).  Same
  * result as cs_fill_buffers + cs_mpigo + cs_deriv1d with the context's ndf. */
