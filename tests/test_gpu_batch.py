@@ -101,15 +101,17 @@ def test_batch_grid_modes_agree(mode):
 
 
 @pytest.mark.gpu
-def test_batch_graph_replays_match_eager():
+@pytest.mark.parametrize("mode", [0, 1, 2], ids=["per-item", "persistent", "per-subdomain"])
+def test_batch_graph_replays_match_eager(mode):
     import torch
     dims = _dims(4, 6)
     nbc = [PER] * len(dims)
     fields = _fields(dims, 200)
     b, f, d = _batch(dims, nbc, fields)
     be, fe, de = _batch(dims, nbc, fields)
+    b.set_persistent(mode)
     st = torch.cuda.Stream()
-    case("batch graph capture")
+    case(f"batch graph capture mode {mode}")
     b.capture(st)
     torch.cuda.synchronize()
     for rep in range(3):
