@@ -78,6 +78,7 @@ int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, dou
 int build_axis_table(int n, int seam, int K, int P, std::vector<double>& h);
 bool filter_fusable(const cs_solver* sv);
 int filter_sweeps(cs_solver* sv, double* q, hipStream_t s, int axis = -1);
+int step_filter(cs_solver* sv, double* qa, hipStream_t s);
 // adjoint (vjp.hip: per-operator products; tape.hip: fused transposed sweeps)
 bool adjoint_fused(const cs_solver* sv);
 long tape_doubles(const cs_solver* sv);

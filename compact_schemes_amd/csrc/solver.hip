@@ -48,6 +48,20 @@ int build_axis_table(int n, int seam, int K, int P, std::vector<double>& h) {
     return CS_OK;
 }
 
+// the filter that closes an RK4 step (cfg.filter): one five-variable sweep
+// per axis when every axis splits into 8-point chunks, else the line solver
+// with the five variables in one launch per axis
+int step_filter(cs_solver* sv, double* qa, hipStream_t s) {
+    if (!sv->cfg.filter) return CS_OK;
+    if (filter_fusable(sv)) return filter_sweeps(sv, qa, s);
+    for (int a = 0; a < 3; a++) {
+        const int sp = sv->ax[a].seam;
+        const int rc = line_apply(sv->num, OP_FILTER, a, qa, qa, sp, sp, nullptr, 0, false, s, 5);
+        if (rc) return rc;
+    }
+    return CS_OK;
+}
+
 }  // namespace fused
 }  // namespace cs
 
@@ -231,15 +245,7 @@ int cs_solver_rk4_step(cs_solver* sv, double* qa, const double* sbcc, double dt,
     if ((rc = rhs_sweeps(sv, sv->qb, sbcc, sv->qc, qa, 3, s))) return rc;
     if ((rc = rhs_sweeps(sv, sv->qc, sbcc, sv->qb, qa, 2, s))) return rc;
     if ((rc = rhs_sweeps(sv, sv->qb, sbcc, qa, qa, 1, s))) return rc;
-    if (sv->cfg.filter && filter_fusable(sv)) {
-        if ((rc = filter_sweeps(sv, qa, s))) return rc;
-    } else if (sv->cfg.filter) {
-        // the five conserved variables in one launch per axis
-        for (int a = 0; a < 3; a++) {
-            const int sp = sv->ax[a].seam;
-            if ((rc = line_apply(sv->num, OP_FILTER, a, qa, qa, sp, sp, nullptr, 0, false, s, 5))) return rc;
-        }
-    }
+    if ((rc = step_filter(sv, qa, s))) return rc;
     if (dt_out) {
         CS_HIP(hipMemcpyAsync(sv->dt_host, sv->dt, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
         CS_HIP(hipStreamSynchronize(s));

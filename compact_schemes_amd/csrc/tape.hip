@@ -690,8 +690,7 @@ int step_recording(cs_solver* sv, double* qa, const double* sbcc, double dt, dou
     }
     sv->rec = nullptr;
     if (rc) return rc;
-    if (sv->cfg.filter) return filter_sweeps(sv, qa, s);
-    return CS_OK;
+    return step_filter(sv, qa, s);
 }
 
 }  // namespace
