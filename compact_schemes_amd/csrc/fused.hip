@@ -45,7 +45,7 @@ template <int AX, bool CLOSE>
 struct S1 {
     static constexpr int NF = 4;
     static constexpr bool kGcbc = false;   // gradients only; BC updates act on the flux sweeps
-    static constexpr bool kPv = CLOSE;     // the closure needs u, v, w, c^2 at its rows
+    static constexpr bool kPv = false;     // the closure re-derives u, v, w, c^2 from q at its rows
     static constexpr int kH = 2;           // derivative stencil half-width
     static constexpr int kWpe = 3;         // waves per SIMD of the y/z kernels (168 VGPRs)
     static constexpr bool kFilter = false;
@@ -82,8 +82,10 @@ struct S1 {
 #pragma unroll
                 for (int a = 0; a < 3; a++) g[3 * f + a] = ph.m[a] * gr[a][f];
             double c2g[3] = {ph.m[0] * gr[0][3], ph.m[1] * gr[1][3], ph.m[2] * gr[2][3]};
-            double t6[6], h3[3], d4;
-            viscous_closure(ph.mu, ph.J, g, c2g, pv, t6, h3, &d4);
+            double t6[6], h3[3], d4, qq[5];
+            const Prim p = prim_at(q, bo, qq);      // q of this point: just read by this sweep
+            const double uvw[3] = {p.u, p.v, p.w};
+            viscous_closure(ph.mu, ph.J, g, c2g, uvw, t6, h3, &d4);
 #pragma unroll
             for (int k = 0; k < 6; k++) stg(tau[k], bo, t6[k]);
 #pragma unroll
