@@ -1,7 +1,7 @@
 // cs_oracle.cpp -- CPU restatement of the compact-scheme hot path.
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
-// compact_schemes_amd/: it is loaded by tests/, by __graft_entry__.smoke() and
+// TEST INFRASTRUCTURE ONLY.  The HIP library in compact_schemes_amd/ is compared
+// against this file: it is loaded by tests/, by __graft_entry__.smoke() and
 // by bench.py's cpu_baseline leg, never by the product path.
 //
 // Each function restates the reference algorithm named in its comment

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/cs_oracle.cpp).
 
 Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product path never imports
+cpu_baseline leg of bench.py, to compare against.  The product path never imports
 this module.  All arrays are float64 numpy arrays in the reference layout
 (SoA, variable-major, x fastest).
 """

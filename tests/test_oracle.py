@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/cs_oracle.cpp) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/cs_oracle.cpp) before the GPU results are compared with it.
 
 Each test checks the oracle against something it does not share code with:
 the reference's own known-answer test and CPU twins (file:line cited), the
