@@ -64,6 +64,7 @@ struct S1 {
         v[2] = p.w;
         v[3] = p.c2;
     }
+    __device__ __forceinline__ void fetch(uoff, double*) const {}
     __device__ __forceinline__ void store(uoff bo, const double* d, const double* pv) const {
         if constexpr (!CLOSE) {
 #pragma unroll
@@ -141,22 +142,28 @@ struct S2 {
             F[4] -= m * ldg(hx, bo);
         }
     }
-    __device__ __forceinline__ void store(uoff bo, const double* d, const double*) const {
-        if (MODE == 0) {
+    // the epilogue inputs of one row folded into d (loads only); store() then
+    // only writes.  The kernels fetch all their rows before their first store:
+    // dout may alias din or q0 for all the compiler knows, so a load placed
+    // after a store waits for it, and interleaved they would run one memory
+    // round trip per field and row.
+    __device__ __forceinline__ void fetch(uoff bo, double* d) const {
+        if (MODE == 1 || MODE == 2) {
 #pragma unroll
-            for (int f = 0; f < 5; f++) stg(dout[f], bo, d[f]);
-        } else if (MODE == 1 || MODE == 2) {
-#pragma unroll
-            for (int f = 0; f < 5; f++) stg(dout[f], bo, ldg(din[f], bo) + d[f]);
-        } else {
+            for (int f = 0; f < 5; f++) d[f] = ldg(din[f], bo) + d[f];
+        } else if (MODE == 3) {
             const double dtk = dtp[0] / (double)div;
             const double fj = dtk * ph.J;
 #pragma unroll
             for (int f = 0; f < 5; f++) {
                 const double de = ldg(din[f], bo) + d[f];
-                stg(dout[f], bo, ldg(q0[f], bo) - fj * de);
+                d[f] = ldg(q0[f], bo) - fj * de;
             }
         }
+    }
+    __device__ __forceinline__ void store(uoff bo, const double* d, const double*) const {
+#pragma unroll
+        for (int f = 0; f < 5; f++) stg(dout[f], bo, d[f]);
     }
     // non-reflective characteristic correction at a face point from the
     // boundary derivatives of this sweep (gcbc_setup_kernel +
@@ -202,6 +209,7 @@ struct SF {
 #pragma unroll
         for (int f = 0; f < 5; f++) v[f] = ldg(q[f], bo);
     }
+    __device__ __forceinline__ void fetch(uoff, double*) const {}
     __device__ __forceinline__ void store(uoff bo, const double* d, const double*) const {
 #pragma unroll
         for (int f = 0; f < 5; f++) stg(q[f], bo, d[f]);
@@ -223,6 +231,7 @@ struct SF1 {
     long lmx;
     __device__ __forceinline__ double* field() const { return q + (long)blockIdx.y * lmx; }
     __device__ __forceinline__ void point(uoff bo, double* v) const { v[0] = ldg(field(), bo); }
+    __device__ __forceinline__ void fetch(uoff, double*) const {}
     __device__ __forceinline__ void store(uoff bo, const double* d, const double*) const { stg(field(), bo, d[0]); }
     __device__ __forceinline__ void gcbc(uoff, int, const double*, double*) const {}
 };
@@ -611,6 +620,16 @@ direct_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
 
     if (!work) return;
     face_corrections<K, NF, SW, GCBC, true>(sw, base, st, N, s, K, c == 0, c == P - 1, r, cm.sbcc);
+    // every row's epilogue loads before the first store
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        double d[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) d[f] = r[f][t];
+        sw.fetch(base + (uoff)(s + t) * st, d);
+#pragma unroll
+        for (int f = 0; f < NF; f++) r[f][t] = d[f];
+    }
 #pragma unroll
     for (int t = 0; t < K; t++) {
         double d[NF], p[NF];
@@ -749,6 +768,19 @@ __global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW 
 #pragma unroll
         for (int j = 0; j < J; j++) xo[f][j] = lds[slot[j]];
     }
+    // every point's epilogue loads before the first store
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const int p = lane + 64 * j;
+        if (p < npts) {
+            double d[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) d[f] = xo[f][j];
+            sw.fetch(run + (uoff)p * st, d);
+#pragma unroll
+            for (int f = 0; f < NF; f++) xo[f][j] = d[f];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < J; j++) {
         const int p = lane + 64 * j;
@@ -850,6 +882,18 @@ __global__ void __launch_bounds__(AX == 0 ? 256 : kYLines * kMaxP) ragged_kernel
 
     if (!work) return;
     face_corrections<K, NF, SW, GCBC, false>(sw, base, st, N, s, L, c == 0, c == P - 1, r, cm.sbcc);
+    // every row's epilogue loads before the first store
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        if (t < L) {
+            double d[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) d[f] = r[f][t];
+            sw.fetch(base + (uoff)(s + t) * st, d);
+#pragma unroll
+            for (int f = 0; f < NF; f++) r[f][t] = d[f];
+        }
+    }
 #pragma unroll
     for (int t = 0; t < K; t++) {
         if (t < L) {
@@ -963,6 +1007,17 @@ __global__ void __launch_bounds__(TL * kMaxP) tile_kernel(AxisDesc ax, SweepComm
 #pragma unroll
         for (int t = 0; t < K; t++) {
             if (EX || t < L) {
+                double d[NF];
+#pragma unroll
+                for (int f = 0; f < NF; f++) d[f] = r[f][t];
+                sw.fetch(base + (uoff)(s + t) * st, d);
+#pragma unroll
+                for (int f = 0; f < NF; f++) r[f][t] = d[f];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < K; t++) {
+            if (EX || t < L) {
                 double d[NF], p[NF];
 #pragma unroll
                 for (int f = 0; f < NF; f++) { d[f] = r[f][t]; p[f] = SW::kPv ? pv[SW::kPv ? f : 0][t] : 0.0; }
@@ -993,7 +1048,9 @@ __global__ void __launch_bounds__(TL * kMaxP) tile_kernel(AxisDesc ax, SweepComm
             double d[NF], pz[NF];
 #pragma unroll
             for (int f = 0; f < NF; f++) { d[f] = lds[(f * RB + row - rs) * TLP + l]; pz[f] = 0.0; }
-            sw.store(line_off(AX, (uoff)ln, lxi, plane) + (uoff)row * st, d, pz);
+            const uoff bo = line_off(AX, (uoff)ln, lxi, plane) + (uoff)row * st;
+            sw.fetch(bo, d);
+            sw.store(bo, d, pz);
         }
     }
 }

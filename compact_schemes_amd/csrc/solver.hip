@@ -130,9 +130,9 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
         ax.seam = seam[a];
         ax.gcbc = gc[a];
         // measured defaults (256^3, MI355X): x lines run the one-wave transposing
-        // kernel; y lines 8 per workgroup with XCD-grouped workgroups; z lines 16
+        // kernel; y and z lines 16 per workgroup, y with XCD-grouped workgroups
         ax.variant = 0;
-        ax.tl = a == 1 ? 8 : 16;
+        ax.tl = 16;
         ax.xcd = a == 1 ? 1 : 0;
         // filter sweeps: x all five variables per wave, y/z one per workgroup
         ax.ffields = a == 0 ? 5 : 1;

@@ -55,6 +55,7 @@ __device__ __forceinline__ void flux_jt(const double* q, const double* nu, doubl
 template <int AX, int MODE>
 struct AT {
     static constexpr int NF = 5;
+    static constexpr int kWpe = 3;        // waves per SIMD of the y/z kernels (168 VGPRs)
     const double* q[5];       // the stage state the RHS was evaluated at
     const double* t[5];       // seed
     const double* gin[5];
@@ -69,28 +70,40 @@ struct AT {
 #pragma unroll
         for (int f = 0; f < 5; f++) v[f] = ldg(t[f], bo);
     }
-    __device__ __forceinline__ void store(uoff bo, const double* nu, int face, long line) const {
+    // J^T nu at one point folded into d with the stage's combination (loads
+    // only); store() then writes.  The kernels fetch all their points before
+    // their first store: the outputs may alias q, gin or lam for all the
+    // compiler knows, so a load placed after a store waits for it.
+    __device__ __forceinline__ void fetch(uoff bo, double* d, int face, long line) const {
         double qq[5], g[5];
 #pragma unroll
         for (int v = 0; v < 5; v++) qq[v] = ldg(q[v], bo);
-        flux_jt<AX>(qq, nu, m, uf, g);
+        flux_jt<AX>(qq, d, m, uf, g);
         if (G && face >= 0) {
 #pragma unroll
             for (int v = 0; v < 5; v++) g[v] += G[(face * 5 + v) * nface + line];
         }
 #pragma unroll
         for (int v = 0; v < 5; v++) {
-            if (MODE == 0) {
-                stg(gout[v], bo, g[v]);
-            } else if (MODE == 1) {
-                stg(gout[v], bo, ldg(gin[v], bo) + g[v]);
-            } else if (MODE == 2) {
-                const double tn = -ck * (ldg(gin[v], bo) + g[v]);
-                stg(tout[v], bo, tn);
-                stg(lam[v], bo, ldg(lam[v], bo) + tn);
-            } else {
-                stg(lam[v], bo, ldg(lam[v], bo) - ck * (ldg(gin[v], bo) + g[v]));
+            if (MODE == 0) d[v] = g[v];
+            else if (MODE == 1) d[v] = ldg(gin[v], bo) + g[v];
+            else if (MODE == 2) d[v] = -ck * (ldg(gin[v], bo) + g[v]);
+            else d[v] = ldg(lam[v], bo) - ck * (ldg(gin[v], bo) + g[v]);
+        }
+    }
+    __device__ __forceinline__ void store(uoff bo, const double* d) const {
+        if (MODE == 2) {
+            double l[5];
+#pragma unroll
+            for (int v = 0; v < 5; v++) l[v] = ldg(lam[v], bo);
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                stg(tout[v], bo, d[v]);
+                stg(lam[v], bo, l[v] + d[v]);
             }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 5; v++) stg(MODE == 3 ? lam[v] : gout[v], bo, d[v]);
         }
     }
 };
@@ -100,6 +113,7 @@ struct AT {
 template <int NFV>
 struct FT {
     static constexpr int NF = NFV;
+    static constexpr int kWpe = NFV == 1 ? 4 : 2;
     double* q;
     long lmx;
     __device__ __forceinline__ double* field(int f) const {
@@ -109,7 +123,8 @@ struct FT {
 #pragma unroll
         for (int f = 0; f < NFV; f++) v[f] = ldg(field(f), bo);
     }
-    __device__ __forceinline__ void store(uoff bo, const double* d, int, long) const {
+    __device__ __forceinline__ void fetch(uoff, double*, int, long) const {}
+    __device__ __forceinline__ void store(uoff bo, const double* d) const {
 #pragma unroll
         for (int f = 0; f < NFV; f++) stg(field(f), bo, d[f]);
     }
@@ -344,6 +359,7 @@ __global__ void __launch_bounds__(64) xtT_kernel(AxisDesc ax, SweepCommon cm, SW
 #pragma unroll
         for (int j = 0; j < J; j++) xo[f][j] = lds[slot[j]];
     }
+    // every point's loads before the first store
 #pragma unroll
     for (int j = 0; j < J; j++) {
         const int p = lane + 64 * j;
@@ -353,7 +369,19 @@ __global__ void __launch_bounds__(64) xtT_kernel(AxisDesc ax, SweepCommon cm, SW
             for (int f = 0; f < NF; f++) d[f] = xo[f][j];
             const int row = lrw[j] & 1023;
             const int face = row == 0 ? 0 : (row == N - 1 ? 1 : -1);
-            sw.store(run + (uoff)p * 8u, d, face, line0 + (lrw[j] >> 10));
+            sw.fetch(run + (uoff)p * 8u, d, face, line0 + (lrw[j] >> 10));
+#pragma unroll
+            for (int f = 0; f < NF; f++) xo[f][j] = d[f];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const int p = lane + 64 * j;
+        if (p < npts) {
+            double d[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) d[f] = xo[f][j];
+            sw.store(run + (uoff)p * 8u, d);
         }
     }
 }
@@ -364,7 +392,7 @@ __global__ void __launch_bounds__(64) xtT_kernel(AxisDesc ax, SweepCommon cm, SW
 // by LDS folds, halo rows of the solution through LDS one field at a time
 // ---------------------------------------------------------------------------
 template <int K, int AX, int TL, class SW, int ST>
-__global__ void __launch_bounds__(TL * kMaxP) directT_kernel(AxisDesc ax, SweepCommon cm, SW sw, const double* tabT,
+__global__ void __launch_bounds__(TL * kMaxP) __attribute__((amdgpu_waves_per_eu(SW::kWpe))) directT_kernel(AxisDesc ax, SweepCommon cm, SW sw, const double* tabT,
                                                              const double* mu, const double* an) {
     constexpr int NF = SW::NF;
     constexpr int H = ST == 0 ? 2 : 5;
@@ -447,6 +475,7 @@ __global__ void __launch_bounds__(TL * kMaxP) directT_kernel(AxisDesc ax, SweepC
     }
 
     if (!work) return;
+    // every row's loads before the first store
 #pragma unroll
     for (int t = 0; t < K; t++) {
         const int row = s + t;
@@ -454,7 +483,16 @@ __global__ void __launch_bounds__(TL * kMaxP) directT_kernel(AxisDesc ax, SweepC
 #pragma unroll
         for (int f = 0; f < NF; f++) d[f] = r[f][t];
         const int face = row == 0 ? 0 : (row == N - 1 ? 1 : -1);
-        sw.store(base + (uoff)row * st, d, face, line);
+        sw.fetch(base + (uoff)row * st, d, face, line);
+#pragma unroll
+        for (int f = 0; f < NF; f++) r[f][t] = d[f];
+    }
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+        double d[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) d[f] = r[f][t];
+        sw.store(base + (uoff)(s + t) * st, d);
     }
 }
 

@@ -80,7 +80,7 @@ class FusedSolver:
         self.de = torch.zeros(5 * lmx, dtype=torch.float64, device="cuda")
         self.num = None
         self.nbc_flat = flat
-        self._tuned = {0: (0, 16, 0), 1: (0, 8, 1), 2: (0, 16, 0)}
+        self._tuned = {0: (0, 16, 0), 1: (0, 16, 1), 2: (0, 16, 0)}
         self._ftuned = {0: (5, 16), 1: (1, 16), 2: (1, 16)}
         self.order = 1
         # the filter of cs_solver_rk4_step runs as one sweep per axis when every
