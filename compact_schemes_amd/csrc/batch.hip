@@ -128,7 +128,7 @@ static int build_phase(cs_batch* b, Phase& p, int op, int axis_lo, int axis_hi) 
                 var.push_back(l);
                 v = (int)var.size() - 1;
             }
-            const int TL = batch_lines_per_item(K, axis);
+            const int TL = batch_lines_per_item(K, axis, a.P);
             var[v].threads = std::max(var[v].threads, TL * a.P);
             var[v].lds = std::max(var[v].lds, batch_lds_bytes(K, axis, a.N));
             const int idx = (int)args.size();
@@ -155,7 +155,7 @@ static int build_phase(cs_batch* b, Phase& p, int op, int axis_lo, int axis_hi) 
         while (pos < work.size() && work[pos].variant == (int)v) pos++;
         l.nitems = (int)pos - l.first;
         int nb = 1;
-        int rc = batch_blocks_per_cu(op, l.K, l.axis, l.threads, l.lds, &nb);
+        int rc = batch_blocks_per_cu(op, l.K, l.axis, l.P, l.threads, l.lds, &nb);
         if (rc) return rc;
         l.grid = std::max(1, std::min(l.nitems, ncu * nb));
     }
@@ -205,7 +205,8 @@ static int run_phase(cs_batch* b, int k, int persistent, hipStream_t s) {
     const Phase& p = b->ph[k];
     for (const Launch& l : p.launches) {
         const int grid = persistent ? l.grid : l.nitems;
-        int rc = launch_batch_lines(p.op, l.K, l.axis, p.args, p.items + l.first, l.nitems, grid, l.threads, l.lds, s);
+        int rc = launch_batch_lines(p.op, l.K, l.axis, l.P, p.args, p.items + l.first, l.nitems, grid, l.threads,
+                                    l.lds, s);
         if (rc) return rc;
     }
     return CS_OK;
@@ -366,7 +367,7 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
         float best = 1e30f;
         int bestK = 0;
         for (int K : {8, 16, 32}) {
-            if ((c.n + K - 1) / K > 16) continue;
+            if ((c.n + K - 1) / K > max_chunks(K)) continue;
             LineOpTable t;
             if ((rc = make_line_op(num, c.op, c.n, start, end, K, &t))) break;
             LineArgs a;
@@ -425,7 +426,7 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
 int cs_tune_set(int op, int n, int K) {
     if ((op != OP_DERIV && op != OP_FILTER) || n < 8 || (K != 0 && K != 8 && K != 16 && K != 32))
         return fail(CS_ERR_ARG, "op 0/1, n >= 8, K in {0, 8, 16, 32}");
-    if (K != 0 && (n + K - 1) / K > 16) return fail(CS_ERR_SHAPE, "chunk length too short for this line");
+    if (K != 0 && (n + K - 1) / K > max_chunks(K)) return fail(CS_ERR_SHAPE, "chunk length too short for this line");
     set_tuned_chunk(op, n, K);
     return CS_OK;
 }

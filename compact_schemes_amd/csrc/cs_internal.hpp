@@ -143,12 +143,16 @@ struct LineArgs {
 
 int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s);
 
-// batched line operators (batch.hip drives them, line_solver.hip launches)
-int batch_lines_per_item(int K, int axis);
+// most chunks per line of a K-point chunked line operator
+int max_chunks(int K);
+
+// batched line operators (batch.hip drives them, line_solver.hip launches);
+// one launch holds operators of one chunk length K, axis and chunk count P
+int batch_lines_per_item(int K, int axis, int P);
 size_t batch_lds_bytes(int K, int axis, int N);
-int batch_blocks_per_cu(int op, int K, int axis, int threads, size_t lds, int* nb);
-int launch_batch_lines(int op, int K, int axis, const LineArgs* args, const int2* items, int nitems, int grid,
-                       int threads, size_t lds, hipStream_t s);
+int batch_blocks_per_cu(int op, int K, int axis, int P, int threads, size_t lds, int* nb);
+int launch_batch_lines(int op, int K, int axis, int P, const LineArgs* args, const int2* items, int nitems,
+                       int grid, int threads, size_t lds, hipStream_t s);
 
 // chunk-length choices per (operator, line length) made by the autotuner;
 // 0 = none (the default rule of choose_chunk applies)

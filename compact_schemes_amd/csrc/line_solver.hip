@@ -11,13 +11,15 @@
 // A line is cut into P chunks of K points, one thread per chunk: each thread
 // runs both recurrences on its chunk with zero carry-in, and the carries are
 // propagated across chunks through LDS using the precomputed chunk products
-// (Mf, Rb).  Any line length 8..512 works.
+// (Mf, Rb).  Any line length 8..1024 works (x lines: as long as the LDS
+// tile fits).
 //
 // Layouts (one workgroup = TL lines x P chunks, threads line-fastest):
-//   y/z lines (AX = 1, 2): TL = 64 (32 for 32-point chunks), lanes of a
-//     wave are consecutive lines, so every load and store of a wave touches
-//     consecutive doubles of one row; with TL = 64 the chunk index is
-//     wave-uniform and the tables are read with scalar loads;
+//   y/z lines (AX = 1, 2): TL = 64 up to 16 chunks, 16 for 17..64 chunks,
+//     32 for 32-point chunks; lanes of a wave are consecutive lines, so every
+//     load and store of a wave touches consecutive doubles of one row; with
+//     TL = 64 the chunk index is wave-uniform and the tables are read with
+//     scalar loads;
 //   x lines   (AX = 0): TL = 16, the 16 lines (contiguous in memory) are
 //     staged through LDS with coalesced loads and stores, transposed so
 //     that a thread reads its chunk conflict-free.
@@ -32,25 +34,33 @@
 namespace cs {
 
 constexpr int kXLines = 16;   // lines per workgroup for x lines
-constexpr int kMaxP   = 16;   // chunks per line
+constexpr int kMaxP   = 64;   // chunks per line (8- and 16-point chunks)
+constexpr int kWideP  = 16;   // chunks per line of 32-point chunks, and of 64-line y/z workgroups
 
-// lines per workgroup: 16 for x (LDS tile), 64 for y/z, 32 for y/z with
-// 32-point chunks so that the register file holds them (512 threads)
-__host__ __device__ constexpr int lines_per_block(int K, int AX) {
-    return AX == 0 ? kXLines : (K == 32 ? 32 : 64);
+// lines per workgroup: 16 for x (LDS tile); for y/z lines 64 (one chunk per
+// wave, 512-byte rows) up to kWideP chunks and 16 beyond (up to kMaxP: 1024
+// threads), 32 with 32-point chunks so that the register file holds them
+// (512 threads)
+__host__ __device__ constexpr int lines_per_block(int K, int AX, int P) {
+    return AX == 0 ? kXLines : (K == 32 ? 32 : (P <= kWideP ? 64 : 16));
 }
 __host__ __device__ constexpr int max_threads(int K) { return K == 32 ? 512 : 1024; }
+int max_chunks(int K) { return K == 32 ? kWideP : kMaxP; }
 
 // ---------------------------------------------------------------------------
 // host: tables
 // ---------------------------------------------------------------------------
 int choose_chunk(int n, int* K, int* P) {
-    static const int ks[] = {8, 16, 32};
-    for (int k : ks) {
-        int p = (n + k - 1) / k;
-        if (p <= kMaxP) { *K = k; *P = p; return CS_OK; }
+    // the shortest chunk that keeps 64-line y/z workgroups (n <= 256), else
+    // 16-point chunks in 16-line workgroups (the 32-point kernels hold one
+    // 5-wave workgroup per CU)
+    for (int k : {8, 16}) {
+        const int p = (n + k - 1) / k;
+        if (p <= kWideP) { *K = k; *P = p; return CS_OK; }
     }
-    return fail(CS_ERR_SHAPE, "line length " + std::to_string(n) + " exceeds 512");
+    const int p = (n + 15) / 16;
+    if (p <= kMaxP) { *K = 16; *P = p; return CS_OK; }
+    return fail(CS_ERR_SHAPE, "line length " + std::to_string(n) + " exceeds 1024");
 }
 
 // Order-2M centred filter coefficients: sum a_n = 1+2af, sum_{n>=1} a_n n^{2m}
@@ -169,7 +179,7 @@ int build_line_table(int op, int n, int start, int end, double af, int K, std::v
 // ---------------------------------------------------------------------------
 // Chunked solve of one line system in arithmetic T with tables in T (layout
 // of build_line_table): forward recurrence with zero carry-in, carries across
-// chunks through LDS (gend/xbeg, [kMaxP][TL] entries of T), backward
+// chunks through LDS (gend/xbeg, [P][TL] entries of T), backward
 // recurrence, carries, and the Sherman-Morrison correction of a cyclic
 // system.  Two workgroup barriers.
 template <int K, typename T>
@@ -239,9 +249,9 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
     constexpr int W = K + 2 * H;
     constexpr int TP = TL + 1;                      // padded LDS row (x tile)
 
-    double* gend = smem;                            // [kMaxP][TL]
-    double* xbeg = smem + kMaxP * TL;               // [kMaxP][TL]
-    double* tile = smem + 2 * kMaxP * TL;           // x only: [N][TP]
+    double* gend = smem;                            // [P][TL]
+    double* xbeg = smem + a.P * TL;                 // [P][TL]
+    double* tile = smem + 2 * a.P * TL;             // x only: [N][TP]
 
     const int tid = threadIdx.x;
     const int lloc = tid % TL;
@@ -423,8 +433,8 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
         double x[K];
 #pragma unroll
         for (int t = 0; t < K; t++) x[t] = (double)x32[t];
-        double* first = smem + 2 * kMaxP * TL + (AX == 0 ? (size_t)N * TP : 0);   // [kMaxP][TL]
-        double* last = first + kMaxP * TL;
+        double* first = smem + 2 * P * TL + (AX == 0 ? (size_t)N * TP : 0);   // [P][TL]
+        double* last = first + P * TL;
         const int L = min(K, N - s);
         const double* Am = tab + 6 * N + 44;
         const double* Bm = Am + N;
@@ -506,30 +516,37 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
 }
 
 // grid y runs a.nfields fields a.fstride apart with the same operator
-template <int K, int OP, int AX, bool MIX>
+template <int K, int OP, int AX, bool MIX, int TL>
 __global__ void __launch_bounds__(max_threads(K)) line_kernel(LineArgs a) {
     extern __shared__ double smem[];
     if (blockIdx.y > 0) {
         a.in += (long)blockIdx.y * a.fstride;
         a.out += (long)blockIdx.y * a.fstride;
     }
-    line_body<K, OP, AX, lines_per_block(K, AX), MIX>(a, blockIdx.x, smem);
+    line_body<K, OP, AX, TL, MIX>(a, blockIdx.x, smem);
+}
+
+template <int K, int OP, int AX, bool MIX, int TL>
+static int launch_tl(const LineArgs& a, hipStream_t s) {
+    dim3 block(TL * a.P);
+    dim3 grid((unsigned)((a.nlines + TL - 1) / TL), (unsigned)(a.nfields > 1 ? a.nfields : 1));
+    size_t lds = sizeof(double) * 2 * a.P * TL;
+    if (AX == 0) lds += sizeof(double) * (size_t)a.N * (TL + 1);
+    if (MIX) lds += sizeof(double) * 2 * a.P * TL;
+    if (lds > 64 * 1024)
+        CS_HIP(hipFuncSetAttribute((const void*)line_kernel<K, OP, AX, MIX, TL>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((line_kernel<K, OP, AX, MIX, TL>), grid, block, lds, s, a);
+    CS_HIP(hipGetLastError());
+    return CS_OK;
 }
 
 template <int K, int OP, int AX, bool MIX>
 static int launch_mix(const LineArgs& a, hipStream_t s) {
-    constexpr int TL = lines_per_block(K, AX);
-    dim3 block(TL * a.P);
-    dim3 grid((unsigned)((a.nlines + TL - 1) / TL), (unsigned)(a.nfields > 1 ? a.nfields : 1));
-    size_t lds = sizeof(double) * 2 * kMaxP * TL;
-    if (AX == 0) lds += sizeof(double) * (size_t)a.N * (TL + 1);
-    if (MIX) lds += sizeof(double) * 2 * kMaxP * TL;
-    if (lds > 64 * 1024)
-        CS_HIP(hipFuncSetAttribute((const void*)line_kernel<K, OP, AX, MIX>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((line_kernel<K, OP, AX, MIX>), grid, block, lds, s, a);
-    CS_HIP(hipGetLastError());
-    return CS_OK;
+    if constexpr (AX != 0 && K != 32) {
+        if (lines_per_block(K, AX, a.P) == 16) return launch_tl<K, OP, AX, MIX, 16>(a, s);
+    }
+    return launch_tl<K, OP, AX, MIX, lines_per_block(K, AX, 1)>(a, s);
 }
 
 template <int K, int OP, int AX>
@@ -548,10 +565,10 @@ static int launch_axis(int axis, const LineArgs& a, hipStream_t s) {
 }
 
 int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s) {
-    if (a.P < 1 || a.P > kMaxP) return fail(CS_ERR_SHAPE, "chunk count out of range");
+    if (a.P < 1 || a.P > max_chunks(K)) return fail(CS_ERR_SHAPE, "chunk count out of range");
     if (a.nfields > 1 && (a.drva || a.recv)) return fail(CS_ERR_ARG, "multi-field launches write no boundary data");
     if ((long)a.P * K < a.N) return fail(CS_ERR_SHAPE, "chunks do not cover the line");
-    if (axis == 0 && (size_t)a.N * (kXLines + 1) * 8 + 4 * kMaxP * kXLines * 8 > 160 * 1024)
+    if (axis == 0 && (size_t)a.N * (kXLines + 1) * 8 + 4 * (size_t)a.P * kXLines * 8 > 160 * 1024)
         return fail(CS_ERR_SHAPE, "x line too long for the LDS tile");
     if (a.mixed && !a.tab32) return fail(CS_ERR_STATE, "mixed precision needs the FP32 tables");
     if (op == OP_DERIV) {
@@ -577,7 +594,7 @@ int launch_line(int op, int axis, int K, const LineArgs& a, hipStream_t s) {
 // args[items[i].x]; workgroups take items round-robin.  Same body and lines
 // per workgroup as line_kernel, so the same results bit for bit.
 // ---------------------------------------------------------------------------
-template <int K, int OP, int AX>
+template <int K, int OP, int AX, int TL>
 __global__ void __launch_bounds__(max_threads(K)) batch_line_kernel(const LineArgs* args, const int2* items,
                                                                     int nitems) {
     extern __shared__ double smem[];
@@ -595,41 +612,48 @@ __global__ void __launch_bounds__(max_threads(K)) batch_line_kernel(const LineAr
         for (int i = 0; i < kWords; i++) words[i] = __builtin_amdgcn_readfirstlane(src[i]);
         LineArgs a;
         __builtin_memcpy(&a, words, sizeof(LineArgs));
-        line_body<K, OP, AX, lines_per_block(K, AX)>(a, blk, smem);
+        line_body<K, OP, AX, TL>(a, blk, smem);
         __syncthreads();   // the LDS of this item is free for the next
     }
 }
 
-int batch_lines_per_item(int K, int axis) { return lines_per_block(K, axis); }
+int batch_lines_per_item(int K, int axis, int P) { return lines_per_block(K, axis, P); }
 
 size_t batch_lds_bytes(int K, int axis, int N) {
-    const int TL = lines_per_block(K, axis);
-    size_t lds = sizeof(double) * 2 * kMaxP * TL;
+    const int P = (N + K - 1) / K;
+    const int TL = lines_per_block(K, axis, P);
+    size_t lds = sizeof(double) * 2 * (size_t)P * TL;
     if (axis == 0) lds += sizeof(double) * (size_t)N * (TL + 1);
     return lds;
 }
 
+template <int K, int OP, int TL>
+static const void* batch_symbol_tl(int axis) {
+    if (axis == 0) return (const void*)batch_line_kernel<K, OP, 0, lines_per_block(K, 0, 1)>;
+    if (axis == 1) return (const void*)batch_line_kernel<K, OP, 1, TL>;
+    return (const void*)batch_line_kernel<K, OP, 2, TL>;
+}
+
 template <int K, int OP>
-static const void* batch_symbol(int axis) {
-    if (axis == 0) return (const void*)batch_line_kernel<K, OP, 0>;
-    if (axis == 1) return (const void*)batch_line_kernel<K, OP, 1>;
-    return (const void*)batch_line_kernel<K, OP, 2>;
+static const void* batch_symbol(int axis, int P) {
+    if (K != 32 && lines_per_block(K, axis, P) == 16) return batch_symbol_tl<K, OP, 16>(axis);
+    return batch_symbol_tl<K, OP, lines_per_block(K, 1, 1)>(axis);
 }
 
-static const void* batch_symbol(int op, int K, int axis) {
+static const void* batch_symbol(int op, int K, int axis, int P) {
     if (op == OP_DERIV) {
-        if (K == 8) return batch_symbol<8, OP_DERIV>(axis);
-        if (K == 16) return batch_symbol<16, OP_DERIV>(axis);
-        return batch_symbol<32, OP_DERIV>(axis);
+        if (K == 8) return batch_symbol<8, OP_DERIV>(axis, P);
+        if (K == 16) return batch_symbol<16, OP_DERIV>(axis, P);
+        return batch_symbol<32, OP_DERIV>(axis, P);
     }
-    if (K == 8) return batch_symbol<8, OP_FILTER>(axis);
-    if (K == 16) return batch_symbol<16, OP_FILTER>(axis);
-    return batch_symbol<32, OP_FILTER>(axis);
+    if (K == 8) return batch_symbol<8, OP_FILTER>(axis, P);
+    if (K == 16) return batch_symbol<16, OP_FILTER>(axis, P);
+    return batch_symbol<32, OP_FILTER>(axis, P);
 }
 
-int batch_blocks_per_cu(int op, int K, int axis, int threads, size_t lds, int* nb) {
+int batch_blocks_per_cu(int op, int K, int axis, int P, int threads, size_t lds, int* nb) {
     if (lds > 160 * 1024) return fail(CS_ERR_SHAPE, "batched x lines too long for the LDS tile");
-    const void* k = batch_symbol(op, K, axis);
+    const void* k = batch_symbol(op, K, axis, P);
     if (lds > 64 * 1024) CS_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int n = 0;
     CS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, lds));
@@ -637,28 +661,36 @@ int batch_blocks_per_cu(int op, int K, int axis, int threads, size_t lds, int* n
     return CS_OK;
 }
 
-template <int K, int OP>
-static void batch_launch(int axis, dim3 g, dim3 b, size_t lds, hipStream_t s, const LineArgs* args,
-                         const int2* items, int n) {
-    if (axis == 0) hipLaunchKernelGGL((batch_line_kernel<K, OP, 0>), g, b, lds, s, args, items, n);
-    else if (axis == 1) hipLaunchKernelGGL((batch_line_kernel<K, OP, 1>), g, b, lds, s, args, items, n);
-    else hipLaunchKernelGGL((batch_line_kernel<K, OP, 2>), g, b, lds, s, args, items, n);
+template <int K, int OP, int TL>
+static void batch_launch_tl(int axis, dim3 g, dim3 b, size_t lds, hipStream_t s, const LineArgs* args,
+                            const int2* items, int n) {
+    if (axis == 0)
+        hipLaunchKernelGGL((batch_line_kernel<K, OP, 0, lines_per_block(K, 0, 1)>), g, b, lds, s, args, items, n);
+    else if (axis == 1) hipLaunchKernelGGL((batch_line_kernel<K, OP, 1, TL>), g, b, lds, s, args, items, n);
+    else hipLaunchKernelGGL((batch_line_kernel<K, OP, 2, TL>), g, b, lds, s, args, items, n);
 }
 
-int launch_batch_lines(int op, int K, int axis, const LineArgs* args, const int2* items, int nitems, int grid,
-                       int threads, size_t lds, hipStream_t s) {
+template <int K, int OP>
+static void batch_launch(int axis, int P, dim3 g, dim3 b, size_t lds, hipStream_t s, const LineArgs* args,
+                         const int2* items, int n) {
+    if (K != 32 && lines_per_block(K, axis, P) == 16) batch_launch_tl<K, OP, 16>(axis, g, b, lds, s, args, items, n);
+    else batch_launch_tl<K, OP, lines_per_block(K, 1, 1)>(axis, g, b, lds, s, args, items, n);
+}
+
+int launch_batch_lines(int op, int K, int axis, int P, const LineArgs* args, const int2* items, int nitems,
+                       int grid, int threads, size_t lds, hipStream_t s) {
     if (nitems <= 0) return CS_OK;
     if (lds > 160 * 1024) return fail(CS_ERR_SHAPE, "batched x lines too long for the LDS tile");
-    if (threads > max_threads(K)) return fail(CS_ERR_SHAPE, "too many chunks for the batched launch");
+    if (threads > max_threads(K) || P > max_chunks(K)) return fail(CS_ERR_SHAPE, "too many chunks for the batched launch");
     const dim3 g(grid), b(threads);
     if (op == OP_DERIV) {
-        if (K == 8) batch_launch<8, OP_DERIV>(axis, g, b, lds, s, args, items, nitems);
-        else if (K == 16) batch_launch<16, OP_DERIV>(axis, g, b, lds, s, args, items, nitems);
-        else batch_launch<32, OP_DERIV>(axis, g, b, lds, s, args, items, nitems);
+        if (K == 8) batch_launch<8, OP_DERIV>(axis, P, g, b, lds, s, args, items, nitems);
+        else if (K == 16) batch_launch<16, OP_DERIV>(axis, P, g, b, lds, s, args, items, nitems);
+        else batch_launch<32, OP_DERIV>(axis, P, g, b, lds, s, args, items, nitems);
     } else {
-        if (K == 8) batch_launch<8, OP_FILTER>(axis, g, b, lds, s, args, items, nitems);
-        else if (K == 16) batch_launch<16, OP_FILTER>(axis, g, b, lds, s, args, items, nitems);
-        else batch_launch<32, OP_FILTER>(axis, g, b, lds, s, args, items, nitems);
+        if (K == 8) batch_launch<8, OP_FILTER>(axis, P, g, b, lds, s, args, items, nitems);
+        else if (K == 16) batch_launch<16, OP_FILTER>(axis, P, g, b, lds, s, args, items, nitems);
+        else batch_launch<32, OP_FILTER>(axis, P, g, b, lds, s, args, items, nitems);
     }
     CS_HIP(hipGetLastError());
     return CS_OK;
@@ -701,7 +733,7 @@ int make_line_op(const cs_numerics* num, int op, int n, int start, int end, int 
         if (K != 8 && K != 16 && K != 32) return fail(CS_ERR_ARG, "chunk length must be 8, 16 or 32");
         t->K = K;
         t->P = (n + K - 1) / K;
-        if (t->P > kMaxP) return fail(CS_ERR_SHAPE, "chunk length too short for this line");
+        if (t->P > max_chunks(K)) return fail(CS_ERR_SHAPE, "chunk length too short for this line");
     } else {
         int rc = choose_chunk(n, &t->K, &t->P);
         if (rc) return rc;

@@ -389,7 +389,7 @@ __global__ void __launch_bounds__(64) xtT_kernel(AxisDesc ax, SweepCommon cm, SW
 // ---------------------------------------------------------------------------
 // transposed sweep, y/z lines: TL lines x P chunks per workgroup, each thread
 // loads and stores its own chunk rows (coalesced across the TL lines), carries
-// by LDS folds, halo rows of the solution through LDS one field at a time
+// by LDS folds, halo rows of the solution through LDS
 // ---------------------------------------------------------------------------
 template <int K, int AX, int TL, class SW, int ST>
 __global__ void __launch_bounds__(TL * kMaxP) __attribute__((amdgpu_waves_per_eu(SW::kWpe))) directT_kernel(AxisDesc ax, SweepCommon cm, SW sw, const double* tabT,
@@ -447,31 +447,41 @@ __global__ void __launch_bounds__(TL * kMaxP) __attribute__((amdgpu_waves_per_eu
     fold_backward<NF, true, K>(xb, tabT, N, P, TL, c, lloc, X);
     apply_carry<K, NF, true>(tabT, 4 * N, s, K, r, X);
 
-    // halo rows hb[k*P + c][TL]: k < H the first H rows of chunk c, k >= H its
-    // last H rows (neighbours wrap on seam lines of the derivative)
+    // halo rows hb[(f*2H + k)*P + c][TL]: k < H the first H rows of chunk c,
+    // k >= H its last H rows (neighbours wrap on seam lines of the
+    // derivative); the derivative's five fields in one exchange, the filter
+    // one field at a time
     const bool wrap = ST == 0 && ax.seam;
     const int cl = c > 0 ? c - 1 : (wrap ? P - 1 : 0), cr = c + 1 < P ? c + 1 : (wrap ? 0 : P - 1);
+    constexpr int FX = ST == 0 ? NF : 1;      // fields per exchange
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
+    for (int f0 = 0; f0 < NF; f0 += FX) {
         __syncthreads();                  // earlier reads of the LDS are done
 #pragma unroll
-        for (int k = 0; k < H; k++) {
-            lds[(k * P + c) * TL + lloc] = r[f][k];
-            lds[((H + k) * P + c) * TL + lloc] = r[f][K - H + k];
+        for (int e = 0; e < FX; e++) {
+#pragma unroll
+            for (int k = 0; k < H; k++) {
+                lds[((e * 2 * H + k) * P + c) * TL + lloc] = r[f0 + e][k];
+                lds[((e * 2 * H + H + k) * P + c) * TL + lloc] = r[f0 + e][K - H + k];
+            }
         }
         __syncthreads();
-        double w[W];
 #pragma unroll
-        for (int k = 0; k < H; k++) {
-            w[k] = lds[((H + k) * P + cl) * TL + lloc];
-            w[K + H + k] = lds[(k * P + cr) * TL + lloc];
+        for (int e = 0; e < FX; e++) {
+            const int f = f0 + e;
+            double w[W];
+#pragma unroll
+            for (int k = 0; k < H; k++) {
+                w[k] = lds[((e * 2 * H + H + k) * P + cl) * TL + lloc];
+                w[K + H + k] = lds[((e * 2 * H + k) * P + cr) * TL + lloc];
+            }
+#pragma unroll
+            for (int t = 0; t < K; t++) w[t + H] = r[f][t];
+            double o[K];
+            stencil_t<K, ST, W>(w, o, s, N, c, P, ax.seam, cm.pb, an);
+#pragma unroll
+            for (int t = 0; t < K; t++) r[f][t] = o[t];
         }
-#pragma unroll
-        for (int t = 0; t < K; t++) w[t + H] = r[f][t];
-        double o[K];
-        stencil_t<K, ST, W>(w, o, s, N, c, P, ax.seam, cm.pb, an);
-#pragma unroll
-        for (int t = 0; t < K; t++) r[f][t] = o[t];
     }
 
     if (!work) return;
@@ -570,7 +580,7 @@ int launch_t(const AxisDesc& ax, const SweepCommon& cm, const SW& sw, const doub
         hipLaunchKernelGGL((xtT_kernel<K, SW, ST>), grid, block, lds, s, ax, cm, sw, tabT, mu, an);
     } else {
         const int TL = ax.tl == 8 ? 8 : 16;
-        const size_t halo = (size_t)2 * H * ax.P * TL;
+        const size_t halo = (size_t)(ST == 0 ? NF : 1) * 2 * H * ax.P * TL;
         const size_t carries = (size_t)2 * NF * ax.P * TL;
         const size_t lds = sizeof(double) * (halo > carries ? halo : carries);
         dim3 block(TL * ax.P), grid((unsigned)((ax.nlines + TL - 1) / TL), gy);

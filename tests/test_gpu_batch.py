@@ -143,7 +143,7 @@ def test_batch_autotune_keeps_identity(tmp_path):
         choices = {(op, n): bm.tune_get(op, n) for dm in dims for n in dm for op in (0, 1)}
         assert all(k in (8, 16, 32) for k in choices.values())
         for (op, n), k in choices.items():
-            assert (n + k - 1) // k <= 16
+            assert (n + k - 1) // k <= (16 if k == 32 else 64)      # chunks per line
         for s in range(len(f)):
             f[s].copy_(fields[s])
         b.step()

@@ -13,7 +13,8 @@ TOL = 1e-12
 NBC_CL = [[10, 10, 10], [10, 10, 10]]
 NBC_PER = [[45, 45, 45], [45, 45, 45]]
 SHAPES = [(64, 64, 64), (48, 40, 36), (256, 16, 12), (16, 256, 12), (12, 16, 256), (100, 33, 17),
-          (300, 9, 8), (9, 300, 8), (8, 9, 300), (512, 8, 8)]
+          (300, 9, 8), (9, 300, 8), (8, 9, 300), (512, 8, 8),
+          (700, 8, 8), (8, 1000, 8), (8, 9, 700)]   # up to 63 chunks per line (16 lines per workgroup)
 
 
 def _field(dims, seed):
