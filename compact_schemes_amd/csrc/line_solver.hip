@@ -296,7 +296,13 @@ __device__ __forceinline__ void line_body(const LineArgs& a, long blk, double* s
 #pragma unroll
         for (int t = 0; t < W; t++) {
             int row = s - H + t;
-            if (OP == OP_FILTER && a.cyclic) row = (row + N) % N;
+            if (OP == OP_FILTER && a.cyclic) {
+                // wrap without a division: the rows an output of this chunk
+                // reads lie in [-H, N + H); rows past those (for t beyond the
+                // chunk's last row) are clamped and not used
+                row = row < 0 ? row + N : (row >= N ? row - N : row);
+                row = min(row, N - 1);
+            }
             w[t] = (row >= 0 && row < N) ? fetch(row) : 0.0;
         }
         if constexpr (OP == OP_SOLVE) {
