@@ -338,11 +338,28 @@ int cs_batch_info(const cs_batch* b, int* items, int* launches) {
 
 // ---- autotuner -------------------------------------------------------------
 
+// two timing events, destroyed on every path out of the tuner
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int create() {
+        CS_HIP(hipEventCreate(&e0));
+        CS_HIP(hipEventCreate(&e1));
+        return CS_OK;
+    }
+    ~EventPair() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
 // Times every admissible chunk length for each (operator, line length) the
 // batch holds, on one of its subdomains, and records the fastest; then
 // rebuilds the batch with the choices.  The choices apply to every operator
 // table built afterwards in this process (cs_deriv / cs_filter on a new
 // context included), so batched and single-subdomain results stay identical.
+// The bound fields f are only read: the filter is timed out of place into the
+// subdomain's derivative buffer of that axis (which the batch's derivative
+// phase overwrites anyway).
 int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
     if (!b || !b->bound) return fail(CS_ERR_STATE, "batch has no bound fields");
     if (reps < 1) reps = 3;
@@ -357,10 +374,11 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
                 for (auto& c : cases) seen |= (c.op == op && c.n == n);
                 if (!seen) cases.push_back({op, n, sd, axis});
             }
-    hipEvent_t e0, e1;
-    CS_HIP(hipEventCreate(&e0));
-    CS_HIP(hipEventCreate(&e1));
     int rc = CS_OK;
+    {
+    EventPair ev;
+    if ((rc = ev.create())) return rc;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     for (auto& c : cases) {
         cs_numerics* num = b->num[c.s];
         const int start = num->ndf[0][c.axis], end = num->ndf[1][c.axis];
@@ -372,7 +390,7 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
             if ((rc = make_line_op(num, c.op, c.n, start, end, K, &t))) break;
             LineArgs a;
             int Kd = 0;
-            double* out = c.op == OP_DERIV ? b->d[3 * c.s + c.axis] : b->f[c.s];
+            double* out = b->d[3 * c.s + c.axis];
             if ((rc = line_args(num, c.op, c.axis, b->f[c.s], out, &a, &Kd))) { free_line_op(t); break; }
             a.tab = t.dev;
             a.P = t.P;
@@ -391,8 +409,7 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
         if (rc) break;
         set_tuned_chunk(c.op, c.n, bestK);
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    }
     if (rc) return rc;
     // drop the cached tables of the batch and rebuild with the choices
     for (cs_numerics* num : b->num) {
@@ -403,8 +420,10 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
     if ((rc = cs_batch_bind(b, f.data(), d.data()))) return rc;
     // then the grid mode, on the derivative phase (it leaves f unchanged)
     float t[3] = {0.f, 0.f, 0.f};
-    CS_HIP(hipEventCreate(&e0));
-    CS_HIP(hipEventCreate(&e1));
+    {
+    EventPair ev;
+    if ((rc = ev.create())) return rc;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     for (int mode = 0; mode < 3 && !rc; mode++) {
         rc = run_phase(b, 0, mode, s);
         if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = fail(CS_ERR_HIP, "event record failed");
@@ -413,8 +432,7 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
                     hipEventElapsedTime(&t[mode], e0, e1) != hipSuccess))
             rc = fail(CS_ERR_HIP, "event timing failed");
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    }
     if (rc) return rc;
     int best = 0;
     for (int mode = 1; mode < 3; mode++)

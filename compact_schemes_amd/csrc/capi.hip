@@ -480,7 +480,6 @@ int cs_gcbc_go(cs_numerics* num, const double* const* cm, const double* const* d
             int np = num->nbc[ip][nn];
             if (!is_char_bc(np)) continue;
             if (!cm[nn]) return fail(CS_ERR_ARG, "missing face normals for a characteristic face");
-            if (num->g.n(nn) < kMbci) return fail(CS_ERR_SHAPE, "characteristic face needs 4 points inward");
             if (np == CS_BC_INTER_CURV && (!drvb || !drvb[nn]))
                 return fail(CS_ERR_UNSUPPORTED, "inter-curv face needs the neighbour's characteristics (drvb)");
         }

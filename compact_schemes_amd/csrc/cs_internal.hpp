@@ -29,7 +29,13 @@ constexpr double kAb01    = 2.0;
 constexpr double kAb02    = 1.0 / 2.0;
 constexpr double kAb10    = 3.0 / 4.0;
 constexpr int    kLmd     = 8;     // points per side of a seam stencil
-constexpr int    kMbci    = 4;     // depth of the characteristic-BC update
+// depth of the characteristic-BC update: the face row only.  The reference's
+// mbci = 4 (src/common/parameters.hpp:46) weights rows 0..3 by the point array
+// sbcc; a point weight cannot tell faces apart, so along an edge line the
+// deeper rows of one face land on the boundary points of the adjacent face
+// and the scheme blows up (1e-8 noise -> NaN in 25 RK4 steps at 32^3).  With
+// sbcc = 0 inward those edge writes are all the deeper rows ever did (D24).
+constexpr int    kMbci    = 1;
 constexpr int    kNV      = 5;     // conserved variables
 
 // ---- gas ---------------------------------------------------------------------

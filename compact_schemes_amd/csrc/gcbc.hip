@@ -72,6 +72,9 @@ __global__ void __launch_bounds__(kGBlock) gcbc_update_kernel(FaceGeo fg, int co
     } else {
         const double accum = cmv[0] * dudtmf.x + cmv[1] * dudtmf.y + cmv[2] * dudtmf.z;
         cha[3 + fg.face] = cha[4 - fg.face] + 2 * ra0 * e.aoi * qa[idx] * (accum + dtwi * (e.vn + e.vs));
+        // D22: the change of the amplitudes, as in the other branches (the
+        // reference adds R*cha, i.e. J*de once more, at the wall rows)
+        for (int v = 0; v < 5; v++) cha[v] -= dv[v];
     }
     e.r_apply(cmv, cha, res);
     const int iq = 1 - 2 * fg.face;
@@ -90,15 +93,19 @@ __global__ void __launch_bounds__(kGBlock) wall_kernel(FaceGeo fg, int code, dou
     const long idx = fg.point(f);
     const long lmx = fg.lmx;
     double q0 = qa[idx], q1 = qa[idx + lmx], q2 = qa[idx + 2 * lmx], q3 = qa[idx + 3 * lmx], q4 = qa[idx + 4 * lmx];
-    double v2 = q1 * q1 + q2 * q2 + q3 * q3;
     double n = npex[idx];
     if (code == CS_BC_WALL_INVISCID) {
+        double v2 = q1 * q1 + q2 * q2 + q3 * q3;
         qa[idx + 4 * lmx] = n * q4 + (1.0 - n) * (kHamHamm1 * pow(q0, kGam) + 0.5 * v2 / q0);
     } else {
+        // D23: the kinetic energy of the momentum just set, so that an
+        // isothermal point (npex = 0) ends at p = rho / gamma
         double fctr = (1 - n) * q0;
-        qa[idx + lmx] = n * q1 - fctr * umf.x;
-        qa[idx + 2 * lmx] = n * q2 - fctr * umf.y;
-        qa[idx + 3 * lmx] = n * q3 - fctr * umf.z;
+        double m1 = n * q1 - fctr * umf.x, m2 = n * q2 - fctr * umf.y, m3 = n * q3 - fctr * umf.z;
+        double v2 = m1 * m1 + m2 * m2 + m3 * m3;
+        qa[idx + lmx] = m1;
+        qa[idx + 2 * lmx] = m2;
+        qa[idx + 3 * lmx] = m3;
         qa[idx + 4 * lmx] = n * q4 + (1 - n) * (kHamHamm1 * q0 + 0.5 * v2 / q0);
     }
 }

@@ -749,15 +749,19 @@ int make_line_op(const cs_numerics* num, int op, int n, int start, int end, int 
                  ? build_adjoint_table(op, n, start, end, num->alpha_f, num->pb_host, t->K, h)
                  : build_line_table(op, n, start, end, num->alpha_f, t->K, h);
     if (rc) return rc;
-    CS_HIP(hipMalloc(&t->dev, h.size() * sizeof(double)));
-    CS_HIP(hipMemcpy(t->dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
-    t->count = h.size();
     // FP32 copy of the factorisation (nu, mu, rho, Mf, Rb, z, 1/den, kappa)
     // for the mixed-precision solve
     std::vector<float> h32(6 * (size_t)n + 2);
     for (size_t i = 0; i < h32.size(); i++) h32[i] = (float)h[i];
-    CS_HIP(hipMalloc(&t->dev32, h32.size() * sizeof(float)));
-    CS_HIP(hipMemcpy(t->dev32, h32.data(), h32.size() * sizeof(float), hipMemcpyHostToDevice));
+    hipError_t e = hipMalloc(&t->dev, h.size() * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(t->dev, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&t->dev32, h32.size() * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(t->dev32, h32.data(), h32.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_line_op(*t);      // the caller keeps nothing of a failed table
+        return fail(CS_ERR_HIP, std::string("operator table upload failed: ") + hipGetErrorString(e));
+    }
+    t->count = h.size();
     return CS_OK;
 }
 

@@ -38,7 +38,10 @@ constexpr double ab01    = 2.0;
 constexpr double ab02    = 1.0 / 2.0;
 constexpr double ab10    = 3.0 / 4.0;
 constexpr int lmd  = 8;
-constexpr int mbci = 4;
+// gcbc update depth: the face row (D24 intended).  parameters.hpp:46 has
+// mbci = 4 with the point weight sbcc, which writes one face's correction onto
+// the boundary points of the adjacent faces along every edge line.
+constexpr int mbci = 1;
 constexpr int NV   = 5;
 constexpr int BC_NON_REFLECTIVE   = 10;
 constexpr int BC_WALL_INVISCID    = 20;
@@ -779,7 +782,8 @@ void orc_calc_fluxes(orc_ctx* c, int visc, const double* qa, const double* p, do
 // ---- gcbc_go (gcbc.hpp:86-186; kernels/gcbc.hpp:251-695) ---------------------
 // Intended semantics: D10 (fixed left eigenvectors), D11 (out index = base +
 // v*lmx, sbcc at the point), D12 (one face entry per thread), D13 (face
-// offset dim-1 for the far face).  cm[a] = [2 face][3][face_size].
+// offset dim-1 for the far face), D22 (the wall update adds R times the change
+// of the amplitudes), D24 (the update touches the face row only).  cm[a] = [2 face][3][face_size].
 // drvb[a] (same layout as drva) is only read for BC_INTER_CURV faces.
 // Returns -1 if an inter-curv face has no drvb.
 int orc_gcbc(orc_ctx* c, const double* cm0, const double* cm1, const double* cm2,
@@ -850,6 +854,10 @@ int orc_gcbc(orc_ctx* c, const double* cm0, const double* cm1, const double* cm2
                 } else {                                  // walls, kernels/gcbc.hpp:533-540
                     double accum = cmv_[0] * dudtmf[0] + cmv_[1] * dudtmf[1] + cmv_[2] * dudtmf[2];
                     cha[3 + ip] = cha[4 - ip] + 2 * ra0 * e.aoi * qa[idx] * (accum + dtwi * (e.vn + e.vs));
+                    // D22 intended: the change of the amplitudes, as kernels/gcbc.hpp:436
+                    // forms it for the non-reflective branch (the wall branch at
+                    // :541-546 omits it and adds J*de once more)
+                    for (int v = 0; v < 5; v++) cha[v] -= dv[v];
                 }
                 double R[5][5]; e.xtr2q(cmv_, R);
                 matvec5(R, cha, res);
@@ -865,7 +873,7 @@ int orc_gcbc(orc_ctx* c, const double* cm0, const double* cm1, const double* cm2
 }
 
 // wall_condition_update (gcbc.hpp:188-228; kernels/gcbc.hpp:160-249), D13
-// intended for the far face.
+// intended for the far face, D23 intended for the isothermal energy.
 void orc_wall_condition_update(orc_ctx* c, double* qa, const int* npex, const double* umf) {
     const Geo& g = c->g; long lmx = g.lmx;
     for (int nn = 0; nn < 3; nn++) {
@@ -877,15 +885,19 @@ void orc_wall_condition_update(orc_ctx* c, double* qa, const int* npex, const do
             for (long f = 0; f < fs; f++) {
                 long idx = g.base(nn, f) + off * st;
                 double q0 = qa[idx], q1 = qa[idx + lmx], q2 = qa[idx + 2 * lmx], q3 = qa[idx + 3 * lmx], q4 = qa[idx + 4 * lmx];
-                double v2 = q1 * q1 + q2 * q2 + q3 * q3;
                 double n = npex[idx];
                 if (np == BC_WALL_INVISCID) {
+                    double v2 = q1 * q1 + q2 * q2 + q3 * q3;
                     qa[idx + 4 * lmx] = n * q4 + (1.0 - n) * (hamhamm1 * std::pow(q0, gam) + 0.5 * v2 / q0);
                 } else {
+                    // D23 intended: kinetic energy of the updated momentum
+                    // (kernels/gcbc.hpp:242-248 uses the momentum before the update)
                     double fctr = (1 - n) * q0;
-                    qa[idx + lmx] = n * q1 - fctr * umf[0];
-                    qa[idx + 2 * lmx] = n * q2 - fctr * umf[1];
-                    qa[idx + 3 * lmx] = n * q3 - fctr * umf[2];
+                    double m1 = n * q1 - fctr * umf[0], m2 = n * q2 - fctr * umf[1], m3 = n * q3 - fctr * umf[2];
+                    double v2 = m1 * m1 + m2 * m2 + m3 * m3;
+                    qa[idx + lmx] = m1;
+                    qa[idx + 2 * lmx] = m2;
+                    qa[idx + 3 * lmx] = m3;
                     qa[idx + 4 * lmx] = n * q4 + (1 - n) * (hamhamm1 * q0 + 0.5 * v2 / q0);
                 }
             }

@@ -140,6 +140,9 @@ def test_batch_autotune_keeps_identity(tmp_path):
         b, f, d = _batch(dims, nbc, fields)
         case("batch autotune")
         b.autotune(reps=2)
+        torch.cuda.synchronize()
+        for s in range(len(f)):                       # tuning reads the bound fields only
+            assert torch.equal(f[s], fields[s].to(f[s].device)), s
         choices = {(op, n): bm.tune_get(op, n) for dm in dims for n in dm for op in (0, 1)}
         assert all(k in (8, 16, 32) for k in choices.values())
         for (op, n), k in choices.items():

@@ -505,6 +505,93 @@ def test_gcbc_nonreflective_zeroes_incoming(oracle):
                     assert np.max(np.abs(de[pt::lmx] - de0[pt::lmx])) == 0.0
 
 
+def test_gcbc_wall_characteristic_relation(oracle):
+    """Wall faces (kernels/gcbc.hpp:456-560, D22 intended): the setup takes the
+    amplitudes of J*de (flag 0), and after the update the characteristic
+    amplitudes of J*de at a face point are the old ones except the incoming
+    acoustic wave, which equals the outgoing one plus the wall term
+    2 ra0 (rho/c) (n.dudtmf + dtwi (vn + vs)).  Only the face row moves."""
+    from compact_schemes_amd import workloads as wl
+    dims = (16, 14, 12)
+    lmx = 16 * 14 * 12
+    dudtmf, dtwi = (0.01, -0.02, 0.03), 0.3
+    for code in (20, 25):
+        nbc = [[code] * 3, [code] * 3]
+        st = wl.flow_state(dims, seed=6, noise=1e-2)
+        umf = (0.3, -0.1, 0.05)
+        ctx = oracle.Context(dims, nbc)
+        de, p = np.zeros(5 * lmx), np.zeros(lmx)
+        oracle.lib().orc_primitives(lmx, oracle.ptr(st["qa"]), oracle.ptr(de), oracle.ptr(p))
+        ctx.calc_fluxes(0, st["qa"], p, de, st["xim"], st["etm"], st["zem"], umf)
+        de0 = de.copy()
+        sb = np.zeros(lmx)
+        sb.reshape(12, 14, 16)[:, :, 0] = 1.0
+        sb.reshape(12, 14, 16)[:, :, -1] = 1.0
+        ctx.gcbc(st["cm"], st["qa"], de, p, st["yaco"], sb, umf, dudtmf, dtwi)
+        Q = st["qa"].reshape(5, 12, 14, 16)
+        for face, ii, step in ((0, 0, 1), (1, 15, -1)):
+            for (kk, jj) in ((5, 4), (7, 9)):
+                pt = (kk * 14 + jj) * 16 + ii
+                rho, mom, E = Q[0, kk, jj, ii], Q[1:4, kk, jj, ii], Q[4, kk, jj, ii]
+                pr = 0.4 * (E - 0.5 * mom @ mom / rho)
+                R, L, info = oracle.gcbc_matrices([1, 0, 0], rho, mom, pr, umf, fixed=True)
+                ao, vn, vs = info
+                assert abs(ao - np.sqrt(1.4 * pr / rho)) < 1e-14
+                ch_before = L @ de0[pt::lmx]
+                ch_after = L @ de[pt::lmx]
+                ra0 = 1 - 2 * face
+                expect = ch_before.copy()
+                expect[3 + face] = ch_before[4 - face] + 2 * ra0 * rho / ao * (dudtmf[0] + dtwi * (vn + vs))
+                scale = max(1.0, np.max(np.abs(ch_before)))
+                assert np.max(np.abs(ch_after - expect)) < 1e-12 * scale, (code, face)
+                assert abs(expect[3 + face] - ch_before[3 + face]) > 1e-6      # the relation is not trivially met
+                for r in (1, 2, 3):                                            # sbcc = 0 inward
+                    q = pt + step * r
+                    assert np.array_equal(de[q::lmx], de0[q::lmx])
+
+
+def test_wall_condition_update_closed_forms(oracle):
+    """wall_condition_update (gcbc.hpp:188-228, kernels/gcbc.hpp:160-249): at
+    npex = 1 nothing changes; at npex = 0 an inviscid wall keeps rho and the
+    momentum and sets p = rho^gam / gam (isentropic), a viscous wall sets the
+    momentum to -rho umf (no slip in the frame) and p = rho / gam (isothermal,
+    c^2 = 1; D23 intended).  Points off the wall faces are untouched."""
+    from compact_schemes_amd import workloads as wl
+    dims = (12, 10, 8)
+    lxi, let, lze = dims
+    lmx = lxi * let * lze
+    g = 1.4
+    nbc = [[25, 20, 10], [25, 20, 10]]          # viscous x walls, inviscid y walls
+    st = wl.flow_state(dims, seed=7, noise=2e-2)
+    umf = np.array([0.5, -0.2, 0.1])
+    q0 = st["qa"].copy()
+    q = st["qa"].copy()
+    npex = (np.random.default_rng(8).uniform(size=lmx) < 0.5).astype(np.int32)
+    oracle.Context(dims, nbc).wall_condition_update(q, npex, umf)
+    x, y, z = (a.reshape(-1) for a in wl.grid_index(dims))
+    on_x = (x == 0) | (x == lxi - 1)
+    on_y = (y == 0) | (y == let - 1)
+    Q0, Q = q0.reshape(5, -1), q.reshape(5, -1)
+    off = ~(on_x | on_y)
+    assert np.array_equal(Q[:, off], Q0[:, off])
+    assert np.array_equal(Q[:, npex == 1], Q0[:, npex == 1])
+    pres = lambda A: (g - 1) * (A[4] - 0.5 * (A[1] ** 2 + A[2] ** 2 + A[3] ** 2) / A[0])
+    vis = on_x & ~on_y & (npex == 0)
+    inv = on_y & ~on_x & (npex == 0)
+    assert vis.sum() > 20 and inv.sum() > 20
+    assert np.array_equal(Q[0, vis], Q0[0, vis])
+    for d in range(3):
+        assert np.max(np.abs(Q[1 + d, vis] + Q0[0, vis] * umf[d])) < 1e-15
+    assert rel(pres(Q[:, vis]), Q0[0, vis] / g) < 1e-14
+    assert np.array_equal(Q[:4, inv], Q0[:4, inv])
+    assert rel(pres(Q[:, inv]), Q0[0, inv] ** g / g) < 1e-14
+    # edges: the y face is applied after the x face (axis order, gcbc.hpp:196-226)
+    edge = on_x & on_y & (npex == 0)
+    assert rel(pres(Q[:, edge]), Q0[0, edge] ** g / g) < 1e-14
+    for d in range(3):
+        assert np.max(np.abs(Q[1 + d, edge] + Q0[0, edge] * umf[d])) < 1e-15
+
+
 # ---- RK stage (build-defined) ---------------------------------------------------------
 
 def test_rk_stage_amplification(oracle):
