@@ -306,34 +306,6 @@ def plain_bench(args, n=64):
     print(json.dumps(line), flush=True)
 
 
-def batch_fields(count=96, seed0=3, lo=48, hi=320):
-    """Subdomain s: sizes drawn with seed seed0+s in [lo, hi]; field = 8 plane
-    waves with seeded wavenumbers 1..20 and phases + 1e-3 N(0,1) noise,
-    generated on the device."""
-    import numpy as np
-    import torch
-    dims, fields = [], []
-    for s in range(count):
-        rng = np.random.default_rng(seed0 + s)
-        d = tuple(int(v) for v in rng.integers(lo, hi + 1, size=3))
-        lxi, let, lze = d
-        z, y, x = torch.meshgrid(torch.arange(lze, device="cuda", dtype=torch.float64),
-                                 torch.arange(let, device="cuda", dtype=torch.float64),
-                                 torch.arange(lxi, device="cuda", dtype=torch.float64), indexing="ij")
-        f = torch.zeros_like(x)
-        for _ in range(8):
-            kx, ky, kz = (int(v) for v in rng.integers(1, 21, size=3))
-            phi = float(rng.uniform(0.0, 2.0 * np.pi))
-            f += torch.sin(2 * np.pi * (kx * x / lxi + ky * y / let + kz * z / lze) + phi)
-        g = torch.Generator(device="cuda")
-        g.manual_seed(seed0 + s)
-        f += 1e-3 * torch.randn(f.shape, generator=g, device="cuda", dtype=torch.float64)
-        dims.append(d)
-        fields.append(f.reshape(-1).contiguous())
-        del x, y, z
-    return dims, fields
-
-
 def batch_bench(args):
     """Config 3: the plain method on 96 uneven periodic subdomains: one step =
     derivative along x, y, z + filter along x, y, z, replayed from a hipGraph
@@ -346,6 +318,7 @@ def batch_bench(args):
     import compact_schemes_amd as cs
     from compact_schemes_amd import batch as bm
     per = [[45, 45, 45], [45, 45, 45]]
+    from compact_schemes_amd.workloads import batch_fields
     dims, f0 = batch_fields()
     npts = sum(d[0] * d[1] * d[2] for d in dims)
     f = [t.clone() for t in f0]
@@ -491,122 +464,50 @@ def adjoint_bench(args, n=128, nsteps=40, every=8, plane=96, ndir=32):
     (CFL 0.5); loss = acoustic energy sum(p'^2/(2 rho0 c0^2) + rho0 |u'|^2/2)
     on the plane z = 96.  The gradient with respect to rho' comes from the
     adjoint with checkpoints every 8 steps (each segment re-run recording
-    step tapes, then stepped backwards).  Central finite differences on 32
-    seeded smooth directions check it over the 40 steps, with these faces
-    and with periodic faces.  With non-reflective faces round-off grows about
-    1.5x per step (a boundary instability of the closure/characteristic
-    treatment that the CPU oracle shares); over 40 steps it stays below the
-    loss differences of the directions that reach the plane."""
+    step tapes, then stepped backwards; compact_schemes_amd.rollout).
+    Central finite differences on 32 random directions (N(0,1) white noise,
+    numpy seeds 500..531) check it direction by direction, with these faces
+    and with periodic faces."""
     import numpy as np
     import torch
     from compact_schemes_amd import workloads as wl
-    from compact_schemes_amd.fused import FusedSolver
-    dims = (n, n, n)
-    lmx = n ** 3
+    from compact_schemes_amd.rollout import AcousticRollout, checkpointed_vjp
     NBC5 = [[45, 10, 10], [45, 10, 10]]
-    st = wl.flow_state(dims, seed=5, amp=1e-4)
-    st0 = wl.flow_state(dims, seed=5, amp=0.0)
-    base = torch.from_numpy(st0["qa"]).cuda()
-    rho1 = torch.from_numpy(st["qa"][:lmx] - st0["qa"][:lmx]).cuda()       # rho' of the pulse
-    gam = wl.GAM
-    p0, rho0, c0 = 1.0 / gam, 1.0, 1.0
+    ro = AcousticRollout((n, n, n), NBC5, nsteps=nsteps, every=every, plane=plane)
+    sv = ro.solver
+    q0 = ro.initial(ro.rho1)
+    tapes = [sv.new_tape() for _ in range(every)]
 
-    def initial(r):
-        q = base.clone()
-        q[:lmx] += r
-        q[4 * lmx:] += (c0 * c0 / (gam - 1.0)) * r            # p' = c0^2 rho', u' = 0
-        return q
+    def forward(keep):
+        q = q0.clone()
+        ck = {}
+        for k in range(nsteps):
+            if keep and k % every == 0:
+                ck[k] = q.clone()
+            sv.rk4_step(q, dt=ro.dt)
+        return q, ck
 
-    def loss_and_grad(q):
-        qp = q.view(5, n, n, n)[:, plane].detach().clone().requires_grad_(True)
-        r, m, E = qp[0], qp[1:4], qp[4]
-        pr = (gam - 1.0) * (E - 0.5 * (m * m).sum(0) / r)
-        u2 = ((m / r) ** 2).sum(0)
-        L = ((pr - p0) ** 2 / (2 * rho0 * c0 * c0) + 0.5 * rho0 * u2).sum()
-        L.backward()
-        lam = torch.zeros(5 * lmx, dtype=torch.float64, device="cuda")
-        lam.view(5, n, n, n)[:, plane] = qp.grad.detach()
-        return float(L.detach()), lam
+    def adjoint(ck, lam):
+        for k0 in sorted(ck, reverse=True):
+            x = ck[k0].clone()
+            m = min(k0 + every, nsteps) - k0
+            for i in range(m):
+                sv.rk4_step_tape(x, tapes[i], dt=ro.dt)
+            for i in reversed(range(m)):
+                sv.rk4_vjp_tape(tapes[i], lam, ro.dt)
+        return lam
 
-    tapes = None
-
-    class Case:
-        """The workload with faces nbc over nk steps: forward with checkpoints
-        and the adjoint from step tapes."""
-
-        def __init__(self, nbc, nk):
-            nonlocal tapes
-            self.sv = FusedSolver(dims, nbc, st, viscous=False, cfl=0.5)
-            self.nk = nk
-            self.dt = self.sv.time_step(initial(rho1))
-            if tapes is None:
-                tapes = [self.sv.new_tape() for _ in range(every)]
-
-        def forward(self, r, keep=False):
-            q = initial(r)
-            ck = {}
-            for k in range(self.nk):
-                if keep and k % every == 0:
-                    ck[k] = q.clone()
-                self.sv.rk4_step(q, dt=self.dt)
-            return q, ck
-
-        def adjoint(self, ck, lam):
-            for k0 in sorted(ck, reverse=True):
-                x = ck[k0].clone()
-                m = min(k0 + every, self.nk) - k0
-                for i in range(m):
-                    self.sv.rk4_step_tape(x, tapes[i], dt=self.dt)
-                for i in reversed(range(m)):
-                    self.sv.rk4_vjp_tape(tapes[i], lam, self.dt)
-            return lam
-
-        def adjoint_recompute(self, ck, lam):
-            # the same with cs_solver_rk4_vjp, which recomputes the stages of each step
-            for k0 in sorted(ck, reverse=True):
-                states = [ck[k0]]
-                for k in range(k0, min(k0 + every, self.nk) - 1):
-                    x = states[-1].clone()
-                    self.sv.rk4_step(x, dt=self.dt)
-                    states.append(x)
-                for x in reversed(states):
-                    self.sv.rk4_vjp(x, lam, self.dt)
-            return lam
-
-        def gradient(self):
-            qT, ck = self.forward(rho1, keep=True)
-            L, lam = loss_and_grad(qT)
-            lam = self.adjoint(ck, lam)
-            return L, lam[:lmx] + (c0 * c0 / (gam - 1.0)) * lam[4 * lmx:]
-
-        def fd_check(self, eps=1e-7):
-            """Central differences on seeded Gaussian bumps (seed 500+d: centre
-            x, y in the middle half, z within 12 points of the loss plane,
-            half-width 4..10 points, random sign; a bump far from the plane
-            moves the plane's state by less than its round-off).  eps = 1e-7:
-            the cubic terms of the loss along a bump give central differences
-            an error ~ eps^2 that is still 1e-3 relative at eps = 1e-5.
-            Norm-wise error max_d |fd - ad| / max_d |ad| and the median of
-            the per-direction relative errors."""
-            _, grad = self.gradient()
-            ax = torch.arange(n, dtype=torch.float64, device="cuda")
-            fds, ads = [], []
-            for d in range(ndir):
-                rng = np.random.default_rng(500 + d)
-                c = rng.uniform(n / 4, 3 * n / 4, size=3)
-                c[2] = rng.uniform(plane - 12, plane + 12)
-                w = rng.uniform(4.0, 10.0)
-                sgn = 1.0 if rng.uniform() < 0.5 else -1.0
-                gz, gy, gx = [torch.exp(-np.log(2.0) * ((ax - c[i]) / w) ** 2) for i in (2, 1, 0)]
-                v = sgn * (gz[:, None, None] * gy[None, :, None] * gx[None, None, :]).reshape(-1)
-                Lp, _ = loss_and_grad(self.forward(rho1 + eps * v)[0])
-                Lm, _ = loss_and_grad(self.forward(rho1 - eps * v)[0])
-                fds.append((Lp - Lm) / (2 * eps))
-                ads.append(float(torch.dot(grad, v)))
-            fds, ads = np.array(fds), np.array(ads)
-            return {"steps": self.nk, "directions": ndir, "eps": eps,
-                    "normwise_rel_err": float(np.max(np.abs(fds - ads)) / np.max(np.abs(ads))),
-                    "median_rel_err": float(np.median(np.abs(fds - ads) / np.abs(ads)))}
+    def adjoint_recompute(ck, lam):
+        # the same with cs_solver_rk4_vjp, which recomputes the stages of each step
+        for k0 in sorted(ck, reverse=True):
+            states = [ck[k0]]
+            for k in range(k0, min(k0 + every, nsteps) - 1):
+                x = states[-1].clone()
+                sv.rk4_step(x, dt=ro.dt)
+                states.append(x)
+            for x in reversed(states):
+                sv.rk4_vjp(x, lam, ro.dt)
+        return lam
 
     def timed(fn, reps=3):
         best = None
@@ -619,15 +520,24 @@ def adjoint_bench(args, n=128, nsteps=40, every=8, plane=96, ndir=32):
             best = dt_ if best is None else min(best, dt_)
         return best, out
 
-    cs5 = Case(NBC5, nsteps)
-    cs5.forward(rho1)                                       # warm-up
-    t_fwd, (qT, ck) = timed(lambda: cs5.forward(rho1, keep=True))
-    L0, lam0 = loss_and_grad(qT)
-    t_adj, lam = timed(lambda: cs5.adjoint(ck, lam0.clone()))
-    t_rec, lam_rec = timed(lambda: cs5.adjoint_recompute(ck, lam0.clone()), reps=1)
+    forward(False)                                          # warm-up
+    t_fwd, (qT, ck) = timed(lambda: forward(True))
+    L0, lam0 = ro.loss_and_grad(qT)
+    t_adj, lam = timed(lambda: adjoint(ck, lam0.clone()))
+    t_rec, lam_rec = timed(lambda: adjoint_recompute(ck, lam0.clone()), reps=1)
     agree = float((lam - lam_rec).abs().max() / lam_rec.abs().max())
-    fd_full = Case(NBC5, nsteps).fd_check()
-    fd_per = Case(wl.NBC_PERIODIC, nsteps).fd_check()
+    L1, lam1 = checkpointed_vjp(sv, q0, nsteps, every, ro.dt, ro.loss_and_grad, tapes)
+    assert torch.equal(lam1, lam)
+
+    def fd_summary(rollout, eps):
+        _, grad = rollout.gradient()
+        fd, ad = rollout.fd_check([rollout.random_direction(500 + d) for d in range(ndir)], eps=eps, grad=grad)
+        rel = np.abs(fd - ad) / np.abs(ad)
+        return {"steps": nsteps, "directions": f"{ndir} x N(0,1) white noise, seeds 500..{500 + ndir - 1}",
+                "eps": eps, "max_rel_err": float(rel.max()), "median_rel_err": float(np.median(rel)),
+                "normwise_rel_err": float(np.max(np.abs(fd - ad)) / np.max(np.abs(ad)))}
+    fd_full = fd_summary(ro, 1e-6)
+    fd_per = fd_summary(AcousticRollout((n, n, n), wl.NBC_PERIODIC, nsteps=nsteps, every=every, plane=plane), 1e-6)
     line = {
         "metric": "adjoint/forward cost ratio, 40 Euler RK4 steps at 128^3 (checkpoints every 8)",
         "value": round(t_adj / t_fwd, 3), "unit": "ratio", "n_gpus": 1, "steps": nsteps, "warmup": 1,
@@ -636,7 +546,7 @@ def adjoint_bench(args, n=128, nsteps=40, every=8, plane=96, ndir=32):
         "data": "synthetic (uniform Mach 0.5 flow + seeded isentropic density pulse, amplitude 1e-4)",
         "config": {"workload": "Euler 128^3, periodic x, non-reflective y/z, GV10 filter, 40 RK4 steps, loss = "
                                "acoustic energy on the plane z=96, gradient w.r.t. the initial density perturbation",
-                   "loss": L0, "dt": cs5.dt, "forward_s": round(t_fwd, 4), "adjoint_s": round(t_adj, 4),
+                   "loss": L0, "dt": ro.dt, "forward_s": round(t_fwd, 4), "adjoint_s": round(t_adj, 4),
                    "adjoint": "per segment: the steps re-run with cs_solver_rk4_step_tape, then "
                               "cs_solver_rk4_vjp_tape backwards (fused transposed sweeps); best of 3",
                    "adjoint_s_stage_recompute_variant": round(t_rec, 4),
@@ -644,7 +554,7 @@ def adjoint_bench(args, n=128, nsteps=40, every=8, plane=96, ndir=32):
                    "tape_vs_recompute_max_rel": agree,
                    "fd_check_these_faces_40_steps": fd_full,
                    "fd_check_periodic_faces_40_steps": fd_per,
-                   "checkpoint_fields": 5 * len(ck), "tape_doubles_per_step": cs5.sv.tape_size()},
+                   "checkpoint_fields": 5 * len(ck), "tape_doubles_per_step": sv.tape_size()},
     }
     print(json.dumps(line), flush=True)
 

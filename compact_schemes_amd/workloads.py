@@ -109,3 +109,31 @@ def flow_state(dims, seed=2, mach=0.5, amp=1e-3, width=8.0, mu=1e-4, noise=0.0):
         "umf": (mach, 0.0, 0.0),
         "center": cen,
     }
+
+
+def batch_fields(count=96, seed0=3, lo=48, hi=320):
+    """Subdomain s: sizes drawn with seed seed0+s in [lo, hi]; field = 8 plane
+    waves with seeded wavenumbers 1..20 and phases + 1e-3 N(0,1) noise,
+    generated on the device."""
+    import numpy as np
+    import torch
+    dims, fields = [], []
+    for s in range(count):
+        rng = np.random.default_rng(seed0 + s)
+        d = tuple(int(v) for v in rng.integers(lo, hi + 1, size=3))
+        lxi, let, lze = d
+        z, y, x = torch.meshgrid(torch.arange(lze, device="cuda", dtype=torch.float64),
+                                 torch.arange(let, device="cuda", dtype=torch.float64),
+                                 torch.arange(lxi, device="cuda", dtype=torch.float64), indexing="ij")
+        f = torch.zeros_like(x)
+        for _ in range(8):
+            kx, ky, kz = (int(v) for v in rng.integers(1, 21, size=3))
+            phi = float(rng.uniform(0.0, 2.0 * np.pi))
+            f += torch.sin(2 * np.pi * (kx * x / lxi + ky * y / let + kz * z / lze) + phi)
+        g = torch.Generator(device="cuda")
+        g.manual_seed(seed0 + s)
+        f += 1e-3 * torch.randn(f.shape, generator=g, device="cuda", dtype=torch.float64)
+        dims.append(d)
+        fields.append(f.reshape(-1).contiguous())
+        del x, y, z
+    return dims, fields

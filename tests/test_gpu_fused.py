@@ -131,6 +131,32 @@ def test_fused_rhs_256_matches_oracle(oracle):
 
 
 @pytest.mark.gpu
+def test_fused_rk4_step_256_matches_oracle(oracle):
+    """The benchmark's timed unit: one RK4 step of the 256^3 viscous workload
+    (time step at CFL 0.95, four stages with in-sweep characteristic BCs and
+    stage updates, the filter along x, y, z) against the oracle's step."""
+    from compact_schemes_amd import workloads as wl
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (256, 256, 256)
+    lmx = 256 ** 3
+    st = wl.flow_state(dims, seed=2)
+    sv = FusedSolver(dims, CL, st)
+    q = dev(st["qa"])
+    case("fused rk4 step 256^3")
+    dt = sv.rk4_step(q, report_dt=True)
+    qh = host(q)
+    del q
+    ctx = oracle.Context(dims, CL)
+    qo = st["qa"].copy()
+    dto = ctx.rk4_step(1, qo, st["ssk"], st["xim"], st["etm"], st["zem"], st["yaco"], st["cm"], st["sbcc"],
+                       st["umf"])
+    assert abs(dt - dto) / dto < 1e-14
+    for v in range(5):
+        sl = slice(v * lmx, (v + 1) * lmx)
+        assert nrel(qh[sl] - st["qa"][sl], qo[sl] - st["qa"][sl]) < 1e-10, f"variable {v}"
+
+
+@pytest.mark.gpu
 def test_fused_errors():
     import compact_schemes_amd as cs
     from compact_schemes_amd.fused import FusedSolver
