@@ -194,6 +194,17 @@ def main():
         roofline["fp64"] = {"achieved": round(ach, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                             "frac": round(ach / FP64_PEAK_TFLOPS, 4), "flops_per_launch": fl}
 
+    if args.path == "fused":
+        # the whole path against the same roof: credited bytes of one RHS
+        # (736 B/pt) and of one RK4 step (4 stages of 776 B/pt, three filter
+        # sweeps of 80, the time step's 40) over their measured times
+        rhs_gbs = RHS_BYTES_PER_PT * lmx / (ms_rhs * 1e-3) / 1e9
+        step_gbs = STEP_BYTES_PER_PT * lmx / (ms_step * 1e-3) / 1e9
+        roofline["whole_rhs"] = {"bytes_per_pt": RHS_BYTES_PER_PT, "ms": round(ms_rhs, 4),
+                                 "achieved": round(rhs_gbs, 1), "frac": round(rhs_gbs / HBM_PEAK_GBS, 4)}
+        roofline["whole_rk4_step"] = {"bytes_per_pt": STEP_BYTES_PER_PT, "ms": round(ms_step, 4),
+                                      "achieved": round(step_gbs, 1), "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(st, dims, args.cpu_reps)
@@ -223,6 +234,8 @@ def main():
 
 
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_rk_stage_256.txt")
+RHS_BYTES_PER_PT = 736                     # six sweeps of one viscous RHS
+STEP_BYTES_PER_PT = 4 * 776 + 3 * 80 + 40  # RK4 step: four stages, filter x/y/z, time step
 PROFILE_FP64 = os.path.join(ROOT, "profiles", "pmc_fp64_rk_stage_256.txt")
 
 
@@ -531,13 +544,15 @@ def adjoint_bench(args, n=128, nsteps=40, every=8, plane=96, ndir=32):
 
     def fd_summary(rollout, eps):
         _, grad = rollout.gradient()
-        fd, ad = rollout.fd_check([rollout.random_direction(500 + d) for d in range(ndir)], eps=eps, grad=grad)
+        fd, ad = rollout.fd_check([rollout.random_direction(500 + d) for d in range(ndir)], eps=eps, grad=grad,
+                                  order=4)
         rel = np.abs(fd - ad) / np.abs(ad)
         return {"steps": nsteps, "directions": f"{ndir} x N(0,1) white noise, seeds 500..{500 + ndir - 1}",
-                "eps": eps, "max_rel_err": float(rel.max()), "median_rel_err": float(np.median(rel)),
+                "stencil": "fourth-order central differences", "eps": eps, "max_rel_err": float(rel.max()),
+                "median_rel_err": float(np.median(rel)),
                 "normwise_rel_err": float(np.max(np.abs(fd - ad)) / np.max(np.abs(ad)))}
-    fd_full = fd_summary(ro, 1e-6)
-    fd_per = fd_summary(AcousticRollout((n, n, n), wl.NBC_PERIODIC, nsteps=nsteps, every=every, plane=plane), 1e-6)
+    fd_full = fd_summary(ro, 1e-5)
+    fd_per = fd_summary(AcousticRollout((n, n, n), wl.NBC_PERIODIC, nsteps=nsteps, every=every, plane=plane), 1e-5)
     line = {
         "metric": "adjoint/forward cost ratio, 40 Euler RK4 steps at 128^3 (checkpoints every 8)",
         "value": round(t_adj / t_fwd, 3), "unit": "ratio", "n_gpus": 1, "steps": nsteps, "warmup": 1,

@@ -41,6 +41,7 @@ EXPORTS = [
     "cs_batch_capture", "cs_batch_replay", "cs_batch_get_persistent", "cs_batch_set_persistent", "cs_batch_info", "cs_batch_autotune", "cs_tune_set", "cs_tune_get",
     "cs_tune_clear", "cs_tune_save", "cs_tune_load",
     "cs_solver_time_step", "cs_solver_rk4_step",
+    "cs_init_pulse", "cs_boundary_weights", "cs_config_defaults", "cs_config_load", "cs_config_dims", "cs_config_grid", "cs_vtk_write",
 ]
 
 
@@ -62,6 +63,18 @@ class SolverConfig(ctypes.Structure):
     _fields_ = [("dims", Dims), ("nbc", ctypes.c_int * 6), ("viscous", ctypes.c_int), ("umf", Point),
                 ("mu", ctypes.c_double), ("metric", ctypes.c_double * 3), ("yaco", ctypes.c_double),
                 ("cfl", ctypes.c_double), ("alpha_f", ctypes.c_double), ("filter", ctypes.c_int)]
+
+
+class RunConfig(ctypes.Structure):
+    """cs_run_config: the run configuration (config.yaml schema + run section)."""
+    _fields_ = [("nblocks", ctypes.c_int), ("nbpc", ctypes.c_int * 3), ("lximb", ctypes.c_int),
+                ("letmb", ctypes.c_int), ("lzemb", ctypes.c_int), ("nbbc", ctypes.c_int * 6),
+                ("mbcd", ctypes.c_int * 6), ("doml0", ctypes.c_double), ("doml1", ctypes.c_double),
+                ("domh", ctypes.c_double), ("span", ctypes.c_double), ("amach", Point),
+                ("cfl", ctypes.c_double), ("tmax", ctypes.c_double), ("ndata", ctypes.c_int),
+                ("nsteps", ctypes.c_int), ("viscous", ctypes.c_int), ("re", ctypes.c_double),
+                ("alpha_f", ctypes.c_double), ("filter", ctypes.c_int), ("pulse_amp", ctypes.c_double),
+                ("pulse_width", ctypes.c_double)]
 
 
 _lib = None
@@ -151,6 +164,14 @@ def lib():
             "cs_tune_clear": (I, []),
             "cs_tune_save": (I, [ctypes.c_char_p]),
             "cs_tune_load": (I, [ctypes.c_char_p]),
+            "cs_init_pulse": (I, [ctypes.POINTER(Dims), Point, D, D, P, P]),
+            "cs_boundary_weights": (I, [ctypes.POINTER(Dims), P, P]),
+            "cs_config_defaults": (I, [ctypes.POINTER(RunConfig)]),
+            "cs_config_load": (I, [ctypes.c_char_p, ctypes.POINTER(RunConfig)]),
+            "cs_config_dims": (I, [ctypes.POINTER(RunConfig), ctypes.POINTER(Dims), ctypes.POINTER(ctypes.c_int)]),
+            "cs_config_grid": (I, [ctypes.POINTER(RunConfig), DP]),
+            "cs_vtk_write": (I, [ctypes.c_char_p, ctypes.POINTER(Dims), DP, I, ctypes.POINTER(ctypes.c_char_p),
+                                 ctypes.POINTER(DP), I]),
             "cs_solver_time_step": (I, [P, P, DP, P]),
             "cs_solver_rk4_step": (I, [P, P, P, D, DP, P]),
         }
@@ -353,6 +374,17 @@ class Physics:
         return dte.value, (m2[0], m2[1])
 
 
+def init_pulse(dims, center, amp, width, qa, stream=None):
+    """cs_init_pulse: uniform flow (stored velocity 0) with a Gaussian pressure pulse."""
+    check(lib().cs_init_pulse(ctypes.byref(Dims(*dims)), Point(*center), float(amp), float(width), _p(qa),
+                              _stream(stream)))
+
+
+def boundary_weights(dims, sbcc, stream=None):
+    """cs_boundary_weights: sbcc = 1 on boundary points, 0 inside."""
+    check(lib().cs_boundary_weights(ctypes.byref(Dims(*dims)), _p(sbcc), _stream(stream)))
+
+
 def primitives(dims, qa, de, p, stream=None):
     check(lib().cs_primitives(ctypes.byref(Dims(*dims)), _p(qa), _p(de), _p(p), _stream(stream)))
 
@@ -408,3 +440,47 @@ def copy_to(addr, t):
     rc = _hiprt().hipMemcpy(ctypes.c_void_p(addr), ctypes.c_void_p(t.data_ptr()), 8 * t.numel(), 3)
     if rc != 0:
         raise RuntimeError(f"hipMemcpy failed ({rc})")
+
+
+# ---- run configuration, grid, VTK output (host code) ------------------------
+def load_config(path):
+    """cs_config_load: the run configuration of a file in config.yaml's schema."""
+    cfg = RunConfig()
+    check(lib().cs_config_load(os.fsencode(path), ctypes.byref(cfg)))
+    return cfg
+
+
+def config_defaults():
+    cfg = RunConfig()
+    check(lib().cs_config_defaults(ctypes.byref(cfg)))
+    return cfg
+
+
+def config_dims(cfg):
+    """(dims, nbc) of the configuration's block: dims = (lxi, let, lze), nbc = [2][3]."""
+    d = Dims()
+    nbc = (ctypes.c_int * 6)()
+    check(lib().cs_config_dims(ctypes.byref(cfg), ctypes.byref(d), nbc))
+    return (d.lxi, d.let, d.lze), [list(nbc[0:3]), list(nbc[3:6])]
+
+
+def config_grid(cfg):
+    """cs_config_grid: host array [3][lmx] of the block's point coordinates."""
+    import numpy as np
+    dims, _ = config_dims(cfg)
+    xyz = np.empty(3 * dims[0] * dims[1] * dims[2], dtype=np.float64)
+    check(lib().cs_config_grid(ctypes.byref(cfg), xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return xyz
+
+
+def write_vtk(path, dims, xyz, fields, precision=32):
+    """cs_vtk_write: legacy VTK structured grid of host arrays; fields = {name: [lmx] array}."""
+    import numpy as np
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in fields.values()]
+    names = (ctypes.c_char_p * max(1, len(arrs)))(*[n.encode() for n in fields.keys()])
+    ptrs = (ctypes.POINTER(ctypes.c_double) * max(1, len(arrs)))(
+        *[a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in arrs])
+    check(lib().cs_vtk_write(os.fsencode(path), ctypes.byref(Dims(*dims)),
+                             xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(arrs), names, ptrs,
+                             int(precision)))

@@ -497,8 +497,8 @@ int cs_gcbc_go(cs_numerics* num, const double* const* cm, const double* const* d
         for (int ip = 0; ip < 2; ip++) {
             int np = num->nbc[ip][nn];
             if (!is_char_bc(np)) continue;
-            int rc = launch_gcbc_update(num, nn, ip, np, cm[nn], drvb ? drvb[nn] : nullptr, qa, de, p, sbcc, umf,
-                                        dudtmf, dtwi, s);
+            int rc = launch_gcbc_update(num, nn, ip, np, cm[nn], drvb ? drvb[nn] : nullptr, qa, de, p, sbcc, yaco,
+                                        umf, dudtmf, dtwi, s);
             if (rc) return rc;
         }
     return CS_OK;

@@ -178,7 +178,9 @@ struct S2 {
         }
         double qq[5];
         Prim p = prim_at(q, bo, qq);
-        const double cm[3] = {AX == 0 ? 1.0 : 0.0, AX == 1 ? 1.0 : 0.0, AX == 2 ? 1.0 : 0.0};
+        // unit normal of the face: grad(xi_AX) / |grad(xi_AX)|, grad(xi_AX) = J * metric * e_AX
+        const double sg = ph.m[AX] * ph.J < 0.0 ? -1.0 : 1.0;
+        const double cm[3] = {AX == 0 ? sg : 0.0, AX == 1 ? sg : 0.0, AX == 2 ? sg : 0.0};
         Eig e;
         e.eleme(cm, qq[0], qq + 1, p.p, ph.umf);
         double cha[5], ch[5];
@@ -190,6 +192,9 @@ struct S2 {
         if (ra0 * (e.vn + e.vs + e.ao) > 0) dl[3] = -ch[3];
         if (ra0 * (e.vn + e.vs - e.ao) > 0) dl[4] = -ch[4];
         e.r_apply(cm, dl, corr);
+        // D25: a change of J*de (the amplitudes were formed from J*drv), so de changes by corr / J
+#pragma unroll
+        for (int v = 0; v < 5; v++) corr[v] /= ph.J;
     }
 };
 

@@ -45,8 +45,8 @@ __global__ void __launch_bounds__(kGBlock) gcbc_setup_kernel(FaceGeo fg, int fla
 __global__ void __launch_bounds__(kGBlock) gcbc_update_kernel(FaceGeo fg, int code, const double* cm,
                                                               const double* drva, const double* drvb,
                                                               const double* qa, double* de, const double* pr,
-                                                              const double* sbcc, cs_point umf, cs_point dudtmf,
-                                                              double dtwi) {
+                                                              const double* sbcc, const double* yaco, cs_point umf,
+                                                              cs_point dudtmf, double dtwi) {
     long f = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= fg.fs) return;
     const long idx = fg.point(f);
@@ -77,6 +77,9 @@ __global__ void __launch_bounds__(kGBlock) gcbc_update_kernel(FaceGeo fg, int co
         for (int v = 0; v < 5; v++) cha[v] -= dv[v];
     }
     e.r_apply(cmv, cha, res);
+    // D25: R*cha is a change of yaco*de (the setup multiplied by yaco), so de
+    // changes by R*cha / yaco (the reference adds R*cha to de; the same when yaco = 1)
+    for (int v = 0; v < 5; v++) res[v] /= yaco[idx];
     const int iq = 1 - 2 * fg.face;
     for (int ii = 0; ii < kMbci; ii++) {
         const long o = idx + (long)ii * iq * fg.st;
@@ -134,12 +137,12 @@ int launch_gcbc_setup(cs_numerics* num, int axis, int face, int flag, const doub
 }
 
 int launch_gcbc_update(cs_numerics* num, int axis, int face, int code, const double* cm, const double* drvb,
-                       const double* qa, double* de, const double* p, const double* sbcc, cs_point umf,
-                       cs_point dudtmf, double dtwi, hipStream_t s) {
+                       const double* qa, double* de, const double* p, const double* sbcc, const double* yaco,
+                       cs_point umf, cs_point dudtmf, double dtwi, hipStream_t s) {
     FaceGeo fg = face_geo(num, axis, face);
     dim3 grid((unsigned)((fg.fs + kGBlock - 1) / kGBlock));
     hipLaunchKernelGGL(gcbc_update_kernel, grid, dim3(kGBlock), 0, s, fg, code, cm, num->drva[axis], drvb, qa, de, p,
-                       sbcc, umf, dudtmf, dtwi);
+                       sbcc, yaco, umf, dudtmf, dtwi);
     CS_HIP(hipGetLastError());
     return CS_OK;
 }

@@ -6,6 +6,8 @@
 // All are one-point-per-thread streaming kernels: HBM-bound, 256 threads.
 #include "physics_kernels.hpp"
 
+#include <cmath>
+
 namespace cs {
 
 constexpr int kBlock = 256;
@@ -170,6 +172,33 @@ __global__ void __launch_bounds__(kBlock) rk_stage_kernel(double* q, const doubl
     for (int v = 0; v < 5; v++) q[t + v * lmx] = q0[t + v * lmx] - f * de[t + v * lmx];
 }
 
+// initial state: uniform flow at rest in the frame (stored velocity 0),
+// rho0 = 1, p0 = 1/gam, and a Gaussian pressure pulse with rho' = p' (c0 = 1)
+__global__ void __launch_bounds__(kBlock) init_pulse_kernel(double* qa, int lxi, int let, long lmx, cs_point cen,
+                                                            double amp, double width) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= lmx) return;
+    const long plane = (long)lxi * let;
+    const double z = (double)(t / plane), y = (double)((t % plane) / lxi), x = (double)(t % lxi);
+    const double r2 = (x - cen.x) * (x - cen.x) + (y - cen.y) * (y - cen.y) + (z - cen.z) * (z - cen.z);
+    const double pp = amp * exp(-M_LN2 * r2 / (width * width));
+    qa[t] = 1.0 + pp;
+    qa[t + lmx] = 0.0;
+    qa[t + 2 * lmx] = 0.0;
+    qa[t + 3 * lmx] = 0.0;
+    qa[t + 4 * lmx] = (kHam + pp) / kGamm1;
+}
+
+// sbcc: 1 on the boundary points of the subdomain, 0 inside
+__global__ void __launch_bounds__(kBlock) boundary_weights_kernel(double* sbcc, int lxi, int let, int lze, long lmx) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= lmx) return;
+    const long plane = (long)lxi * let;
+    const long z = t / plane, y = (t % plane) / lxi, x = t % lxi;
+    const bool on = x == 0 || x == lxi - 1 || y == 0 || y == let - 1 || z == 0 || z == lze - 1;
+    sbcc[t] = on ? 1.0 : 0.0;
+}
+
 // fill_gpu_buffer_kernel<Axis>, src/cuda/kernels/numerics_pc.hpp:42-122: for
 // every line of the face, the two pbco-weighted 8-point sums from the face
 // inward (istart/increment of numerics_pc.hpp:275-280).
@@ -266,4 +295,34 @@ int launch_fill(const double* in, double* send, const double* pbco, int axis, in
     return CS_OK;
 }
 
+int launch_init_pulse(const cs_dims* d, cs_point cen, double amp, double width, double* qa, hipStream_t s) {
+    const long lmx = (long)d->lxi * d->let * d->lze;
+    hipLaunchKernelGGL(init_pulse_kernel, grid_for(lmx), dim3(kBlock), 0, s, qa, d->lxi, d->let, lmx, cen, amp, width);
+    CS_HIP(hipGetLastError());
+    return CS_OK;
+}
+
+int launch_boundary_weights(const cs_dims* d, double* sbcc, hipStream_t s) {
+    const long lmx = (long)d->lxi * d->let * d->lze;
+    hipLaunchKernelGGL(boundary_weights_kernel, grid_for(lmx), dim3(kBlock), 0, s, sbcc, d->lxi, d->let, d->lze, lmx);
+    CS_HIP(hipGetLastError());
+    return CS_OK;
+}
+
 }  // namespace cs
+
+using namespace cs;
+
+extern "C" int cs_init_pulse(const cs_dims* dims, cs_point center, double amp, double width, double* qa,
+                             cs_stream stream) {
+    if (!dims || !qa) return fail(CS_ERR_ARG, "null argument");
+    if (dims->lxi < 1 || dims->let < 1 || dims->lze < 1) return fail(CS_ERR_SHAPE, "grid dimensions must be positive");
+    if (!(width > 0.0) || !std::isfinite(amp)) return fail(CS_ERR_ARG, "pulse width must be > 0 and amp finite");
+    return launch_init_pulse(dims, center, amp, width, qa, (hipStream_t)stream);
+}
+
+extern "C" int cs_boundary_weights(const cs_dims* dims, double* sbcc, cs_stream stream) {
+    if (!dims || !sbcc) return fail(CS_ERR_ARG, "null argument");
+    if (dims->lxi < 1 || dims->let < 1 || dims->lze < 1) return fail(CS_ERR_SHAPE, "grid dimensions must be positive");
+    return launch_boundary_weights(dims, sbcc, (hipStream_t)stream);
+}

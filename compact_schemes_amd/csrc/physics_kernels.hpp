@@ -161,8 +161,8 @@ int launch_fill(const double* in, double* send, const double* pbco, int axis, in
 int launch_gcbc_setup(cs_numerics* num, int axis, int face, int flag, const double* cm, const double* qa,
                       const double* de, const double* p, const double* yaco, cs_point umf, hipStream_t s);
 int launch_gcbc_update(cs_numerics* num, int axis, int face, int code, const double* cm, const double* drvb,
-                       const double* qa, double* de, const double* p, const double* sbcc, cs_point umf,
-                       cs_point dudtmf, double dtwi, hipStream_t s);
+                       const double* qa, double* de, const double* p, const double* sbcc, const double* yaco,
+                       cs_point umf, cs_point dudtmf, double dtwi, hipStream_t s);
 int launch_wall(cs_numerics* num, int axis, int face, int code, double* qa, const int* npex, cs_point umf,
                 hipStream_t s);
 

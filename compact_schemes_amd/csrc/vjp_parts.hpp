@@ -34,7 +34,8 @@ __device__ __forceinline__ void flux_row(int ax, const T* q, double m, cs_point 
 template <class T, class X>
 __device__ __forceinline__ void face_corr(int ax, int face, const T* q, const X* drv, const EulerPhys& ph, X* corr) {
     PrimT<T> p = primitive(q);
-    const double cm[3] = {ax == 0 ? 1.0 : 0.0, ax == 1 ? 1.0 : 0.0, ax == 2 ? 1.0 : 0.0};
+    const double sg = ph.m[ax] * ph.J < 0.0 ? -1.0 : 1.0;   // unit normal of grad(xi_ax) = J * metric * e_ax
+    const double cm[3] = {ax == 0 ? sg : 0.0, ax == 1 ? sg : 0.0, ax == 2 ? sg : 0.0};
     EigT<T> e;
     e.eleme(cm, q[0], q + 1, p.p, ph.umf);
     X cha[5], ch[5];
@@ -46,6 +47,8 @@ __device__ __forceinline__ void face_corr(int ax, int face, const T* q, const X*
     if (ra0 * value_of(e.vn + e.vs + e.ao) > 0) dl[3] = -ch[3];
     if (ra0 * value_of(e.vn + e.vs - e.ao) > 0) dl[4] = -ch[4];
     e.r_apply(cm, dl, corr);
+#pragma unroll
+    for (int v = 0; v < 5; v++) corr[v] = corr[v] / ph.J;   // D25: de changes by R*cha / J
 }
 
 }  // namespace fused

@@ -103,16 +103,25 @@ class AcousticRollout:
         import torch
         return torch.from_numpy(np.random.default_rng(seed).standard_normal(self.lmx)).cuda()
 
-    def fd_check(self, directions, eps=1e-7, grad=None):
-        """Central differences (L(r + eps v) - L(r - eps v)) / (2 eps) against
-        <grad, v> for each direction v; returns the per-direction arrays."""
+    def fd_check(self, directions, eps=1e-7, grad=None, order=2):
+        """Central differences against <grad, v> for each direction v; returns
+        the per-direction arrays.  order 2: (L(r + e v) - L(r - e v)) / (2 e);
+        order 4: (-L(r + 2e v) + 8 L(r + e v) - 8 L(r - e v) + L(r - 2e v)) / (12 e),
+        whose truncation error is O(e^4) instead of O(e^2)."""
         import torch
         if grad is None:
             _, grad = self.gradient()
+        if order not in (2, 4):
+            raise ValueError("order must be 2 or 4")
         fds, ads = [], []
         for v in directions:
             lp = self.loss(self.rho1 + eps * v)
             lm = self.loss(self.rho1 - eps * v)
-            fds.append((lp - lm) / (2 * eps))
+            if order == 2:
+                fds.append((lp - lm) / (2 * eps))
+            else:
+                lpp = self.loss(self.rho1 + 2 * eps * v)
+                lmm = self.loss(self.rho1 - 2 * eps * v)
+                fds.append((-lpp + 8 * lp - 8 * lm + lmm) / (12 * eps))
             ads.append(float(torch.dot(grad, v)))
         return np.array(fds), np.array(ads)

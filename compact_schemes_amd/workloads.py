@@ -137,3 +137,24 @@ def batch_fields(count=96, seed0=3, lo=48, hi=320):
         fields.append(f.reshape(-1).contiguous())
         del x, y, z
     return dims, fields
+
+
+def with_spacing(st, dims, spacing):
+    """The state `st` on a Cartesian grid with point spacings (dx, dy, dz)
+    (negative: the coordinate decreases with the index): metrics are the
+    cofactors xim = (dy dz, 0, 0), etm = (0, dz dx, 0), zem = (0, 0, dx dy),
+    yaco = 1 / (dx dy dz), and the face normals cm[a] are the unit vectors of
+    grad xi_a = yaco * metric_a * e_a.  Returns a new dict."""
+    dx, dy, dz = spacing
+    lmx = dims[0] * dims[1] * dims[2]
+    out = dict(st)
+    m = (dy * dz, dz * dx, dx * dy)
+    J = 1.0 / (dx * dy * dz)
+    xim, etm, zem = np.zeros(3 * lmx), np.zeros(3 * lmx), np.zeros(3 * lmx)
+    xim[0:lmx] = m[0]
+    etm[lmx:2 * lmx] = m[1]
+    zem[2 * lmx:3 * lmx] = m[2]
+    out["xim"], out["etm"], out["zem"] = xim, etm, zem
+    out["yaco"] = np.full(lmx, J)
+    out["cm"] = [c * np.sign(J * m[a]) for a, c in enumerate(face_normals(dims))]
+    return out

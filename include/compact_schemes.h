@@ -176,6 +176,15 @@ int cs_gcbc_go(cs_numerics* num, const double* const* cm, const double* const* d
 /* wall_condition_update, src/gcbc.hpp:188-228. */
 int cs_wall_condition_update(cs_numerics* num, double* qa, const int* npex, cs_point umf,
                              cs_stream stream);
+/* initial state (no reference counterpart: the driver's physics init,
+ * driver/canard.cu:129, is not defined upstream): uniform flow carried by the
+ * frame velocity umf (stored velocity 0), rho0 = 1, p0 = 1/gam (c0 = 1), plus
+ * a Gaussian pressure pulse p' = amp exp(-ln2 r^2 / width^2) around `center`
+ * (index units) with rho' = p' / c0^2: qa = (rho, 0, 0, 0, p / (gam - 1)). */
+int cs_init_pulse(const cs_dims* dims, cs_point center, double amp, double width, double* qa, cs_stream stream);
+/* characteristic-update weights sbcc (allocated and never set upstream,
+ * src/gcbc.hpp:51-67): 1 on every boundary point of the subdomain, 0 inside. */
+int cs_boundary_weights(const cs_dims* dims, double* sbcc, cs_stream stream);
 /* RK stage update (absent upstream; canard.cu:196-200 names the weights):
  * q = q0 - dtk * yaco * de for the 5 conserved variables. */
 int cs_rk_stage(const cs_dims* dims, double* q, const double* q0, const double* de,
@@ -188,7 +197,10 @@ int cs_rk_stage(const cs_dims* dims, double* q, const double* q0, const double* 
  * cs_solver computes the same quantities with six fused line sweeps per RHS
  * on a uniform Cartesian grid: the metrics are the constant diagonals
  * xim = (metric[0],0,0), etm = (0,metric[1],0), zem = (0,0,metric[2]), yaco
- * and ssk (= mu) are constants, cm = +e_axis.  Per axis both faces must be
+ * and ssk (= mu) are constants, cm = sign(yaco * metric[a]) e_a (the unit
+ * normal of grad xi_a = yaco * metric[a] e_a; it is -e_a on an axis whose
+ * coordinate decreases with the index, as z of src/grid.hpp:103).  Per axis
+ * both faces must be
  * non-reflective (10) or both seam codes (35/40/45); walls and curved
  * interfaces use the per-operator API above. */
 typedef struct {
@@ -332,6 +344,52 @@ int cs_tune_clear(void);
 /* persisted as text, one "op length K" line per choice */
 int cs_tune_save(const char* path);
 int cs_tune_load(const char* path);
+
+/* ---- run configuration, block grid and output (host code, no device) ---
+ * The reference's driver reads config.yaml with yaml-cpp
+ * (driver/canard.cu:37-98), generates the block's coordinates
+ * (src/grid.hpp:89-110) and writes legacy VTK (src/IO.hpp:108-159).  The
+ * sections driver, domdcomp, grid and physics follow config.yaml:1-18 (values
+ * of block 0; one block runs as one subdomain on one GPU, whatever its nbpc);
+ * the optional section run holds what driver/canard.cu:117-127 fixes in code
+ * plus this build's solver settings and initial pulse. */
+typedef struct {
+    int      nblocks;               /* driver.nblocks                          */
+    int      nbpc[3];               /* domdcomp.nbpc of block 0                */
+    int      lximb, letmb, lzemb;   /* last point index per axis (points - 1)  */
+    int      nbbc[6];               /* face BC codes, [2 faces][3 axes]        */
+    int      mbcd[6];               /* neighbour blocks, [2 faces][3 axes]     */
+    double   doml0, doml1, domh, span;   /* grid extents (grid.hpp:74-87)     */
+    cs_point amach;                 /* amach1..3 -> umf (physics_base.hpp:135) */
+    double   cfl, tmax;             /* run.cfl (0.95), run.tmax (1.0)          */
+    int      ndata;                 /* run.ndata: outputs over tmax (2)        */
+    int      nsteps;                /* run.nsteps: step cap, 0 = until tmax    */
+    int      viscous;               /* run.viscous (1)                         */
+    double   re, alpha_f;           /* run.re (1e4), run.alpha_f (0.49)        */
+    int      filter;                /* run.filter (1)                          */
+    double   pulse_amp, pulse_width;   /* initial acoustic pulse (1e-3, 8 cells) */
+} cs_run_config;
+
+/* the defaults of every field (the run section's values above, one block of
+ * non-reflective faces) */
+int cs_config_defaults(cs_run_config* cfg);
+/* domdcomp::read_config + grid::read_config + physics_base::read_config
+ * (src/domdcomp.hpp:67-107, src/grid.hpp:74-87, src/physics_base.hpp:135-145)
+ * of a file in the schema of config.yaml:1-18; errors name the file and line. */
+int cs_config_load(const char* path, cs_run_config* cfg);
+/* t_dcomp of block 0 (points = last index + 1) and its face BC codes nbc6
+ * ([2][3], may be NULL) */
+int cs_config_dims(const cs_run_config* cfg, cs_dims* dims, int* nbc6);
+/* grid::generate, src/grid.hpp:89-110: host array xyz = [3][lmx] of point
+ * coordinates x = -doml0 + i (doml0+doml1)/lximb, y = -domh + j 2 domh/letmb,
+ * z = span ((lzemb - k)/lzemb - 1/2). */
+int cs_config_grid(const cs_run_config* cfg, double* xyz);
+/* IOwriter::go_vtk, src/IO.hpp:108-159: legacy VTK structured grid, BINARY
+ * (big-endian), points then one SCALARS block per variable.  Host arrays:
+ * xyz = [3][lmx], vars[v] = [lmx]; precision 32 writes float (as the
+ * reference), 64 double. */
+int cs_vtk_write(const char* path, const cs_dims* dims, const double* xyz, int nvars, const char* const* names,
+                 const double* const* vars, int precision);
 
 #ifdef __cplusplus
 }

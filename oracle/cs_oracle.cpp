@@ -783,7 +783,8 @@ void orc_calc_fluxes(orc_ctx* c, int visc, const double* qa, const double* p, do
 // Intended semantics: D10 (fixed left eigenvectors), D11 (out index = base +
 // v*lmx, sbcc at the point), D12 (one face entry per thread), D13 (face
 // offset dim-1 for the far face), D22 (the wall update adds R times the change
-// of the amplitudes), D24 (the update touches the face row only).  cm[a] = [2 face][3][face_size].
+// of the amplitudes), D24 (the update touches the face row only), D25 (the
+// update divides R*cha by yaco).  cm[a] = [2 face][3][face_size].
 // drvb[a] (same layout as drva) is only read for BC_INTER_CURV faces.
 // Returns -1 if an inter-curv face has no drvb.
 int orc_gcbc(orc_ctx* c, const double* cm0, const double* cm1, const double* cm2,
@@ -861,6 +862,9 @@ int orc_gcbc(orc_ctx* c, const double* cm0, const double* cm1, const double* cm2
                 }
                 double R[5][5]; e.xtr2q(cmv_, R);
                 matvec5(R, cha, res);
+                // D25 intended: R*cha is a change of J*de (the setup multiplied
+                // by yaco); de itself changes by R*cha / yaco
+                for (int v = 0; v < 5; v++) res[v] /= yaco[idx];
                 int iq = 1 - 2 * ip;
                 for (int ii = 0; ii < mbci; ii++) {
                     long o = idx + (long)ii * iq * st;

@@ -131,29 +131,44 @@ def test_fused_rhs_256_matches_oracle(oracle):
 
 
 @pytest.mark.gpu
-def test_fused_rk4_step_256_matches_oracle(oracle):
+@pytest.mark.parametrize("filt", [False, True], ids=["stages", "stages+filter"])
+def test_fused_rk4_step_256_matches_oracle(oracle, filt):
     """The benchmark's timed unit: one RK4 step of the 256^3 viscous workload
     (time step at CFL 0.95, four stages with in-sweep characteristic BCs and
-    stage updates, the filter along x, y, z) against the oracle's step."""
+    stage updates, the filter along x, y, z) against the oracle's step.
+
+    The state after the step agrees to 1e-13 of its size.  The change of the
+    state over the step (3e-5 for rho on a unit state) agrees to 1e-10 without
+    the filter.  The filter's matrix (alpha_f = 0.49) has a condition number
+    of about 99, so two correct solves of its rows -- the chunked recurrences
+    here, Thomas in the oracle -- differ by up to cond * eps * |q|, about 2e-14
+    of the unit state, i.e. about 1e-9 of rho's change: with the filter the
+    change is held to 3e-9."""
     from compact_schemes_amd import workloads as wl
     from compact_schemes_amd.fused import FusedSolver
     dims = (256, 256, 256)
     lmx = 256 ** 3
     st = wl.flow_state(dims, seed=2)
-    sv = FusedSolver(dims, CL, st)
+    sv = FusedSolver(dims, CL, st, filt=filt)
     q = dev(st["qa"])
-    case("fused rk4 step 256^3")
+    case(f"fused rk4 step 256^3, filter {filt}")
     dt = sv.rk4_step(q, report_dt=True)
     qh = host(q)
     del q
+    sv.close()
     ctx = oracle.Context(dims, CL)
     qo = st["qa"].copy()
     dto = ctx.rk4_step(1, qo, st["ssk"], st["xim"], st["etm"], st["zem"], st["yaco"], st["cm"], st["sbcc"],
-                       st["umf"])
+                       st["umf"], filt=filt)
     assert abs(dt - dto) / dto < 1e-14
+    bar = 3e-9 if filt else 1e-10
     for v in range(5):
         sl = slice(v * lmx, (v + 1) * lmx)
-        assert nrel(qh[sl] - st["qa"][sl], qo[sl] - st["qa"][sl]) < 1e-10, f"variable {v}"
+        e_state = nrel(qh[sl], qo[sl])
+        e_change = nrel(qh[sl] - st["qa"][sl], qo[sl] - st["qa"][sl])
+        print(f"variable {v}: state {e_state:.2e}, change {e_change:.2e}", flush=True)
+        assert e_state < 1e-13, f"variable {v}"
+        assert e_change < bar, f"variable {v}"
 
 
 @pytest.mark.gpu

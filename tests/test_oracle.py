@@ -620,3 +620,59 @@ def test_interface_table_fixture_reproducible():
     old = json.load(open(os.path.join(GOLD, "interface_table.json")))
     for k in ("pbco_shipped", "pbci_shipped", "pbci_fixed_5", "pbci_fixed_7"):
         assert np.max(np.abs(np.array(old[k]) - np.array(new[k]))) < 1e-15, k
+
+
+def _rk4_steps(oracle, dims, nbc, st, visc, steps, dt=0.0):
+    ctx = oracle.Context(dims, nbc)
+    q = st["qa"].copy()
+    dts = []
+    for _ in range(steps):
+        dts.append(ctx.rk4_step(int(visc), q, st["ssk"], st["xim"], st["etm"], st["zem"], st["yaco"], st["cm"],
+                                st["sbcc"], st["umf"], dt=dt))
+    return q, dts
+
+
+@pytest.mark.parametrize("visc", [True, False], ids=["viscous", "euler"])
+def test_rk4_steps_are_similar_under_grid_scaling(oracle, visc):
+    """Scaling x and t by h (and mu by h) leaves the Navier-Stokes solution
+    unchanged: the steps on a grid of spacing h = 1/4 (metrics h^2, yaco
+    h^-3) with mu*h take dt*h and give the unit grid's states.  Powers of two
+    scale exactly, so this fixes how every metric and yaco factor enters the
+    fluxes, the stress, the characteristic update (D25: a yaco too many
+    there made the scaled run blow up in one step) and the time step."""
+    from compact_schemes_amd import workloads as wl
+    dims = (24, 20, 16)
+    nbc = [[45, 10, 10], [45, 10, 10]]
+    h = 0.25
+    base = wl.flow_state(dims, seed=31, mu=1e-3, amp=1e-2, width=3.0, noise=1e-3)
+    base["umf"] = (0.5, 0.0, 0.1)
+    qa, dta = _rk4_steps(oracle, dims, nbc, base, visc, 3)
+    sc = wl.with_spacing(base, dims, (h, h, h))
+    sc["ssk"] = base["ssk"] * h
+    qb, dtb = _rk4_steps(oracle, dims, nbc, sc, visc, 3)
+    assert all(b == a * h for a, b in zip(dta, dtb))
+    assert np.all(np.isfinite(qb))
+    assert np.max(np.abs(qb - qa)) <= 1e-13 * np.max(np.abs(qa))
+
+
+def test_rk4_steps_mirror_on_a_reversed_axis(oracle):
+    """A grid whose z coordinate decreases with k (src/grid.hpp:103; metrics
+    xim = etm = -1, zem = 1, yaco = -1, face normals -e_z on the z faces)
+    carries the mirror image of the flow on the increasing grid: with w and
+    the frame velocity's z component negated, every step gives the same
+    state with the z momentum negated."""
+    from compact_schemes_amd import workloads as wl
+    dims = (24, 20, 16)
+    lmx = 24 * 20 * 16
+    nbc = [[45, 10, 10], [45, 10, 10]]
+    base = wl.flow_state(dims, seed=32, mu=1e-3, amp=1e-2, width=3.0, noise=1e-3)
+    base["umf"] = (0.5, 0.0, 0.1)
+    qa, dta = _rk4_steps(oracle, dims, nbc, base, True, 3)
+    mir = wl.with_spacing(base, dims, (1.0, 1.0, -1.0))
+    mir["umf"] = (0.5, 0.0, -0.1)
+    mir["qa"] = base["qa"].copy()
+    mir["qa"][3 * lmx:4 * lmx] *= -1.0
+    qb, dtb = _rk4_steps(oracle, dims, nbc, mir, True, 3)
+    qb[3 * lmx:4 * lmx] *= -1.0
+    assert np.allclose(dtb, dta, rtol=1e-15, atol=0)
+    assert np.max(np.abs(qb - qa)) <= 1e-13 * np.max(np.abs(qa))
