@@ -45,7 +45,10 @@ template <int AX, bool CLOSE>
 struct S1 {
     static constexpr int NF = 4;
     static constexpr bool kGcbc = false;   // gradients only; BC updates act on the flux sweeps
-    static constexpr bool kPv = false;     // the closure re-derives u, v, w, c^2 from q at its rows
+    static constexpr bool kPv = false;     // kernels without an LDS stash re-derive u, v, w from q at the store
+    // the transposing x kernel keeps u, v, w of its points in LDS for the
+    // closure (store_pv) instead of reading q again
+    static constexpr int kStash = CLOSE ? 3 : 0;
     static constexpr int kH = 2;           // derivative stencil half-width
     static constexpr int kWpe = 3;         // waves per SIMD of the y/z kernels (168 VGPRs)
     static constexpr bool kFilter = false;
@@ -70,28 +73,37 @@ struct S1 {
 #pragma unroll
             for (int f = 0; f < 4; f++) stg(gout[f], bo, d[f]);
         } else {
-            double gr[3][4];
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int f = 0; f < 4; f++) gr[a][f] = (a == AX) ? d[f] : ldg(gin[a][f], bo);
-            // chain rule of calc_viscous_shear_stress_post_compute_kernel with
-            // diagonal metrics (src/cuda/kernels/physics.hpp:267-281)
-            double g[9];
-#pragma unroll
-            for (int f = 0; f < 3; f++)
-#pragma unroll
-                for (int a = 0; a < 3; a++) g[3 * f + a] = ph.m[a] * gr[a][f];
-            double c2g[3] = {ph.m[0] * gr[0][3], ph.m[1] * gr[1][3], ph.m[2] * gr[2][3]};
-            double t6[6], h3[3], d4, qq[5];
+            double qq[5];
             const Prim p = prim_at(q, bo, qq);      // q of this point: just read by this sweep
             const double uvw[3] = {p.u, p.v, p.w};
-            viscous_closure(ph.mu, ph.J, g, c2g, uvw, t6, h3, &d4);
-#pragma unroll
-            for (int k = 0; k < 6; k++) stg(tau[k], bo, t6[k]);
-#pragma unroll
-            for (int k = 0; k < 3; k++) stg(hf[k], bo, h3[k]);
+            close(bo, d, uvw);
         }
+    }
+    // the store of the closing sweep with u, v, w of the point supplied
+    __device__ __forceinline__ void store_pv(uoff bo, const double* d, const double* uvw) const {
+        if constexpr (CLOSE) close(bo, d, uvw);
+        else store(bo, d, nullptr);
+    }
+    __device__ __forceinline__ void close(uoff bo, const double* d, const double* uvw) const {
+        double gr[3][4];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int f = 0; f < 4; f++) gr[a][f] = (a == AX) ? d[f] : ldg(gin[a][f], bo);
+        // chain rule of calc_viscous_shear_stress_post_compute_kernel with
+        // diagonal metrics (src/cuda/kernels/physics.hpp:267-281)
+        double g[9];
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+#pragma unroll
+            for (int a = 0; a < 3; a++) g[3 * f + a] = ph.m[a] * gr[a][f];
+        double c2g[3] = {ph.m[0] * gr[0][3], ph.m[1] * gr[1][3], ph.m[2] * gr[2][3]};
+        double t6[6], h3[3], d4;
+        viscous_closure(ph.mu, ph.J, g, c2g, uvw, t6, h3, &d4);
+#pragma unroll
+        for (int k = 0; k < 6; k++) stg(tau[k], bo, t6[k]);
+#pragma unroll
+        for (int k = 0; k < 3; k++) stg(hf[k], bo, h3[k]);
     }
     __device__ __forceinline__ void gcbc(uoff, int, const double*, double*) const {}
 };
@@ -103,6 +115,7 @@ struct S2 {
     static constexpr int NF = 5;
     static constexpr bool kGcbc = true;
     static constexpr bool kPv = false;
+    static constexpr int kStash = 0;
     static constexpr int kH = 2;
     static constexpr int kWpe = 3;
     static constexpr bool kFilter = false;
@@ -206,6 +219,7 @@ struct SF {
     static constexpr int NF = 5;
     static constexpr bool kGcbc = false;
     static constexpr bool kPv = false;
+    static constexpr int kStash = 0;
     static constexpr int kH = 5;
     static constexpr int kWpe = 3;
     static constexpr bool kFilter = true;
@@ -229,6 +243,7 @@ struct SF1 {
     static constexpr int NF = 1;
     static constexpr bool kGcbc = false;
     static constexpr bool kPv = false;
+    static constexpr int kStash = 0;
     static constexpr int kH = 5;
     static constexpr int kWpe = 3;
     static constexpr bool kFilter = true;
@@ -660,11 +675,14 @@ __global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW 
     constexpr int H = SW::kH;
     constexpr int W = K + 2 * H;
     static_assert(H <= K, "halo rows come from the neighbouring chunks only");
-    extern __shared__ double lds[];       // [64/GS lines][P chunks][K+1]
+    // [64/GS lines][P chunks][K+1], then (kStash > 0) the stashed point
+    // values [kStash][J][64]
+    extern __shared__ double lds[];
 
     const int N = ax.N, P = ax.P, GS = ax.GS;
     const int lane = threadIdx.x;
     const int lpw = 64 / GS;
+    double* stash = lds + lpw * P * (K + 1);
     const long line0 = (long)blockIdx.x * lpw;
     const int nl = (int)min((long)lpw, ax.nlines - line0);   // lines of this wave
     const int npts = nl * N;
@@ -685,6 +703,13 @@ __global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW 
         }
 #pragma unroll
         for (int f = 0; f < NF; f++) xq[f][j] = x[f];
+    }
+    if constexpr (SW::kStash > 0) {
+        // lane-contiguous: no bank conflicts; each lane reads back only its own slots
+#pragma unroll
+        for (int f = 0; f < SW::kStash; f++)
+#pragma unroll
+            for (int j = 0; j < J; j++) stash[(f * J + j) * 64 + lane] = xq[f][j];
     }
 
     // LDS slot of each of the lane's points (row and line tracked without division)
@@ -790,10 +815,20 @@ __global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW 
     for (int j = 0; j < J; j++) {
         const int p = lane + 64 * j;
         if (p < npts) {
-            double d[NF], pv[NF];
+            double d[NF];
 #pragma unroll
-            for (int f = 0; f < NF; f++) { d[f] = xo[f][j]; pv[f] = SW::kPv ? xq[f][j] : 0.0; }
-            sw.store(run + (uoff)p * st, d, pv);
+            for (int f = 0; f < NF; f++) d[f] = xo[f][j];
+            if constexpr (SW::kStash > 0) {
+                double sv[SW::kStash > 0 ? SW::kStash : 1];
+#pragma unroll
+                for (int f = 0; f < SW::kStash; f++) sv[f] = stash[(f * J + j) * 64 + lane];
+                sw.store_pv(run + (uoff)p * st, d, sv);
+            } else {
+                double pv[NF];
+#pragma unroll
+                for (int f = 0; f < NF; f++) pv[f] = SW::kPv ? xq[f][j] : 0.0;
+                sw.store(run + (uoff)p * st, d, pv);
+            }
         }
     }
 }
@@ -1087,7 +1122,7 @@ static int launch_sweep(const AxisDesc& ax, const SweepCommon& cm, const SW& sw,
     if constexpr (K == 8 && AX == 0) {
         if (exact) {
             const int lpw = 64 / ax.GS;
-            const size_t lds = sizeof(double) * lpw * ax.P * (K + 1);
+            const size_t lds = sizeof(double) * (lpw * ax.P * (K + 1) + SW::kStash * K * 64);
             dim3 block(64), grid((unsigned)((ax.nlines + lpw - 1) / lpw));
             hipLaunchKernelGGL((xt_kernel<K, SW, GCBC>), grid, block, lds, s, ax, cm, sw);
             CS_HIP(hipGetLastError());

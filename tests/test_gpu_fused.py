@@ -137,9 +137,10 @@ def test_fused_rk4_step_256_matches_oracle(oracle, filt):
     (time step at CFL 0.95, four stages with in-sweep characteristic BCs and
     stage updates, the filter along x, y, z) against the oracle's step.
 
-    The state after the step agrees to 1e-13 of its size.  The change of the
-    state over the step (3e-5 for rho on a unit state) agrees to 1e-10 without
-    the filter.  The filter's matrix (alpha_f = 0.49) has a condition number
+    Density and energy (unit-size base values) agree to 1e-13 of their size
+    after the step.  The change of the state over the step (3e-5 for rho on a
+    unit state; the momenta are their own change, their base is 0) agrees to
+    1e-10 without the filter.  The filter's matrix (alpha_f = 0.49) has a condition number
     of about 99, so two correct solves of its rows -- the chunked recurrences
     here, Thomas in the oracle -- differ by up to cond * eps * |q|, about 2e-14
     of the unit state, i.e. about 1e-9 of rho's change: with the filter the
@@ -167,7 +168,8 @@ def test_fused_rk4_step_256_matches_oracle(oracle, filt):
         e_state = nrel(qh[sl], qo[sl])
         e_change = nrel(qh[sl] - st["qa"][sl], qo[sl] - st["qa"][sl])
         print(f"variable {v}: state {e_state:.2e}, change {e_change:.2e}", flush=True)
-        assert e_state < 1e-13, f"variable {v}"
+        if v in (0, 4):
+            assert e_state < 1e-13, f"variable {v}"
         assert e_change < bar, f"variable {v}"
 
 
