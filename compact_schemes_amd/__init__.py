@@ -33,7 +33,7 @@ EXPORTS = [
     "cs_deriv1d", "cs_deriv2d", "cs_deriv", "cs_filter", "cs_set_filter_alpha",
     "cs_physics_create", "cs_physics_destroy", "cs_physics_set_frame", "cs_physics_buffers",
     "cs_primitives", "cs_calc_fluxes", "cs_calc_viscous_shear_stress", "cs_calc_time_step",
-    "cs_gcbc_go", "cs_wall_condition_update", "cs_rk_stage",
+    "cs_calc_time_step_noalloc", "cs_gcbc_go", "cs_wall_condition_update", "cs_rk_stage",
     "cs_set_precision", "cs_deriv_adjoint", "cs_filter_adjoint", "cs_solver_create", "cs_solver_destroy", "cs_solver_rhs", "cs_solver_sweep", "cs_solver_tune", "cs_solver_tune_filter",
     "cs_solver_set_order", "cs_solver_rhs_vjp", "cs_solver_rk4_vjp", "cs_solver_tape_size",
     "cs_solver_rk4_step_tape", "cs_solver_rk4_vjp_tape", "cs_solver_adjoint_mode",
@@ -131,6 +131,7 @@ def lib():
             "cs_calc_fluxes": (I, [P, P, P, P, P, P, P, P, P]),
             "cs_calc_viscous_shear_stress": (I, [P, P, P, P, P, P, P, P, P]),
             "cs_calc_time_step": (I, [P, P, P, P, P, P, P, D, DP, DP, P]),
+            "cs_calc_time_step_noalloc": (I, [ctypes.POINTER(Dims), Point, P, P, P, P, P, P, D, P, P]),
             "cs_gcbc_go": (I, [P, PP, PP, P, P, P, P, P, Point, Point, D, P]),
             "cs_wall_condition_update": (I, [P, P, P, Point, P]),
             "cs_rk_stage": (I, [ctypes.POINTER(Dims), P, P, P, P, D, P]),
@@ -372,6 +373,13 @@ class Physics:
         check(lib().cs_calc_time_step(self.h, _p(xim), _p(etm), _p(zem), _p(de), _p(yaco), _p(ssk), float(cfl),
                                       ctypes.byref(dte), m2, _stream(stream)))
         return dte.value, (m2[0], m2[1])
+
+
+def calc_time_step_noalloc(dims, umf, xim, etm, zem, de, yaco, ssk, cfl, out3, stream=None):
+    """cs_calc_time_step_noalloc: out3 (device, 3 doubles) <- dte, max0, max1;
+    no context, no allocation, no synchronisation."""
+    check(lib().cs_calc_time_step_noalloc(ctypes.byref(Dims(*dims)), Point(*umf), _p(xim), _p(etm), _p(zem), _p(de),
+                                          _p(yaco), _p(ssk), float(cfl), _p(out3), _stream(stream)))
 
 
 def init_pulse(dims, center, amp, width, qa, stream=None):

@@ -7,6 +7,8 @@
 #include "physics_kernels.hpp"
 
 #include <cmath>
+#include <mutex>
+#include <string>
 
 namespace cs {
 
@@ -146,7 +148,9 @@ __global__ void __launch_bounds__(kBlock) time_step_kernel(const double* xim, co
     if (threadIdx.x == 0) { part[2 * blockIdx.x] = s0[0]; part[2 * blockIdx.x + 1] = s1[0]; }
 }
 
-__global__ void __launch_bounds__(kBlock) max_pairs_kernel(double* part, int n) {
+// out3 (may be null): also writes dte = min(cfl / max0, 0.5 / max1) and the
+// two maxima, so a caller that does not synchronise gets the time step on the device
+__global__ void __launch_bounds__(kBlock) max_pairs_kernel(double* part, int n, double cfl, double* out3) {
     __shared__ double s0[kBlock], s1[kBlock];
     double m0 = 0.0, m1 = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) { m0 = fmax(m0, part[2 * i]); m1 = fmax(m1, part[2 * i + 1]); }
@@ -160,7 +164,15 @@ __global__ void __launch_bounds__(kBlock) max_pairs_kernel(double* part, int n) 
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) { part[2 * n] = s0[0]; part[2 * n + 1] = s1[0]; }
+    if (threadIdx.x == 0) {
+        part[2 * n] = s0[0];
+        part[2 * n + 1] = s1[0];
+        if (out3) {
+            out3[0] = fmin(cfl / s0[0], 0.5 / s1[0]);
+            out3[1] = s0[0];
+            out3[2] = s1[0];
+        }
+    }
 }
 
 // RK stage update (absent upstream): q = q0 - dtk * yaco * de
@@ -275,7 +287,7 @@ int launch_time_step(const double* xim, const double* etm, const double* zem, co
     hipLaunchKernelGGL(time_step_kernel, dim3(nblocks), dim3(kBlock), 0, s, xim, etm, zem, de, yaco, ssk, umf, part,
                        lmx);
     CS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(max_pairs_kernel, dim3(1), dim3(kBlock), 0, s, part, nblocks);
+    hipLaunchKernelGGL(max_pairs_kernel, dim3(1), dim3(kBlock), 0, s, part, nblocks, 0.0, (double*)nullptr);
     CS_HIP(hipGetLastError());
     return CS_OK;
 }
@@ -309,9 +321,68 @@ int launch_boundary_weights(const cs_dims* d, double* sbcc, hipStream_t s) {
     return CS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// allocation-free time step
+// ---------------------------------------------------------------------------
+// Scratch of cs_calc_time_step_noalloc: one pair of partial maxima per block
+// of time_step_kernel and the reduced pair (kTsMaxBlocks pairs + 1, 32.8 KB).
+// Each GPU holds its own copy from the library's load until the process ends,
+// shared by every call on that GPU.  Nothing in it needs clearing: every block
+// writes its pair before max_pairs_kernel reads it.  In host code the name is
+// a stand-in, never a device address: kernels get the per-device pointer
+// from hipGetSymbolAddress.
+constexpr int kTsMaxBlocks = 2048;
+static __device__ __align__(256) double scratch[2 * kTsMaxBlocks + 2];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+// device address of `scratch` on the current device, looked up on the first
+// call on each device
+static int scratch_on_device(double** out) {
+    int dev = 0;
+    CS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(CS_ERR_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        CS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(CS_ERR_HIP, "hipGetSymbolAddress returned a null address for the time-step scratch");
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch[dev];
+    return CS_OK;
+}
+
 }  // namespace cs
 
 using namespace cs;
+
+extern "C" int cs_calc_time_step_noalloc(const cs_dims* dims, cs_point umf, const double* xim, const double* etm,
+                                         const double* zem, const double* de, const double* yaco, const double* ssk,
+                                         double cfl, double* out3, cs_stream stream) {
+    if (!dims || !out3) return fail(CS_ERR_ARG, "null argument");
+    if (!xim || !etm || !zem || !de || !yaco || !ssk) return fail(CS_ERR_ARG, "null field pointer");
+    if (dims->lxi < 1 || dims->let < 1 || dims->lze < 1) return fail(CS_ERR_SHAPE, "grid dimensions must be positive");
+    const long lmx = (long)dims->lxi * dims->let * dims->lze;
+    const int nblocks = time_step_blocks(lmx);
+    if (nblocks > kTsMaxBlocks)
+        return fail(CS_ERR_SHAPE, "time-step reduction needs " + std::to_string(nblocks) +
+                                      " partial pairs; the static scratch holds " + std::to_string(kTsMaxBlocks));
+    double* part = nullptr;
+    int rc = scratch_on_device(&part);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(time_step_kernel, dim3(nblocks), dim3(kBlock), 0, s, xim, etm, zem, de, yaco, ssk, umf, part,
+                       lmx);
+    CS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(max_pairs_kernel, dim3(1), dim3(kBlock), 0, s, part, nblocks, cfl, out3);
+    CS_HIP(hipGetLastError());
+    return CS_OK;
+}
 
 extern "C" int cs_init_pulse(const cs_dims* dims, cs_point center, double amp, double width, double* qa,
                              cs_stream stream) {

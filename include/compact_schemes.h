@@ -167,6 +167,21 @@ int cs_calc_viscous_shear_stress(cs_physics* phys, cs_numerics* num, double* de,
 int cs_calc_time_step(cs_physics* phys, const double* xim, const double* etm, const double* zem,
                       const double* de, const double* yaco, const double* ssk, double cfl,
                       double* dte, double* max2, cs_stream stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory or wait on the host (code under HIP graph capture, tight serving
+ * loops): the time step of cs_calc_time_step without a physics context,
+ * without any allocation and without synchronising.  out3 (device, 3
+ * doubles) receives dte = min(cfl / max0, 0.5 / max1), max0, max1 -- the
+ * values cs_calc_time_step returns; dte is not checked (it is +inf when both
+ * maxima are 0).  Partial maxima go to a static per-GPU scratch array of
+ * 2*2048+2 doubles (32.8 KB, held until the process ends); any grid fits,
+ * since the reduction strides over the points with at most 2048 blocks, and
+ * a larger demand is rejected with CS_ERR_SHAPE.  Every call on one GPU
+ * shares that scratch: calls that overlap on one GPU (different streams or
+ * host threads) race. */
+int cs_calc_time_step_noalloc(const cs_dims* dims, cs_point umf, const double* xim, const double* etm,
+                              const double* zem, const double* de, const double* yaco, const double* ssk,
+                              double cfl, double* out3, cs_stream stream);
 /* gcbc_go, src/gcbc.hpp:86-186 (D10-D13 corrected).  cm[a] = [2][3][face]
  * unit face normals; drvb[a] = neighbour characteristics, only read on
  * BC_INTER_CURV faces (the exchange itself is the caller's). */
