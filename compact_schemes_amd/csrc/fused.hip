@@ -488,7 +488,8 @@ __device__ __forceinline__ void face_corrections(const SW& sw, uoff base, uoff s
 // LDS for y/z lines)
 // ---------------------------------------------------------------------------
 template <int K, int AX, int TL, class SW, bool GCBC>
-__global__ void __launch_bounds__(AX == 0 ? 256 : TL * kMaxP) __attribute__((amdgpu_waves_per_eu(AX == 0 ? 1 : SW::kWpe)))
+__global__ void __launch_bounds__(AX == 0 ? 256 : TL * kMaxP)
+__attribute__((amdgpu_waves_per_eu(AX == 0 ? 1 : (K == 16 ? 2 : SW::kWpe))))
 direct_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
     constexpr int NF = SW::NF;
     constexpr int H = SW::kH;
@@ -1338,27 +1339,27 @@ static int s2_sweep(cs_solver* sv, int k, const SweepCommon& cm, const double* q
                     int div, hipStream_t s) {
     const int ax = kS2Axes[sv->order][k];
     if (k == 0) {
-        if (ax == 0) return dispatch<0>(sv->ax[0], cm, make_s2<0, 0, V>(sv, q, nullptr, sv->dpart, nullptr, 0), s);
-        return dispatch<2>(sv->ax[2], cm, make_s2<2, 0, V>(sv, q, nullptr, sv->dpart, nullptr, 0), s);
+        if (ax == 0) return dispatch<0>(sv->rax[0], cm, make_s2<0, 0, V>(sv, q, nullptr, sv->dpart, nullptr, 0), s);
+        return dispatch<2>(sv->rax[2], cm, make_s2<2, 0, V>(sv, q, nullptr, sv->dpart, nullptr, 0), s);
     }
-    if (k == 1) return dispatch<1>(sv->ax[1], cm, make_s2<1, 1, V>(sv, q, sv->dpart, sv->dpart, nullptr, 0), s);
+    if (k == 1) return dispatch<1>(sv->rax[1], cm, make_s2<1, 1, V>(sv, q, sv->dpart, sv->dpart, nullptr, 0), s);
     if (ax == 2) {
-        if (q0) return dispatch<2>(sv->ax[2], cm, make_s2<2, 3, V>(sv, q, sv->dpart, out, q0, div), s);
-        return dispatch<2>(sv->ax[2], cm, make_s2<2, 2, V>(sv, q, sv->dpart, out, nullptr, 0), s);
+        if (q0) return dispatch<2>(sv->rax[2], cm, make_s2<2, 3, V>(sv, q, sv->dpart, out, q0, div), s);
+        return dispatch<2>(sv->rax[2], cm, make_s2<2, 2, V>(sv, q, sv->dpart, out, nullptr, 0), s);
     }
-    if (q0) return dispatch<0>(sv->ax[0], cm, make_s2<0, 3, V>(sv, q, sv->dpart, out, q0, div), s);
-    return dispatch<0>(sv->ax[0], cm, make_s2<0, 2, V>(sv, q, sv->dpart, out, nullptr, 0), s);
+    if (q0) return dispatch<0>(sv->rax[0], cm, make_s2<0, 3, V>(sv, q, sv->dpart, out, q0, div), s);
+    return dispatch<0>(sv->rax[0], cm, make_s2<0, 2, V>(sv, q, sv->dpart, out, nullptr, 0), s);
 }
 
 static int s1_sweep(cs_solver* sv, int k, const SweepCommon& cm, const double* q, hipStream_t s) {
     if (sv->order == 0) {
-        if (k == 0) return dispatch<0>(sv->ax[0], cm, make_s1<0, false>(sv, q), s);
-        if (k == 1) return dispatch<1>(sv->ax[1], cm, make_s1<1, false>(sv, q), s);
-        return dispatch<2>(sv->ax[2], cm, make_s1<2, true>(sv, q), s);
+        if (k == 0) return dispatch<0>(sv->rax[0], cm, make_s1<0, false>(sv, q), s);
+        if (k == 1) return dispatch<1>(sv->rax[1], cm, make_s1<1, false>(sv, q), s);
+        return dispatch<2>(sv->rax[2], cm, make_s1<2, true>(sv, q), s);
     }
-    if (k == 0) return dispatch<1>(sv->ax[1], cm, make_s1<1, false>(sv, q), s);
-    if (k == 1) return dispatch<2>(sv->ax[2], cm, make_s1<2, false>(sv, q), s);
-    return dispatch<0>(sv->ax[0], cm, make_s1<0, true>(sv, q), s);
+    if (k == 0) return dispatch<1>(sv->rax[1], cm, make_s1<1, false>(sv, q), s);
+    if (k == 1) return dispatch<2>(sv->rax[2], cm, make_s1<2, false>(sv, q), s);
+    return dispatch<0>(sv->rax[0], cm, make_s1<0, true>(sv, q), s);
 }
 
 static int sweep_at(cs_solver* sv, int pos, const SweepCommon& cm, const double* q, double* out, const double* q0,

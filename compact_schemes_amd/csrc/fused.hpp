@@ -21,7 +21,7 @@ struct AxisDesc {
     int seam;            // both faces seam (periodic wrap on one subdomain)
     int gcbc;            // both faces non-reflective characteristic BCs
     // kernel variant (chosen per axis, see cs_solver_tune):
-    int variant;         // 0: direct (ragged when N != P*K); 1: LDS row-block tiles
+    int variant;         // 0: direct (ragged when N != P*K); 1: LDS row-block tiles; 2: direct, 16-point chunks
     int tl;              // lines per workgroup (y/z lines of variant 0, all of variant 1): 8 or 16
     int xcd;             // 1: neighbouring line groups go to workgroups on the same XCD
     // filter sweep along this axis: 1 = one field per workgroup (grid.y over
@@ -45,8 +45,12 @@ struct cs_solver {
     cs_solver_config cfg;
     cs::Geo g;
     cs_numerics* num = nullptr;         // interface table, filter operators
-    cs::fused::AxisDesc ax[3];
+    cs::fused::AxisDesc ax[3];     // 8-point chunks up to 256 points: filter sweeps, adjoint sweeps
+    // the right-hand-side sweeps: ax, or the same axis cut into 16-point
+    // chunks (cs_solver_tune variant 2, with tabs16)
+    cs::fused::AxisDesc rax[3];
     double* tabs[3] = {nullptr, nullptr, nullptr};
+    double* tabs16[3] = {nullptr, nullptr, nullptr};
     double* ftabs[3] = {nullptr, nullptr, nullptr};
     int order = 1;             // sweep order (cs_solver_set_order)
     double* ga = nullptr;      // [4][lmx] gradients of the first S1 sweep

@@ -80,7 +80,10 @@ class FusedSolver:
         self.de = torch.zeros(5 * lmx, dtype=torch.float64, device="cuda")
         self.num = None
         self.nbc_flat = flat
-        self._tuned = {0: (0, 16, 0), 1: (0, 16, 1), 2: (0, 16, 0)}
+        # launch shapes of the RHS sweeps per axis (cs_solver_create's defaults:
+        # z lines of 32..256 points, a multiple of 16, run 16-point chunks)
+        nz = self.dims[2]
+        self._tuned = {0: (0, 16, 0), 1: (0, 16, 1), 2: (2 if (nz % 16 == 0 and 32 <= nz <= 256) else 0, 16, 0)}
         self._ftuned = {0: (5, 16), 1: (1, 16), 2: (1, 16)}
         self.order = 1
         # the filter of cs_solver_rk4_step runs as one sweep per axis when every
@@ -133,10 +136,10 @@ class FusedSolver:
             return f"cs::fused::direct_kernel<8, {ax}, {lines}, {sf}, false>"
         ax = S1_AXES[self.order][which] if which < 3 else S2_AXES[self.order][min(which, 5) - 3]
         n = self.dims[ax]
-        K = 8 if n <= 256 else 16
+        variant, lines, _ = self._tuned[ax]
+        K = 8 if n <= 256 and variant != 2 else 16
         P = (n + K - 1) // K
         exact = n == P * K
-        variant, lines, _ = self._tuned[ax]
         nbc = self.nbc_flat
         gcbc = nbc[ax] == 10 and nbc[3 + ax] == 10
         if which < 3:

@@ -255,9 +255,11 @@ int cs_solver_sweep(cs_solver* sv, int which, const double* qa, double* de, cons
  * reference specialises its NVRTC kernels with -D macros instead,
  * src/numerics_rtc.hpp:120-126): variant 0 = each chunk thread evaluates its
  * own rows (halo rows exchanged between chunks), 1 = LDS row-block tiles
- * with coalesced loads; lines = 8 or 16 lines per workgroup (y/z lines of
- * variant 0, all lines of variant 1); xcd = 1 keeps neighbouring line groups
- * on one XCD.  Results agree to rounding whatever the choice. */
+ * with coalesced loads, 2 = variant 0 on 16-point chunks (y/z lines of 32 to
+ * 256 points, a multiple of 16; the default along z); lines = 8 or 16 lines
+ * per workgroup (y/z lines of variants 0 and 2, all lines of variant 1);
+ * xcd = 1 keeps neighbouring line groups on one XCD.  Results agree to
+ * rounding whatever the choice. */
 int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd);
 /* launch shape of the filter sweep along axis (no reference counterpart):
  * fields = 1 filters one conserved variable per workgroup (the five variables

@@ -266,3 +266,30 @@ def test_fused_rk4_graph_replays_match_eager(dims, nbc):
         assert torch.isfinite(junk)
         assert torch.isfinite(qe).all(), rep
         assert torch.equal(q, qe), rep
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nbc", [CL, PER], ids=["closure", "periodic"])
+@pytest.mark.parametrize("axis", [1, 2])
+@pytest.mark.parametrize("shape", [(0, 16, 0), (0, 8, 1), (1, 16, 0), (2, 16, 0), (2, 8, 1)],
+                         ids=["direct16", "direct8-xcd", "tile16", "chunks16", "chunks16-8-xcd"])
+def test_fused_launch_shapes_agree(oracle, nbc, axis, shape):
+    """Every launch shape of the y/z sweeps (cs_solver_tune: direct kernel,
+    row-block tiles, 16-point chunks; 8 or 16 lines per workgroup; XCD
+    grouping) gives the oracle's RHS and the default shape's to rounding."""
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (32, 48, 64)
+    lmx = dims[0] * dims[1] * dims[2]
+    st = _state(dims, seed=11)
+    q = dev(st["qa"])
+    ref = _oracle_rhs(oracle, dims, nbc, st, True)
+    base = FusedSolver(dims, nbc, st)
+    d0 = host(base.rhs(q)).copy()
+    sv = FusedSolver(dims, nbc, st)
+    sv.tune(axis, *shape)
+    case(f"fused launch shape axis={axis} shape={shape} nbc={nbc[0]}")
+    de = host(sv.rhs(q))
+    for m in range(5):
+        sl = slice(m * lmx, (m + 1) * lmx)
+        assert nrel(de[sl], ref[sl]) < 1e-12, f"variable {m}"
+        assert nrel(de[sl], d0[sl]) < 1e-13, f"variable {m}"
