@@ -140,7 +140,7 @@ def lib():
             "cs_solver_rhs": (I, [P, P, P, P, P]),
             "cs_solver_sweep": (I, [P, I, P, P, P, P]),
             "cs_solver_tune": (I, [P, I, I, I, I]),
-            "cs_solver_tune_filter": (I, [P, I, I, I]),
+            "cs_solver_tune_filter": (I, [P, I, I, I, I]),
             "cs_solver_set_order": (I, [P, I]),
             "cs_solver_rhs_vjp": (I, [P, P, P, P, P, P]),
             "cs_solver_rk4_vjp": (I, [P, P, P, D, P, P]),

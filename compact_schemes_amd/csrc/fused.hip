@@ -1171,14 +1171,15 @@ static int launch_sweep(const AxisDesc& ax, const SweepCommon& cm, const SW& sw,
     return CS_OK;
 }
 
-// the filter sweep: equal 8-point chunks only (filter_fusable); SW = SF<AX>
-// (five variables per workgroup) or SF1<AX> (one variable per workgroup,
-// grid.y = 5)
-template <int AX, class SW>
+// the filter sweep on equal chunks of K points (8 along x; 8 or 16 along y/z,
+// the axis's filter descriptor); SW = SF<AX> (five variables per workgroup)
+// or SF1<AX> (one variable per workgroup, grid.y = 5)
+template <int K, int AX, class SW>
 static int launch_filter(const AxisDesc& ax, const SweepCommon& cm, const SW& sw, hipStream_t s) {
-    constexpr int K = 8, NF = SW::NF, H = SW::kH;
+    constexpr int NF = SW::NF, H = SW::kH;
+    static_assert(AX != 0 || K == 8, "x filter sweeps run 8-point chunks");
     const unsigned gy = NF == 1 ? 5u : 1u;
-    if (ax.N != ax.P * K || !ax.ftab) return fail(CS_ERR_STATE, "fused filter needs equal 8-point chunks");
+    if (ax.K != K || ax.N != ax.P * K || !ax.ftab) return fail(CS_ERR_STATE, "fused filter needs equal chunks");
     if constexpr (AX == 0) {
         const int lpw = 64 / ax.GS;
         const size_t lds = sizeof(double) * lpw * ax.P * (K + 1);
@@ -1405,14 +1406,18 @@ bool filter_fusable(const cs_solver* sv) {
 
 template <int AX>
 static int filter_axis(cs_solver* sv, double* q, const SweepCommon& cm, hipStream_t s) {
-    const AxisDesc& ax = sv->ax[AX];
+    const AxisDesc& ax = sv->fax[AX];
     if (ax.ffields == 1) {
         SF1<AX> f{q, sv->g.lmx};
-        return launch_filter<AX>(ax, cm, f, s);
+        if constexpr (AX != 0)
+            if (ax.K == 16) return launch_filter<16, AX>(ax, cm, f, s);
+        return launch_filter<8, AX>(ax, cm, f, s);
     }
     SF<AX> f{};
     fields(q, sv->g.lmx, f.q);
-    return launch_filter<AX>(ax, cm, f, s);
+    if constexpr (AX != 0)
+        if (ax.K == 16) return launch_filter<16, AX>(ax, cm, f, s);
+    return launch_filter<8, AX>(ax, cm, f, s);
 }
 
 // the compact filter of the five conserved variables in place: along x, y, z

@@ -49,8 +49,11 @@ struct cs_solver {
     // the right-hand-side sweeps: ax, or the same axis cut into 16-point
     // chunks (cs_solver_tune variant 2, with tabs16)
     cs::fused::AxisDesc rax[3];
+    // the filter sweeps: ax, or (y/z, cs_solver_tune_filter) 16-point chunks with ftabs16
+    cs::fused::AxisDesc fax[3];
     double* tabs[3] = {nullptr, nullptr, nullptr};
     double* tabs16[3] = {nullptr, nullptr, nullptr};
+    double* ftabs16[3] = {nullptr, nullptr, nullptr};
     double* ftabs[3] = {nullptr, nullptr, nullptr};
     int order = 1;             // sweep order (cs_solver_set_order)
     double* ga = nullptr;      // [4][lmx] gradients of the first S1 sweep

@@ -85,6 +85,7 @@ int cs_solver_destroy(cs_solver* sv) {
     if (!sv) return CS_OK;
     double* bufs[] = {sv->ga, sv->gb, sv->tau, sv->hf, sv->dpart, sv->qb, sv->qc, sv->red, sv->dt, sv->vjp,
                       sv->tabs[0], sv->tabs[1], sv->tabs[2], sv->tabs16[0], sv->tabs16[1], sv->tabs16[2],
+                      sv->ftabs16[0], sv->ftabs16[1], sv->ftabs16[2],
                       sv->ftabs[0], sv->ftabs[1], sv->ftabs[2],
                       sv->ttabs[0], sv->ttabs[1], sv->ttabs[2], sv->fttabs[0], sv->fttabs[1], sv->fttabs[2],
                       sv->adj};
@@ -119,6 +120,33 @@ static int use_chunks16(cs_solver* sv, int axis, int lines, int xcd) {
     r.tl = lines;
     r.xcd = xcd ? 1 : 0;
     sv->rax[axis] = r;
+    return CS_OK;
+}
+
+// the filter sweep of `axis` (y or z) on 16-point chunks (lines of 32..256
+// points, a multiple of 16) or back on the 8-point chunks of ax
+static int filter_chunks(cs_solver* sv, int axis, int chunk) {
+    const AxisDesc& ax = sv->ax[axis];
+    AxisDesc f = ax;
+    f.ffields = sv->fax[axis].ffields;
+    f.ftl = sv->fax[axis].ftl;
+    if (chunk == 16) {
+        if (axis == 0 || !ax.ftab || ax.N % 16 != 0 || ax.N < 32)
+            return fail(CS_ERR_UNSUPPORTED, "16-point filter chunks: y or z lines of 32..256 points, a multiple of 16");
+        if (!sv->ftabs16[axis]) {
+            std::vector<double> hf;
+            int rc = build_line_table(OP_FILTER, ax.N, ax.seam, ax.seam, sv->cfg.alpha_f, 16, hf);
+            if (rc) return rc;
+            CS_HIP(hipMalloc(&sv->ftabs16[axis], hf.size() * sizeof(double)));
+            CS_HIP(hipMemcpy(sv->ftabs16[axis], hf.data(), hf.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        f.K = 16;
+        f.P = ax.N / 16;
+        f.GS = 1;
+        while (f.GS < f.P) f.GS <<= 1;
+        f.ftab = sv->ftabs16[axis];
+    }
+    sv->fax[axis] = f;
     return CS_OK;
 }
 
@@ -189,7 +217,14 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
             ax.ftab = sv->ftabs[a];
         }
     }
-    for (int a = 0; a < 3; a++) sv->rax[a] = sv->ax[a];
+    for (int a = 0; a < 3; a++) sv->rax[a] = sv->fax[a] = sv->ax[a];
+    // measured default (256^3, MI355X): the y and z filter sweeps run 16-point chunks
+    for (int a = 1; a < 3 && cfg->filter; a++)
+        if (sv->ax[a].ftab && sv->ax[a].N % 16 == 0 && sv->ax[a].N >= 32 && (rc = filter_chunks(sv, a, 16))) {
+            std::string msg = cs_last_error();
+            cs_solver_destroy(sv);
+            return fail(rc, msg);
+        }
     // measured default (256^3, MI355X): z lines of up to 256 points run the
     // right-hand-side sweeps on 16-point chunks (S1z 0.345 -> 0.305 ms, S2z
     // 0.525 -> 0.518 ms; y lines measured the same or slower)
@@ -246,12 +281,15 @@ int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd) {
     return CS_OK;
 }
 
-int cs_solver_tune_filter(cs_solver* sv, int axis, int fields, int lines) {
+int cs_solver_tune_filter(cs_solver* sv, int axis, int fields, int lines, int chunk) {
     if (!sv || axis < 0 || axis > 2) return fail(CS_ERR_ARG, "bad solver or axis");
-    if ((fields != 1 && fields != 5) || (lines != 8 && lines != 16))
-        return fail(CS_ERR_ARG, "fields 1/5, lines 8/16");
-    sv->ax[axis].ffields = fields;
-    sv->ax[axis].ftl = lines;
+    if ((fields != 1 && fields != 5) || (lines != 8 && lines != 16) || (chunk != 8 && chunk != 16))
+        return fail(CS_ERR_ARG, "fields 1/5, lines 8/16, chunk 8/16");
+    if (!filter_fusable(sv)) return fail(CS_ERR_UNSUPPORTED, "the fused filter needs equal 8-point chunks");
+    int rc = filter_chunks(sv, axis, chunk);
+    if (rc) return rc;
+    sv->ax[axis].ffields = sv->fax[axis].ffields = fields;
+    sv->ax[axis].ftl = sv->fax[axis].ftl = lines;
     return CS_OK;
 }
 

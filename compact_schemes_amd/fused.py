@@ -84,7 +84,10 @@ class FusedSolver:
         # z lines of 32..256 points, a multiple of 16, run 16-point chunks)
         nz = self.dims[2]
         self._tuned = {0: (0, 16, 0), 1: (0, 16, 1), 2: (2 if (nz % 16 == 0 and 32 <= nz <= 256) else 0, 16, 0)}
-        self._ftuned = {0: (5, 16), 1: (1, 16), 2: (1, 16)}
+        # filter sweeps: (fields per workgroup, lines, points per chunk); y/z
+        # lines of 32..256 points, a multiple of 16, run 16-point chunks
+        fch = [16 if (n % 16 == 0 and 32 <= n <= 256) else 8 for n in self.dims]
+        self._ftuned = {0: (5, 16, 8), 1: (1, 16, fch[1]), 2: (1, 16, fch[2])}
         self.order = 1
         # the filter of cs_solver_rk4_step runs as one sweep per axis when every
         # axis splits into 8-point chunks (else one line-solver launch per axis)
@@ -114,9 +117,9 @@ class FusedSolver:
         check(lib().cs_solver_tune(self.h, int(axis), int(variant), int(lines), int(xcd)))
         self._tuned[int(axis)] = (int(variant), int(lines), int(xcd))
 
-    def tune_filter(self, axis, fields, lines=16):
-        check(lib().cs_solver_tune_filter(self.h, int(axis), int(fields), int(lines)))
-        self._ftuned[int(axis)] = (int(fields), int(lines))
+    def tune_filter(self, axis, fields, lines=16, chunk=8):
+        check(lib().cs_solver_tune_filter(self.h, int(axis), int(fields), int(lines), int(chunk)))
+        self._ftuned[int(axis)] = (int(fields), int(lines), int(chunk))
 
     def set_order(self, order):
         check(lib().cs_solver_set_order(self.h, int(order)))
@@ -129,11 +132,11 @@ class FusedSolver:
             return " + ".join(self.kernel_name(w) for w in FILTER_SWEEPS)
         if which >= 8:
             ax = which - 8
-            fields, lines = self._ftuned[ax]
+            fields, lines, chunk = self._ftuned[ax]
             sf = f"cs::fused::SF{'1' if fields == 1 else ''}<{ax}>"
             if ax == 0:
                 return f"cs::fused::xt_kernel<8, {sf}, false>"
-            return f"cs::fused::direct_kernel<8, {ax}, {lines}, {sf}, false>"
+            return f"cs::fused::direct_kernel<{chunk}, {ax}, {lines}, {sf}, false>"
         ax = S1_AXES[self.order][which] if which < 3 else S2_AXES[self.order][min(which, 5) - 3]
         n = self.dims[ax]
         variant, lines, _ = self._tuned[ax]

@@ -265,8 +265,10 @@ int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd);
  * fields = 1 filters one conserved variable per workgroup (the five variables
  * spread over the grid; default along y and z), 5 all five per workgroup
  * (default along x); lines = 8 or 16 lines per workgroup of the y/z sweeps
- * (default 16).  Results agree to rounding whatever the choice. */
-int cs_solver_tune_filter(cs_solver* sv, int axis, int fields, int lines);
+ * (default 16); chunk = 8 or 16 points per chunk thread (16: y/z lines of 32
+ * to 256 points, a multiple of 16; the default there).  Results agree to
+ * rounding whatever the choice. */
+int cs_solver_tune_filter(cs_solver* sv, int axis, int fields, int lines, int chunk);
 /* order of the six sweeps (no reference counterpart): 0 = gradients along
  * x, y, then z with the stress closure, fluxes along x, y, then z with the
  * RHS/stage write; 1 (default) = gradients y, z, x (closure in the x sweep),

@@ -189,11 +189,13 @@ def test_fused_errors():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("dims,nbc", [((32, 24, 40), CL), ((40, 32, 48), MIX), ((24, 32, 16), PER)])
-@pytest.mark.parametrize("shape", [(1, 16), (1, 8), (5, 16), (5, 8)], ids=lambda s: f"fields{s[0]}-lines{s[1]}")
+@pytest.mark.parametrize("shape", [(1, 16, 8), (1, 8, 8), (5, 16, 8), (5, 8, 8), (1, 16, 16), (1, 8, 16), (5, 16, 16)],
+                         ids=lambda s: f"fields{s[0]}-lines{s[1]}-chunk{s[2]}")
 def test_fused_filter_matches_oracle(oracle, dims, nbc, shape):
     """The filter sweeps (x, y, z) against the oracle's filter of each
-    variable along x, then y, then z, for each launch shape; the shapes agree
-    to rounding, and the three single-axis sweeps run in turn bit for bit."""
+    variable along x, then y, then z, for each launch shape (16-point chunks
+    on the y/z axes that take them); the shapes agree to rounding, and the
+    three single-axis sweeps run in turn bit for bit."""
     import torch
     from compact_schemes_amd.fused import FusedSolver
     st = _state(dims, seed=8)
@@ -204,8 +206,10 @@ def test_fused_filter_matches_oracle(oracle, dims, nbc, shape):
     ref_sv = FusedSolver(dims, nbc, st)                # default launch shapes
     want = torch.zeros_like(q)
     ref_sv.sweep(7, q, want)
+    fields, lines, chunk = shape
     for a in range(3):
-        sv.tune_filter(a, *shape)
+        ok16 = a > 0 and dims[a] % 16 == 0 and 32 <= dims[a] <= 256
+        sv.tune_filter(a, fields, lines, chunk if ok16 else 8)
     out = torch.zeros_like(q)
     sv.sweep(7, q, out)
     got = host(out)
