@@ -398,32 +398,28 @@ def batch_bench(args):
 
 def precision_bench(args, dims=(512, 512, 256), nsteps=50):
     """Config 4: the flow of the RHS workload on 512x512x256 with periodic
-    faces, 50 RK4 steps (reference-ordered operators, viscous, filter every
-    step, dt fixed from the CFL limit of the initial state) with FP64 line
-    solves, then with FP32 solves + one and + two FP64-residual refinement
-    sweeps (fluxes, boundary rows, stage update in FP64; the mixed filter
-    solves for its correction x - f, so the FP32 rounding scales with the
-    correction, not with the field).  Reports the error
-    of the mixed runs against the FP64 run per conserved variable, relative
-    to the state and to its change over the run; value = the largest
-    state-relative error with one sweep."""
-    import numpy as np
+    faces, 50 RK4 steps (viscous, filter every step, dt fixed from the CFL
+    limit of the initial state).  Primary: the fused solver with FP64
+    intermediates against the same solver with the viscous intermediates
+    (gradients, stress, heat flux) stored in FP32 (cs_solver_set_precision
+    1: 588 instead of 736 HBM bytes per point and RHS).  Secondary: the
+    reference-ordered operators with FP64 line solves against FP32 solves +
+    one FP64-residual refinement sweep (cs_set_precision 1).  Errors per
+    conserved variable relative to the state and to its change over the run;
+    value = the largest state-relative error of the fused FP32 run."""
     import torch
     from compact_schemes_amd import workloads as wl
+    from compact_schemes_amd.fused import FusedSolver
     from compact_schemes_amd.stepper import Stepper
     per = wl.NBC_PERIODIC
     st = wl.flow_state(dims, seed=4)
     lmx = dims[0] * dims[1] * dims[2]
     q0 = torch.from_numpy(st["qa"]).cuda()
     del st["qa"]
-    res, times = {}, {}
-    dt = None
-    for mode in (0, 1, 2):
-        stepper = Stepper(dims, per, True, st["umf"], st)
-        stepper.num.set_precision(mode)
+
+    def run(make, tag):
+        stepper = make()
         q = q0.clone()
-        if dt is None:
-            dt = stepper.time_step(q, 0.95)
         stepper.rk4_step(q, dt=dt)                 # warm-up step, then restart
         q.copy_(q0)
         torch.cuda.synchronize()
@@ -432,18 +428,34 @@ def precision_bench(args, dims=(512, 512, 256), nsteps=50):
             stepper.rk4_step(q, dt=dt)
             if (k + 1) % 10 == 0:
                 torch.cuda.synchronize()
-                print(f"[precision] mode {mode}: {k + 1} steps", file=sys.stderr, flush=True)
+                print(f"[precision] {tag}: {k + 1} steps", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
-        times[mode] = (time.perf_counter() - t0) / nsteps
-        res[mode] = q
+        t = (time.perf_counter() - t0) / nsteps
         del stepper
         torch.cuda.empty_cache()
+        return q, t
 
-    def errors(mode):
+    def fused(prec):
+        sv = FusedSolver(dims, per, st)
+        sv.set_precision(prec)
+        return sv
+
+    def ref_ordered(mode):
+        stp = Stepper(dims, per, True, st["umf"], st)
+        stp.num.set_precision(mode)
+        return stp
+
+    dt = FusedSolver(dims, per, st).time_step(q0)
+    f64, t_f64 = run(lambda: fused(0), "fused fp64")
+    f32, t_f32 = run(lambda: fused(1), "fused fp32 intermediates")
+    r64, t_r64 = run(lambda: ref_ordered(0), "per-operator fp64")
+    r32, t_r32 = run(lambda: ref_ordered(1), "per-operator fp32 solves + refinement")
+
+    def errors(a_all, b_all):
         err = []
         for v in range(5):
             sl = slice(v * lmx, (v + 1) * lmx)
-            a, b, c0 = res[mode][sl], res[0][sl], q0[sl]
+            a, b, c0 = a_all[sl], b_all[sl], q0[sl]
             d = a - b
             ch = b - c0
             err.append({"var": ["rho", "rho_u", "rho_v", "rho_w", "E"][v],
@@ -452,19 +464,22 @@ def precision_bench(args, dims=(512, 512, 256), nsteps=50):
                         "max_rel_change": float(d.abs().max() / ch.abs().max()),
                         "l2_rel_change": float(d.norm() / ch.norm())})
         return err
-    e1, e2 = errors(1), errors(2)
+    ef, er = errors(f32, f64), errors(r32, r64)
     line = {
-        "metric": "FP32+refinement vs FP64: max relative error after 50 RK4 steps",
-        "value": max(e["max_rel_state"] for e in e1), "unit": "relative", "n_gpus": 1, "steps": nsteps, "warmup": 1,
-        "ms_per_step": round(1e3 * times[1], 3), "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 solves + f64 refinement",
+        "metric": "FP32 intermediates vs FP64: max relative error after 50 RK4 steps",
+        "value": max(e["max_rel_state"] for e in ef), "unit": "relative", "n_gpus": 1, "steps": nsteps,
+        "warmup": 1, "ms_per_step": round(1e3 * t_f32, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 arithmetic, f32 storage of the viscous intermediates",
         "data": "synthetic (seeded acoustic pulse on uniform Mach 0.5 flow, periodic faces)",
-        "config": {"workload": "viscous RHS, 512x512x256, periodic, 50 RK4 steps with GV10 filter, "
-                               "reference-ordered operators", "dt": dt,
-                   "ms_per_step_fp64": round(1e3 * times[0], 3), "ms_per_step_one_sweep": round(1e3 * times[1], 3),
-                   "ms_per_step_two_sweeps": round(1e3 * times[2], 3),
-                   "two_sweeps_max_rel_state": max(e["max_rel_state"] for e in e2),
-                   "errors_one_sweep": e1, "errors_two_sweeps": e2},
+        "config": {"workload": "viscous RHS, 512x512x256, periodic, 50 RK4 steps with GV10 filter", "dt": dt,
+                   "fused_ms_per_step_fp64": round(1e3 * t_f64, 3),
+                   "fused_ms_per_step_fp32_intermediates": round(1e3 * t_f32, 3),
+                   "fused_speedup": round(t_f64 / t_f32, 4),
+                   "fused_fp32_errors": ef,
+                   "per_operator_ms_per_step_fp64": round(1e3 * t_r64, 3),
+                   "per_operator_ms_per_step_fp32_solves_one_refinement": round(1e3 * t_r32, 3),
+                   "per_operator_fp32_solves_max_rel_state": max(e["max_rel_state"] for e in er),
+                   "per_operator_fp32_solve_errors": er},
     }
     print(json.dumps(line), flush=True)
 

@@ -35,7 +35,7 @@ EXPORTS = [
     "cs_primitives", "cs_calc_fluxes", "cs_calc_viscous_shear_stress", "cs_calc_time_step",
     "cs_calc_time_step_noalloc", "cs_gcbc_go", "cs_wall_condition_update", "cs_rk_stage",
     "cs_set_precision", "cs_deriv_adjoint", "cs_filter_adjoint", "cs_solver_create", "cs_solver_destroy", "cs_solver_rhs", "cs_solver_sweep", "cs_solver_tune", "cs_solver_tune_filter",
-    "cs_solver_set_order", "cs_solver_rhs_vjp", "cs_solver_rk4_vjp", "cs_solver_tape_size",
+    "cs_solver_set_order", "cs_solver_set_precision", "cs_solver_rhs_vjp", "cs_solver_rk4_vjp", "cs_solver_tape_size",
     "cs_solver_rk4_step_tape", "cs_solver_rk4_vjp_tape", "cs_solver_adjoint_mode",
     "cs_batch_create", "cs_batch_destroy", "cs_batch_bind", "cs_batch_deriv", "cs_batch_filter", "cs_batch_step",
     "cs_batch_capture", "cs_batch_replay", "cs_batch_get_persistent", "cs_batch_set_persistent", "cs_batch_info", "cs_batch_autotune", "cs_tune_set", "cs_tune_get",
@@ -142,6 +142,7 @@ def lib():
             "cs_solver_tune": (I, [P, I, I, I, I]),
             "cs_solver_tune_filter": (I, [P, I, I, I, I]),
             "cs_solver_set_order": (I, [P, I]),
+            "cs_solver_set_precision": (I, [P, I]),
             "cs_solver_rhs_vjp": (I, [P, P, P, P, P, P]),
             "cs_solver_rk4_vjp": (I, [P, P, P, D, P, P]),
             "cs_solver_tape_size": (I, [P, ctypes.POINTER(ctypes.c_size_t)]),

@@ -74,12 +74,16 @@ struct cs_solver {
     double* adj = nullptr;     // seed, accumulator (5 fields each), face buffers, internal tape
     double* rec = nullptr;     // faces of the RHS being evaluated go here when set ([3 axes][2][5][face])
     int adj_mode = 1;          // 1: fused transposed sweeps where the lines allow; 0: per-operator products
+    int prec = 0;              // 1: the viscous intermediates (gradients, stress, heat flux) stored in FP32
 };
 
 namespace cs {
 namespace fused {
 int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, const double* q0, int div,
                hipStream_t s);
+// sweep `pos` of a viscous RHS with the intermediates in FP32 (fused_f32.hip)
+int sweep_at_f32(cs_solver* sv, int pos, const SweepCommon& cm, const double* q, double* out, const double* q0,
+                 int div, hipStream_t s);
 int time_step(cs_solver* sv, const double* q, double dt_fixed, hipStream_t s);
 int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, double* out, hipStream_t s);
 int build_axis_table(int n, int seam, int K, int P, std::vector<double>& h);

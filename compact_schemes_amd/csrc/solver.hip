@@ -293,6 +293,12 @@ int cs_solver_tune_filter(cs_solver* sv, int axis, int fields, int lines, int ch
     return CS_OK;
 }
 
+int cs_solver_set_precision(cs_solver* sv, int mode) {
+    if (!sv || (mode != 0 && mode != 1)) return fail(CS_ERR_ARG, "precision mode is 0 or 1");
+    sv->prec = mode;
+    return CS_OK;
+}
+
 int cs_solver_set_order(cs_solver* sv, int order) {
     if (!sv || (order != 0 && order != 1)) return fail(CS_ERR_ARG, "order is 0 or 1");
     sv->order = order;

@@ -31,6 +31,10 @@ SWEEP_BYTES_VISC = [8 * (5 + 4), 8 * (5 + 4), 8 * (5 + 8 + 9), 8 * (5 + 4 + 5), 
                     8 * (5 + 4 + 5 + 5), 8 * (5 + 4 + 5 + 5 + 5), 3 * 8 * (5 + 5)] + [8 * (5 + 5)] * 3
 SWEEP_BYTES_EULER = [0, 0, 0, 8 * (5 + 5), 8 * (5 + 5 + 5), 8 * (5 + 5 + 5), 8 * (5 + 5 + 5 + 5),
                      3 * 8 * (5 + 5)] + [8 * (5 + 5)] * 3
+# viscous intermediates (4 gradients per gradient sweep, stress and heat flux) in FP32
+SWEEP_BYTES_VISC_F32 = [8 * 5 + 4 * 4, 8 * 5 + 4 * 4, 8 * 5 + 4 * 8 + 4 * 9, 8 * (5 + 5) + 4 * 4,
+                        8 * (5 + 5 + 5) + 4 * 4, 8 * (5 + 5 + 5) + 4 * 4, 8 * (5 + 5 + 5 + 5) + 4 * 4,
+                        3 * 8 * (5 + 5)] + [8 * (5 + 5)] * 3
 # the filter sweeps (one per axis) that close an RK4 step
 FILTER_SWEEPS = [8, 9, 10]
 # the sweeps of one RK stage, in launch order
@@ -89,6 +93,7 @@ class FusedSolver:
         fch = [16 if (n % 16 == 0 and 32 <= n <= 256) else 8 for n in self.dims]
         self._ftuned = {0: (5, 16, 8), 1: (1, 16, fch[1]), 2: (1, 16, fch[2])}
         self.order = 1
+        self.precision = 0
         # the filter of cs_solver_rk4_step runs as one sweep per axis when every
         # axis splits into 8-point chunks (else one line-solver launch per axis)
         self.filter_fused = bool(filt) and all(n <= 256 and n % 8 == 0 for n in self.dims)
@@ -121,6 +126,11 @@ class FusedSolver:
         check(lib().cs_solver_tune_filter(self.h, int(axis), int(fields), int(lines), int(chunk)))
         self._ftuned[int(axis)] = (int(fields), int(lines), int(chunk))
 
+    def set_precision(self, mode):
+        """1: viscous intermediates stored in FP32; 0: FP64 (cs_solver_set_precision)."""
+        check(lib().cs_solver_set_precision(self.h, int(mode)))
+        self.precision = int(mode)
+
     def set_order(self, order):
         check(lib().cs_solver_set_order(self.h, int(order)))
         self.order = int(order)
@@ -145,11 +155,12 @@ class FusedSolver:
         exact = n == P * K
         nbc = self.nbc_flat
         gcbc = nbc[ax] == 10 and nbc[3 + ax] == 10
+        f32 = "F" if (self.viscous and self.precision == 1) else ""
         if which < 3:
-            sw, gc = f"cs::fused::S1<{ax}, {str(which == 2).lower()}>", "false"
+            sw, gc = f"cs::fused::S1{f32}<{ax}, {str(which == 2).lower()}>", "false"
         else:
             mode = {3: 0, 4: 1, 5: 2, 6: 3}[which]
-            sw, gc = f"cs::fused::S2<{ax}, {mode}, {str(self.viscous).lower()}>", str(gcbc).lower()
+            sw, gc = f"cs::fused::S2{f32}<{ax}, {mode}, {str(self.viscous).lower()}>", str(gcbc).lower()
         if variant == 1 and K == 8:
             return f"cs::fused::tile_kernel<{K}, {ax}, {lines}, {str(exact).lower()}, {sw}, {gc}>"
         if exact and ax == 0 and K == 8:
@@ -215,7 +226,8 @@ class FusedSolver:
         self.time_step(q, stream=stream)
         self.rhs(q, stream=stream)
         res = []
-        nbytes = SWEEP_BYTES_VISC if self.viscous else SWEEP_BYTES_EULER
+        nbytes = (SWEEP_BYTES_VISC_F32 if self.precision == 1 else SWEEP_BYTES_VISC) if self.viscous \
+            else SWEEP_BYTES_EULER
         for w in (RK_STAGE_SWEEPS if sweeps is None else sweeps):
             if nbytes[w] == 0:
                 continue

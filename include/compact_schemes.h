@@ -269,6 +269,15 @@ int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd);
  * to 256 points, a multiple of 16; the default there).  Results agree to
  * rounding whatever the choice. */
 int cs_solver_tune_filter(cs_solver* sv, int axis, int fields, int lines, int chunk);
+/* storage precision of the viscous intermediates between the sweeps (no
+ * reference counterpart; the reference templates everything on one Type):
+ * 0 = FP64 (default); 1 = the velocity and c^2 gradients of the first two
+ * gradient sweeps, the stress tensor and the heat flux stored in FP32 (588
+ * instead of 736 HBM bytes per point and RHS).  The conserved variables, the
+ * partial RHS, closure and seam rows, characteristic BCs and the RK
+ * accumulation stay FP64, and every sweep computes in FP64.  No effect on
+ * inviscid solvers. */
+int cs_solver_set_precision(cs_solver* sv, int mode);
 /* order of the six sweeps (no reference counterpart): 0 = gradients along
  * x, y, then z with the stress closure, fluxes along x, y, then z with the
  * RHS/stage write; 1 (default) = gradients y, z, x (closure in the x sweep),
