@@ -122,7 +122,9 @@ int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, dou
     const SweepCommon cm = common_of(sv, sbcc);
     if (which < 0 || which > 10) return fail(CS_ERR_ARG, "sweep index out of 0..10");
     if (which >= 7) {
-        if (!filter_fusable(sv)) return fail(CS_ERR_UNSUPPORTED, "the fused filter needs equal 8-point chunks");
+        for (int a = 0; a < 3; a++)
+            if ((which == 7 || which - 8 == a) && !filter_axis_fused(sv, a))
+                return fail(CS_ERR_UNSUPPORTED, "the filter of this axis has no fused sweep");
         if (out != q) CS_HIP(hipMemcpyAsync(out, q, sizeof(double) * 5 * sv->g.lmx, hipMemcpyDeviceToDevice, s));
         return filter_sweeps(sv, out, s, which == 7 ? -1 : which - 8);
     }
@@ -133,9 +135,12 @@ int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, dou
 
 bool filter_fusable(const cs_solver* sv) {
     for (int a = 0; a < 3; a++)
-        if (sv->ax[a].K != 8 || sv->ax[a].N != sv->ax[a].P * 8 || !sv->ax[a].ftab) return false;
+        if (sv->ax[a].K != 8 || sv->ax[a].N != sv->ax[a].P * 8 || sv->ax[a].P > kMaxP || !sv->ax[a].ftab)
+            return false;
     return true;
 }
+
+bool filter_axis_fused(const cs_solver* sv, int a) { return sv->fax[a].ftab != nullptr; }
 
 template <int AX>
 static int filter_axis(cs_solver* sv, double* q, const SweepCommon& cm, hipStream_t s) {

@@ -87,7 +87,10 @@ int sweep_at_f32(cs_solver* sv, int pos, const SweepCommon& cm, const double* q,
 int time_step(cs_solver* sv, const double* q, double dt_fixed, hipStream_t s);
 int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, double* out, hipStream_t s);
 int build_axis_table(int n, int seam, int K, int P, std::vector<double>& h);
+// every axis filters with the 8-point fused sweep of ax (the adjoint's assumption)
 bool filter_fusable(const cs_solver* sv);
+// the filter along axis a runs a fused sweep (fax), else the line solver
+bool filter_axis_fused(const cs_solver* sv, int a);
 int filter_sweeps(cs_solver* sv, double* q, hipStream_t s, int axis = -1);
 int step_filter(cs_solver* sv, double* qa, hipStream_t s);
 // adjoint (vjp.hip: per-operator products; tape.hip: fused transposed sweeps)

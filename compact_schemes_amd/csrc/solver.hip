@@ -53,10 +53,14 @@ int build_axis_table(int n, int seam, int K, int P, std::vector<double>& h) {
 // with the five variables in one launch per axis
 int step_filter(cs_solver* sv, double* qa, hipStream_t s) {
     if (!sv->cfg.filter) return CS_OK;
-    if (filter_fusable(sv)) return filter_sweeps(sv, qa, s);
     for (int a = 0; a < 3; a++) {
-        const int sp = sv->ax[a].seam;
-        const int rc = line_apply(sv->num, OP_FILTER, a, qa, qa, sp, sp, nullptr, 0, false, s, 5);
+        int rc;
+        if (filter_axis_fused(sv, a)) {
+            rc = filter_sweeps(sv, qa, s, a);
+        } else {
+            const int sp = sv->ax[a].seam;
+            rc = line_apply(sv->num, OP_FILTER, a, qa, qa, sp, sp, nullptr, 0, false, s, 5);
+        }
         if (rc) return rc;
     }
     return CS_OK;
@@ -130,8 +134,10 @@ static int filter_chunks(cs_solver* sv, int axis, int chunk) {
     AxisDesc f = ax;
     f.ffields = sv->fax[axis].ffields;
     f.ftl = sv->fax[axis].ftl;
-    if (chunk == 16) {
-        if (axis == 0 || !ax.ftab || ax.N % 16 != 0 || ax.N < 32)
+    if (chunk != ax.K && (ax.K != 8 || axis == 0))
+        return fail(CS_ERR_UNSUPPORTED, "this axis's filter sweep runs " + std::to_string(ax.K) + "-point chunks");
+    if (chunk == 16 && ax.K == 8) {
+        if (!ax.ftab || ax.N % 16 != 0 || ax.N < 32)
             return fail(CS_ERR_UNSUPPORTED, "16-point filter chunks: y or z lines of 32..256 points, a multiple of 16");
         if (!sv->ftabs16[axis]) {
             std::vector<double> hf;
@@ -176,7 +182,10 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
     for (int a = 0; a < 3; a++) {
         AxisDesc& ax = sv->ax[a];
         ax.N = g.n(a);
-        ax.K = ax.N <= 256 ? 8 : 16;
+        // 8-point chunks up to 256 points, and along x up to 512 when they are
+        // equal (one line per wave of the transposing x kernel, 64 chunks);
+        // 16-point chunks otherwise
+        ax.K = (ax.N <= 256 || (a == 0 && ax.N % 8 == 0)) ? 8 : 16;
         ax.P = (ax.N + ax.K - 1) / ax.K;
         ax.GS = 1;
         while (ax.GS < ax.P) ax.GS <<= 1;
@@ -201,11 +210,11 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
         }
         ax.tab = sv->tabs[a];
         ax.ftab = nullptr;
-        if (cfg->filter && ax.K == 8 && ax.N == ax.P * 8) {
-            // the filter of the five variables in one sweep per axis (8-point chunks)
+        if (cfg->filter && ax.N == ax.P * ax.K && (a == 0 ? ax.K == 8 : ax.P <= kMaxP)) {
+            // the filter of the five variables in one sweep per axis (equal chunks)
             std::vector<double> hf;
             const int sp = ax.seam;
-            if ((rc = build_line_table(OP_FILTER, ax.N, sp, sp, cfg->alpha_f, 8, hf))) {
+            if ((rc = build_line_table(OP_FILTER, ax.N, sp, sp, cfg->alpha_f, ax.K, hf))) {
                 cs_solver_destroy(sv);
                 return rc;
             }
@@ -220,7 +229,8 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
     for (int a = 0; a < 3; a++) sv->rax[a] = sv->fax[a] = sv->ax[a];
     // measured default (256^3, MI355X): the y and z filter sweeps run 16-point chunks
     for (int a = 1; a < 3 && cfg->filter; a++)
-        if (sv->ax[a].ftab && sv->ax[a].N % 16 == 0 && sv->ax[a].N >= 32 && (rc = filter_chunks(sv, a, 16))) {
+        if (sv->ax[a].ftab && sv->ax[a].K == 8 && sv->ax[a].N % 16 == 0 && sv->ax[a].N >= 32 &&
+            (rc = filter_chunks(sv, a, 16))) {
             std::string msg = cs_last_error();
             cs_solver_destroy(sv);
             return fail(rc, msg);
@@ -271,7 +281,8 @@ int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd) {
     if (!sv || axis < 0 || axis > 2) return fail(CS_ERR_ARG, "bad solver or axis");
     if (variant < 0 || variant > 2 || (lines != 8 && lines != 16))
         return fail(CS_ERR_ARG, "variant 0/1/2, lines 8/16");
-    if (variant == 1 && sv->ax[axis].K != 8) return fail(CS_ERR_UNSUPPORTED, "tiled variant needs 8-point chunks");
+    if (variant == 1 && (sv->ax[axis].K != 8 || sv->ax[axis].P > kMaxP))
+        return fail(CS_ERR_UNSUPPORTED, "tiled variant needs 8-point chunks, at most 256 points");
     if (variant == 2) return use_chunks16(sv, axis, lines, xcd);
     AxisDesc& ax = sv->ax[axis];
     ax.variant = variant;
@@ -285,7 +296,7 @@ int cs_solver_tune_filter(cs_solver* sv, int axis, int fields, int lines, int ch
     if (!sv || axis < 0 || axis > 2) return fail(CS_ERR_ARG, "bad solver or axis");
     if ((fields != 1 && fields != 5) || (lines != 8 && lines != 16) || (chunk != 8 && chunk != 16))
         return fail(CS_ERR_ARG, "fields 1/5, lines 8/16, chunk 8/16");
-    if (!filter_fusable(sv)) return fail(CS_ERR_UNSUPPORTED, "the fused filter needs equal 8-point chunks");
+    if (!filter_axis_fused(sv, axis)) return fail(CS_ERR_UNSUPPORTED, "the filter of this axis has no fused sweep");
     int rc = filter_chunks(sv, axis, chunk);
     if (rc) return rc;
     sv->ax[axis].ffields = sv->fax[axis].ffields = fields;

@@ -746,7 +746,7 @@ int step_recording(cs_solver* sv, double* qa, const double* sbcc, double dt, dou
 bool adjoint_fused(const cs_solver* sv) {
     if (sv->adj_mode != 1 || sv->cfg.viscous) return false;
     for (int a = 0; a < 3; a++)
-        if (sv->ax[a].K != 8 || sv->ax[a].N != sv->ax[a].P * 8) return false;
+        if (sv->ax[a].K != 8 || sv->ax[a].N != sv->ax[a].P * 8 || sv->ax[a].P > kMaxP) return false;
     return !sv->cfg.filter || filter_fusable(sv);
 }
 

@@ -90,13 +90,17 @@ class FusedSolver:
         self._tuned = {0: (0, 16, 0), 1: (0, 16, 1), 2: (2 if (nz % 16 == 0 and 32 <= nz <= 256) else 0, 16, 0)}
         # filter sweeps: (fields per workgroup, lines, points per chunk); y/z
         # lines of 32..256 points, a multiple of 16, run 16-point chunks
-        fch = [16 if (n % 16 == 0 and 32 <= n <= 256) else 8 for n in self.dims]
+        fch = [16 if (n % 16 == 0 and 32 <= n <= 512) else 8 for n in self.dims]
         self._ftuned = {0: (5, 16, 8), 1: (1, 16, fch[1]), 2: (1, 16, fch[2])}
         self.order = 1
         self.precision = 0
-        # the filter of cs_solver_rk4_step runs as one sweep per axis when every
-        # axis splits into 8-point chunks (else one line-solver launch per axis)
-        self.filter_fused = bool(filt) and all(n <= 256 and n % 8 == 0 for n in self.dims)
+        # the filter of cs_solver_rk4_step runs one fused sweep per axis where the
+        # axis splits into equal chunks (x: 8-point chunks up to 512 points; y/z:
+        # 8 or 16 points up to 256, 16 up to 512), else a line-solver launch
+        self.filter_axes = [bool(filt) and ((n % 8 == 0 and n <= 256) or (a == 0 and n % 8 == 0 and n <= 512) or
+                                            (a > 0 and n % 16 == 0 and n <= 512))
+                            for a, n in enumerate(self.dims)]
+        self.filter_fused = all(self.filter_axes)
 
     def close(self):
         if getattr(self, "h", None):
@@ -150,7 +154,7 @@ class FusedSolver:
         ax = S1_AXES[self.order][which] if which < 3 else S2_AXES[self.order][min(which, 5) - 3]
         n = self.dims[ax]
         variant, lines, _ = self._tuned[ax]
-        K = 8 if n <= 256 and variant != 2 else 16
+        K = 8 if (n <= 256 or (ax == 0 and n % 8 == 0)) and variant != 2 else 16
         P = (n + K - 1) // K
         exact = n == P * K
         nbc = self.nbc_flat

@@ -1,9 +1,11 @@
 """GPU parity of the fused solver on axes longer than 256 points.
 
-Axes of 257-512 points run the 16-point-chunk kernels: the exact form
-(N = 16 P) and the ragged form (any N), the K = 16 characteristic face path
-and the line-solver filter of an RK4 step.  One long axis at a time, on x,
-y and z in turn, with the other two axes short so the oracle stays fast.
+x lines of 257-512 points that split into 8-point chunks run the transposing
+x kernel with one line per wave (up to 64 chunks) and the fused x filter;
+other long axes run the 16-point-chunk kernels: the exact form (N = 16 P)
+with the fused y/z filter, and the ragged form (any N) with the line-solver
+filter of an RK4 step.  One long axis at a time, on x, y and z in turn, with
+the other two axes short so the oracle stays fast.
 Bars as for the other fused tests: 1e-12 norm-wise on the RHS, 1e-10 on the
 change of the state over one RK4 step.
 """
@@ -16,8 +18,10 @@ CL = [[10, 10, 10], [10, 10, 10]]
 PER = [[45, 45, 45], [45, 45, 45]]
 MIX = [[45, 10, 10], [45, 10, 10]]
 
-LONG = [(272, 16, 16), (16, 272, 16), (16, 16, 272),      # exact: 17 chunks of 16
-        (264, 20, 16), (20, 264, 16), (16, 20, 264)]      # ragged
+LONG = [(272, 16, 16), (16, 272, 16), (16, 16, 272),      # x: 34 chunks of 8; y, z: 17 chunks of 16
+        (264, 20, 16), (20, 264, 16), (16, 20, 264),      # x: 33 chunks of 8; y, z: ragged
+        (512, 16, 16), (16, 512, 16), (16, 16, 512),      # x: 64 chunks of 8; y, z: 32 chunks of 16
+        (270, 20, 16)]                                    # x: ragged 16-point chunks
 
 
 def _state(dims, seed, noise=1e-2, width=3.0):
