@@ -29,6 +29,8 @@ using namespace cs;
 
 namespace {
 
+constexpr int kLanes = 8;      // streams of a grid-mode-2 step
+
 struct Launch {                // one kernel variant (K, axis) and chunk count P of a phase
     int K = 8, axis = 0, P = 1;
     int first = 0, nitems = 0;  // range in the phase's item list
@@ -58,12 +60,37 @@ struct cs_batch {
                                 // 2: one launch per subdomain and operator (line_kernel)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // grid mode 2, whole steps: the subdomains dealt over independent streams
+    // (default priority) so that the small launches of different subdomains
+    // overlap; fork/join through events (nothing allocates in a capture)
+    std::vector<hipStream_t> lanes;
+    std::vector<hipEvent_t> lane_ev;   // [0]: fork, [1 + l]: join of lane l
 };
 
 static void free_phase(Phase& p) {
     if (p.args) (void)hipFree(p.args);
     if (p.items) (void)hipFree(p.items);
     p = Phase();
+}
+
+// streams and events of the lanes, created once outside any capture
+static int make_lanes(cs_batch* b) {
+    if (!b->lanes.empty()) return CS_OK;
+    const int n = std::min(kLanes, b->count);
+    b->lanes.assign(n, nullptr);
+    b->lane_ev.assign(n + 1, nullptr);
+    for (int l = 0; l < n; l++) CS_HIP(hipStreamCreateWithFlags(&b->lanes[l], hipStreamNonBlocking));
+    for (int e = 0; e <= n; e++) CS_HIP(hipEventCreateWithFlags(&b->lane_ev[e], hipEventDisableTiming));
+    return CS_OK;
+}
+
+static void drop_lanes(cs_batch* b) {
+    for (hipStream_t l : b->lanes)
+        if (l) (void)hipStreamDestroy(l);
+    for (hipEvent_t e : b->lane_ev)
+        if (e) (void)hipEventDestroy(e);
+    b->lanes.clear();
+    b->lane_ev.clear();
 }
 
 static void drop_graph(cs_batch* b) {
@@ -187,6 +214,34 @@ static int run_direct(cs_batch* b, int k, hipStream_t s) {
     return CS_OK;
 }
 
+// a whole step in grid mode 2: subdomain sd's derivatives and then its
+// filter run in order on lane sd % lanes; the lanes fork from and join back
+// into s.  Each subdomain sees the same launches as run_direct, in the same
+// order, so the results are the same bit for bit.
+static int run_direct_lanes(cs_batch* b, hipStream_t s) {
+    int rc = make_lanes(b);
+    if (rc) return rc;
+    const int n = (int)b->lanes.size();
+    CS_HIP(hipEventRecord(b->lane_ev[0], s));
+    for (int l = 0; l < n; l++) CS_HIP(hipStreamWaitEvent(b->lanes[l], b->lane_ev[0], 0));
+    for (int sd = 0; sd < b->count && !rc; sd++) {
+        cs_numerics* num = b->num[sd];
+        hipStream_t ls = b->lanes[sd % n];
+        for (int axis = 0; axis < 3 && !rc; axis++)
+            rc = line_apply(num, OP_DERIV, axis, b->f[sd], b->d[3 * sd + axis], num->ndf[0][axis],
+                            num->ndf[1][axis], nullptr, 0, false, ls);
+        for (int axis = 0; axis < 3 && !rc; axis++)
+            rc = line_apply(num, OP_FILTER, axis, b->f[sd], b->f[sd], num->ndf[0][axis], num->ndf[1][axis],
+                            nullptr, 0, false, ls);
+    }
+    // join every lane, also after an error, so that a capture can end cleanly
+    for (int l = 0; l < n; l++) {
+        CS_HIP(hipEventRecord(b->lane_ev[1 + l], b->lanes[l]));
+        CS_HIP(hipStreamWaitEvent(s, b->lane_ev[1 + l], 0));
+    }
+    return rc;
+}
+
 // the operator tables mode 2 launches with, built now so that nothing
 // allocates while a graph is being captured
 static int warm_tables(cs_batch* b) {
@@ -217,6 +272,7 @@ extern "C" {
 int cs_batch_destroy(cs_batch* b) {
     if (!b) return CS_OK;
     drop_graph(b);
+    drop_lanes(b);
     for (Phase& p : b->ph) free_phase(p);
     for (cs_numerics* n : b->num) cs_numerics_destroy(n);
     delete b;
@@ -279,6 +335,7 @@ int cs_batch_filter(cs_batch* b, cs_stream stream) {
 }
 
 int cs_batch_step(cs_batch* b, cs_stream stream) {
+    if (b && b->bound && b->persistent == 2) return run_direct_lanes(b, (hipStream_t)stream);
     int rc = cs_batch_deriv(b, stream);
     if (rc) return rc;
     return cs_batch_filter(b, stream);
@@ -288,7 +345,8 @@ int cs_batch_set_persistent(cs_batch* b, int persistent) {
     if (!b) return fail(CS_ERR_ARG, "null batch");
     if (persistent < 0 || persistent > 2) return fail(CS_ERR_ARG, "grid mode is 0, 1 or 2");
     if (persistent == 2) {
-        const int rc = warm_tables(b);
+        int rc = warm_tables(b);
+        if (!rc) rc = make_lanes(b);
         if (rc) return rc;
     }
     b->persistent = persistent;

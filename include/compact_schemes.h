@@ -348,7 +348,10 @@ int cs_batch_step(cs_batch* b, cs_stream stream);       /* deriv, then filter */
  * the hardware dispatcher balances; 1 = persistent, as many workgroups as
  * fit on the device at once, items dealt round-robin; 2 = one launch per
  * subdomain and operator of the single-subdomain kernels (cs_deriv /
- * cs_filter), cheap to launch once captured.  Same results. */
+ * cs_filter), cheap to launch once captured; a whole step (cs_batch_step,
+ * cs_batch_capture) deals the subdomains over eight streams that fork from
+ * and join back into `stream`, so that small launches overlap.  Same
+ * results. */
 int cs_batch_set_persistent(cs_batch* b, int persistent);
 /* the grid mode in use (set, or picked by cs_batch_autotune) */
 int cs_batch_get_persistent(const cs_batch* b, int* mode);
