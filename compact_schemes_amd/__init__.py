@@ -35,11 +35,12 @@ EXPORTS = [
     "cs_primitives", "cs_calc_fluxes", "cs_calc_viscous_shear_stress", "cs_calc_time_step",
     "cs_calc_time_step_noalloc", "cs_gcbc_go", "cs_wall_condition_update", "cs_rk_stage",
     "cs_set_precision", "cs_deriv_adjoint", "cs_filter_adjoint", "cs_solver_create", "cs_solver_destroy", "cs_solver_rhs", "cs_solver_sweep", "cs_solver_tune", "cs_solver_tune_filter",
+    "cs_solver_get_tune", "cs_solver_autotune",
     "cs_solver_set_order", "cs_solver_set_precision", "cs_solver_rhs_vjp", "cs_solver_rk4_vjp", "cs_solver_tape_size",
     "cs_solver_rk4_step_tape", "cs_solver_rk4_vjp_tape", "cs_solver_adjoint_mode",
     "cs_batch_create", "cs_batch_destroy", "cs_batch_bind", "cs_batch_deriv", "cs_batch_filter", "cs_batch_step",
     "cs_batch_capture", "cs_batch_replay", "cs_batch_get_persistent", "cs_batch_set_persistent", "cs_batch_info", "cs_batch_autotune", "cs_tune_set", "cs_tune_get",
-    "cs_tune_clear", "cs_tune_save", "cs_tune_load",
+    "cs_tune_set_sweep", "cs_tune_get_sweep", "cs_tune_set_filter_sweep", "cs_tune_get_filter_sweep", "cs_tune_clear", "cs_tune_save", "cs_tune_load",
     "cs_solver_time_step", "cs_solver_rk4_step",
     "cs_init_pulse", "cs_boundary_weights", "cs_config_defaults", "cs_config_load", "cs_config_dims", "cs_config_grid", "cs_vtk_write",
 ]
@@ -141,6 +142,8 @@ def lib():
             "cs_solver_sweep": (I, [P, I, P, P, P, P]),
             "cs_solver_tune": (I, [P, I, I, I, I]),
             "cs_solver_tune_filter": (I, [P, I, I, I, I]),
+            "cs_solver_get_tune": (I, [P, I, P]),
+            "cs_solver_autotune": (I, [P, P, P, I, P]),
             "cs_solver_set_order": (I, [P, I]),
             "cs_solver_set_precision": (I, [P, I]),
             "cs_solver_rhs_vjp": (I, [P, P, P, P, P, P]),
@@ -161,8 +164,12 @@ def lib():
             "cs_batch_replay": (I, [P, P]),
             "cs_batch_info": (I, [P, P, P]),
             "cs_batch_autotune": (I, [P, I, P]),
-            "cs_tune_set": (I, [I, I, I]),
-            "cs_tune_get": (I, [I, I, P]),
+            "cs_tune_set": (I, [I, I, I, I, I, I]),
+            "cs_tune_get": (I, [I, I, I, I, I, P]),
+            "cs_tune_set_sweep": (I, [I, I, I, I, I, I, I]),
+            "cs_tune_get_sweep": (I, [I, I, I, I, P, P]),
+            "cs_tune_set_filter_sweep": (I, [I, I, I, I, I, I]),
+            "cs_tune_get_filter_sweep": (I, [I, I, I, P, P]),
             "cs_tune_clear": (I, []),
             "cs_tune_save": (I, [ctypes.c_char_p]),
             "cs_tune_load": (I, [ctypes.c_char_p]),

@@ -3,7 +3,7 @@ and the compact filter along x, y, z of many independent subdomains in one
 launch per phase; see include/compact_schemes.h."""
 import ctypes
 
-from . import Dims, _p, _stream, check, lib
+from . import Dims, _p, _stream, check, lib, tune
 
 OP_DERIV, OP_FILTER = 0, 1
 
@@ -81,23 +81,15 @@ class Batch:
         check(lib().cs_batch_autotune(self.h, int(reps), _stream(stream)))
 
 
-def tune_set(op, n, K):
-    check(lib().cs_tune_set(int(op), int(n), int(K)))
+def tune_set(op, n, K, start=0, end=0, precision=0):
+    """chunk length K of the line kernels (see compact_schemes_amd.tune)"""
+    tune.set_line(op, n, K, start, end, precision)
 
 
-def tune_get(op, n):
-    k = ctypes.c_int()
-    check(lib().cs_tune_get(int(op), int(n), ctypes.byref(k)))
-    return k.value
+def tune_get(op, n, start=0, end=0, precision=0):
+    return tune.get_line(op, n, start, end, precision)
 
 
-def tune_clear():
-    check(lib().cs_tune_clear())
-
-
-def tune_save(path):
-    check(lib().cs_tune_save(str(path).encode()))
-
-
-def tune_load(path):
-    check(lib().cs_tune_load(str(path).encode()))
+tune_clear = tune.clear
+tune_save = tune.save
+tune_load = tune.load

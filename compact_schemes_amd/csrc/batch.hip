@@ -422,15 +422,18 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
     if (!b || !b->bound) return fail(CS_ERR_STATE, "batch has no bound fields");
     if (reps < 1) reps = 3;
     hipStream_t s = (hipStream_t)stream;
-    struct Case { int op, n, s, axis; };
+    // one case per (operator, line length, start/end rows, precision)
+    struct Case { int op, n, start, end, prec, s, axis; };
     std::vector<Case> cases;
     for (int sd = 0; sd < b->count; sd++)
         for (int axis = 0; axis < 3; axis++)
             for (int op : {OP_DERIV, OP_FILTER}) {
-                const int n = b->num[sd]->g.n(axis);
+                const cs_numerics* num = b->num[sd];
+                const Case c{op, num->g.n(axis), num->ndf[0][axis], num->ndf[1][axis], num->precision, sd, axis};
                 bool seen = false;
-                for (auto& c : cases) seen |= (c.op == op && c.n == n);
-                if (!seen) cases.push_back({op, n, sd, axis});
+                for (auto& o : cases)
+                    seen |= (o.op == c.op && o.n == c.n && o.start == c.start && o.end == c.end && o.prec == c.prec);
+                if (!seen) cases.push_back(c);
             }
     int rc = CS_OK;
     {
@@ -439,7 +442,7 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
     hipEvent_t e0 = ev.e0, e1 = ev.e1;
     for (auto& c : cases) {
         cs_numerics* num = b->num[c.s];
-        const int start = num->ndf[0][c.axis], end = num->ndf[1][c.axis];
+        const int start = c.start, end = c.end;
         float best = 1e30f;
         int bestK = 0;
         for (int K : {8, 16, 32}) {
@@ -465,7 +468,7 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
             if (ms < best) { best = ms; bestK = K; }
         }
         if (rc) break;
-        set_tuned_chunk(c.op, c.n, bestK);
+        set_tuned_chunk(c.op, c.n, c.start, c.end, c.prec, bestK);
     }
     }
     if (rc) return rc;
@@ -499,51 +502,5 @@ int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream) {
     return best == 2 ? warm_tables(b) : CS_OK;
 }
 
-int cs_tune_set(int op, int n, int K) {
-    if ((op != OP_DERIV && op != OP_FILTER) || n < 8 || (K != 0 && K != 8 && K != 16 && K != 32))
-        return fail(CS_ERR_ARG, "op 0/1, n >= 8, K in {0, 8, 16, 32}");
-    if (K != 0 && (n + K - 1) / K > max_chunks(K)) return fail(CS_ERR_SHAPE, "chunk length too short for this line");
-    set_tuned_chunk(op, n, K);
-    return CS_OK;
-}
-
-int cs_tune_get(int op, int n, int* K) {
-    if (!K) return fail(CS_ERR_ARG, "null argument");
-    *K = tuned_chunk(op, n);
-    return CS_OK;
-}
-
-int cs_tune_clear(void) {
-    clear_tuned_chunks();
-    return CS_OK;
-}
-
-// text file, one line per choice: "<op> <line length> <chunk length>"
-// (op 0 = derivative, 1 = filter; FP64)
-int cs_tune_save(const char* path) {
-    if (!path) return fail(CS_ERR_ARG, "null path");
-    FILE* fp = std::fopen(path, "w");
-    if (!fp) return fail(CS_ERR_ARG, std::string("cannot write ") + path);
-    std::fprintf(fp, "# compact_schemes_amd chunk-length choices: op length K (op 0 deriv, 1 filter; fp64)\n");
-    for (auto& e : tuned_chunks()) std::fprintf(fp, "%d %d %d\n", e[0], e[1], e[2]);
-    std::fclose(fp);
-    return CS_OK;
-}
-
-int cs_tune_load(const char* path) {
-    if (!path) return fail(CS_ERR_ARG, "null path");
-    FILE* fp = std::fopen(path, "r");
-    if (!fp) return fail(CS_ERR_ARG, std::string("cannot read ") + path);
-    char line[256];
-    int rc = CS_OK;
-    while (std::fgets(line, sizeof line, fp)) {
-        if (line[0] == '#') continue;
-        int op, n, K;
-        if (std::sscanf(line, "%d %d %d", &op, &n, &K) != 3) continue;
-        if ((rc = cs_tune_set(op, n, K))) break;
-    }
-    std::fclose(fp);
-    return rc;
-}
 
 }  // extern "C"

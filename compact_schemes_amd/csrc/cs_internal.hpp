@@ -160,12 +160,17 @@ int batch_blocks_per_cu(int op, int K, int axis, int P, int threads, size_t lds,
 int launch_batch_lines(int op, int K, int axis, int P, const LineArgs* args, const int2* items, int nitems,
                        int grid, int threads, size_t lds, hipStream_t s);
 
-// chunk-length choices per (operator, line length) made by the autotuner;
-// 0 = none (the default rule of choose_chunk applies)
-int tuned_chunk(int op, int n);
-void set_tuned_chunk(int op, int n, int K);
-void clear_tuned_chunks();
-std::vector<std::array<int, 3>> tuned_chunks();
+// launch-shape choices (tune.hip): chunk length per (operator, line length,
+// start/end rows, precision), 0 = none (the default rule of choose_chunk);
+// fused right-hand-side sweep shapes per (axis, line length, seam axis,
+// precision of the intermediates), fused filter shapes per (axis, line
+// length, seam axis)
+int tuned_chunk(int op, int n, int start, int end, int prec);
+void set_tuned_chunk(int op, int n, int start, int end, int prec, int K);
+bool tuned_sweep(int axis, int n, int seam, int prec, int* variant_lines_xcd);
+void set_tuned_sweep(int axis, int n, int seam, int prec, const int* variant_lines_xcd);
+bool tuned_filter(int axis, int n, int seam, int* fields_lines_chunk);
+void set_tuned_filter(int axis, int n, int seam, const int* fields_lines_chunk);
 
 // ---- contexts -----------------------------------------------------------------
 }  // namespace cs

@@ -79,6 +79,11 @@ struct cs_solver {
 
 namespace cs {
 namespace fused {
+// axes of the S1 sweeps (the third one closes) and of the S2 sweeps (the
+// third one finishes the RHS) for each sweep order
+constexpr int kS1Axes[2][3] = {{0, 1, 2}, {1, 2, 0}};
+constexpr int kS2Axes[2][3] = {{0, 1, 2}, {2, 1, 0}};
+
 int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, const double* q0, int div,
                hipStream_t s);
 // sweep `pos` of a viscous RHS with the intermediates in FP32 (fused_f32.hip)

@@ -1219,11 +1219,6 @@ static int dispatch(const AxisDesc& ax, const SweepCommon& cm, const SW& sw, hip
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
-// axes of the S1 sweeps (the third one closes) and of the S2 sweeps (the
-// third one finishes the RHS) for each sweep order
-constexpr int kS1Axes[2][3] = {{0, 1, 2}, {1, 2, 0}};
-constexpr int kS2Axes[2][3] = {{0, 1, 2}, {2, 1, 0}};
-
 // the solver's FP64 buffers of the viscous intermediates hold FP32 fields of
 // the same point count when the intermediates are stored in FP32
 template <class TS>

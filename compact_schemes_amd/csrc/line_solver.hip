@@ -705,36 +705,13 @@ int launch_batch_lines(int op, int K, int axis, int P, const LineArgs* args, con
 // ---------------------------------------------------------------------------
 // context helpers
 // ---------------------------------------------------------------------------
-static std::mutex g_tune_lock;
-static std::map<std::pair<int, int>, int> g_tuned;
-
-int tuned_chunk(int op, int n) {
-    std::lock_guard<std::mutex> g(g_tune_lock);
-    auto it = g_tuned.find({op, n});
-    return it == g_tuned.end() ? 0 : it->second;
-}
-void set_tuned_chunk(int op, int n, int K) {
-    std::lock_guard<std::mutex> g(g_tune_lock);
-    g_tuned[{op, n}] = K;
-}
-void clear_tuned_chunks() {
-    std::lock_guard<std::mutex> g(g_tune_lock);
-    g_tuned.clear();
-}
-std::vector<std::array<int, 3>> tuned_chunks() {
-    std::lock_guard<std::mutex> g(g_tune_lock);
-    std::vector<std::array<int, 3>> v;
-    for (auto& e : g_tuned) v.push_back({e.first.first, e.first.second, e.second});
-    return v;
-}
-
 // Device table of one line operator with chunk length K (0 = the tuned or
 // default choice).
 int make_line_op(const cs_numerics* num, int op, int n, int start, int end, int K, LineOpTable* t) {
     const int minlen = (op == OP_FILTER && start == 1) ? 11 : 8;
     if (n < minlen) return fail(CS_ERR_SHAPE, "line length " + std::to_string(n) + " below the stencil size");
     t->n = n;
-    if (K == 0) K = tuned_chunk(op, n);
+    if (K == 0 && (op == OP_DERIV || op == OP_FILTER)) K = tuned_chunk(op, n, start, end, num->precision);
     if (K != 0) {
         if (K != 8 && K != 16 && K != 32) return fail(CS_ERR_ARG, "chunk length must be 8, 16 or 32");
         t->K = K;

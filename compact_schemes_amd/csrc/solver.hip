@@ -83,6 +83,31 @@ static int axis_mode(const int* nbc, int axis, int* seam, int* gcbc) {
                                     "(walls and curved interfaces use the per-operator API)");
 }
 
+namespace {
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+// mean time of one call of `fn` over `reps` calls after one untimed call
+template <class F>
+int time_calls(F&& fn, int reps, hipStream_t s, EventPair& ev, float* ms) {
+    int rc = fn();
+    if (rc) return rc;
+    CS_HIP(hipEventRecord(ev.a, s));
+    for (int i = 0; i < reps; i++)
+        if ((rc = fn())) return rc;
+    CS_HIP(hipEventRecord(ev.b, s));
+    CS_HIP(hipEventSynchronize(ev.b));
+    CS_HIP(hipEventElapsedTime(ms, ev.a, ev.b));
+    *ms /= reps;
+    return CS_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int cs_solver_destroy(cs_solver* sv) {
@@ -155,6 +180,8 @@ static int filter_chunks(cs_solver* sv, int axis, int chunk) {
     sv->fax[axis] = f;
     return CS_OK;
 }
+
+static void apply_tuned(cs_solver* sv, bool filters);
 
 int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
     if (!cfg || !out) return fail(CS_ERR_ARG, "null argument");
@@ -259,6 +286,7 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
         cs_solver_destroy(sv);
         return fail(CS_ERR_HIP, "allocation of solver buffers failed");
     }
+    apply_tuned(sv, true);
     *out = sv;
     return CS_OK;
 }
@@ -277,8 +305,8 @@ int cs_solver_sweep(cs_solver* sv, int which, const double* qa, double* de, cons
     return one_sweep(sv, which, qa, sbcc, de, (hipStream_t)stream);
 }
 
-int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd) {
-    if (!sv || axis < 0 || axis > 2) return fail(CS_ERR_ARG, "bad solver or axis");
+// launch shape of the right-hand-side sweeps along `axis`
+static int sweep_shape(cs_solver* sv, int axis, int variant, int lines, int xcd) {
     if (variant < 0 || variant > 2 || (lines != 8 && lines != 16))
         return fail(CS_ERR_ARG, "variant 0/1/2, lines 8/16");
     if (variant == 1 && (sv->ax[axis].K != 8 || sv->ax[axis].P > kMaxP))
@@ -292,8 +320,8 @@ int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd) {
     return CS_OK;
 }
 
-int cs_solver_tune_filter(cs_solver* sv, int axis, int fields, int lines, int chunk) {
-    if (!sv || axis < 0 || axis > 2) return fail(CS_ERR_ARG, "bad solver or axis");
+// launch shape of the fused filter sweep along `axis`
+static int filter_shape(cs_solver* sv, int axis, int fields, int lines, int chunk) {
     if ((fields != 1 && fields != 5) || (lines != 8 && lines != 16) || (chunk != 8 && chunk != 16))
         return fail(CS_ERR_ARG, "fields 1/5, lines 8/16, chunk 8/16");
     if (!filter_axis_fused(sv, axis)) return fail(CS_ERR_UNSUPPORTED, "the filter of this axis has no fused sweep");
@@ -304,9 +332,126 @@ int cs_solver_tune_filter(cs_solver* sv, int axis, int fields, int lines, int ch
     return CS_OK;
 }
 
+// applies the process-wide launch-shape choices for this solver's axes
+// (sweeps at the current precision; filters when `filters`); a choice that
+// this solver cannot run (a cache written for another shape) is skipped
+static void apply_tuned(cs_solver* sv, bool filters) {
+    for (int a = 0; a < 3; a++) {
+        const AxisDesc& ax = sv->ax[a];
+        int v[3];
+        if (tuned_sweep(a, ax.N, ax.seam, sv->cfg.viscous ? sv->prec : 0, v)) (void)sweep_shape(sv, a, v[0], v[1], v[2]);
+        if (filters && filter_axis_fused(sv, a) && tuned_filter(a, ax.N, ax.seam, v))
+            (void)filter_shape(sv, a, v[0], v[1], v[2]);
+    }
+}
+
+int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd) {
+    if (!sv || axis < 0 || axis > 2) return fail(CS_ERR_ARG, "bad solver or axis");
+    return sweep_shape(sv, axis, variant, lines, xcd);
+}
+
+int cs_solver_tune_filter(cs_solver* sv, int axis, int fields, int lines, int chunk) {
+    if (!sv || axis < 0 || axis > 2) return fail(CS_ERR_ARG, "bad solver or axis");
+    return filter_shape(sv, axis, fields, lines, chunk);
+}
+
+int cs_solver_get_tune(const cs_solver* sv, int axis, int* shape6) {
+    if (!sv || axis < 0 || axis > 2 || !shape6) return fail(CS_ERR_ARG, "bad solver, axis or output");
+    const AxisDesc& r = sv->rax[axis];
+    const AxisDesc& f = sv->fax[axis];
+    const bool fused = filter_axis_fused(sv, axis);
+    const int v[6] = {r.variant, r.tl, r.xcd, fused ? f.ffields : 0, fused ? f.ftl : 0, fused ? f.K : 0};
+    for (int i = 0; i < 6; i++) shape6[i] = v[i];
+    return CS_OK;
+}
+
+
+int cs_solver_autotune(cs_solver* sv, const double* qa, const double* sbcc, int reps, cs_stream stream) {
+    if (!sv || !qa) return fail(CS_ERR_ARG, "null argument");
+    if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
+        return fail(CS_ERR_ARG, "non-reflective faces need sbcc");
+    if (reps < 1 || reps > 1000) return fail(CS_ERR_ARG, "reps in 1..1000");
+    hipStream_t s = (hipStream_t)stream;
+    EventPair ev;
+    CS_HIP(hipEventCreate(&ev.a));
+    CS_HIP(hipEventCreate(&ev.b));
+    double* out = sv->qb;   // stage buffer, free outside cs_solver_rk4_step
+    const int prec = sv->cfg.viscous ? sv->prec : 0;
+    // one whole RHS so that every intermediate a single sweep reads is written
+    int rc = rhs_sweeps(sv, qa, sbcc, out, nullptr, 0, s);
+    if (rc) return rc;
+    // a candidate replaces the shape in use only when it is 2% faster, so
+    // that timing noise does not flip between equal shapes
+    constexpr float kGain = 0.98f;
+    for (int a = 0; a < 3; a++) {
+        // the gradient and flux sweeps of this axis, by launch position
+        int pos[2], np = 0;
+        for (int k = 0; k < 3; k++) {
+            if (sv->cfg.viscous && kS1Axes[sv->order][k] == a) pos[np++] = k;
+            if (kS2Axes[sv->order][k] == a) pos[np++] = 3 + k;
+        }
+        auto sweeps = [&]() {
+            for (int i = 0; i < np; i++) {
+                const int r = one_sweep(sv, pos[i], qa, sbcc, out, s);
+                if (r) return r;
+            }
+            return (int)CS_OK;
+        };
+        const AxisDesc cur = sv->rax[a], ax0 = sv->ax[a];
+        int best[3] = {cur.variant, cur.tl, cur.xcd};
+        float tbest;
+        if ((rc = time_calls(sweeps, reps, s, ev, &tbest))) return rc;
+        for (int v = 0; v < 3; v++)
+            for (int l : {8, 16})
+                for (int x = 0; x < 2; x++) {
+                    // x lines: the transposing kernel ignores lines and XCD
+                    // grouping, the tiled one takes lines only
+                    if (a == 0 && (x == 1 || (v == 0 && l == 8))) continue;
+                    if (v == best[0] && l == best[1] && x == best[2]) continue;
+                    if (sweep_shape(sv, a, v, l, x)) continue;   // not admissible on this axis
+                    float t;
+                    if ((rc = time_calls(sweeps, reps, s, ev, &t))) return rc;
+                    if (t < kGain * tbest) {
+                        tbest = t;
+                        best[0] = v, best[1] = l, best[2] = x;
+                    }
+                }
+        sv->ax[a] = ax0;
+        if ((rc = sweep_shape(sv, a, best[0], best[1], best[2]))) return rc;
+        set_tuned_sweep(a, sv->ax[a].N, sv->ax[a].seam, prec, best);
+    }
+    if (!sv->cfg.filter) return CS_OK;
+    CS_HIP(hipMemcpyAsync(out, qa, sizeof(double) * 5 * sv->g.lmx, hipMemcpyDeviceToDevice, s));
+    for (int a = 0; a < 3; a++) {
+        if (!filter_axis_fused(sv, a)) continue;
+        auto filt = [&]() { return filter_sweeps(sv, out, s, a); };
+        const AxisDesc cur = sv->fax[a];
+        int best[3] = {cur.ffields, cur.ftl, cur.K};
+        float tbest;
+        if ((rc = time_calls(filt, reps, s, ev, &tbest))) return rc;
+        for (int f : {1, 5})
+            for (int l : {8, 16})
+                for (int c : {8, 16}) {
+                    if (a == 0 && l == 8) continue;   // the x filter ignores lines
+                    if (f == best[0] && l == best[1] && c == best[2]) continue;
+                    if (filter_shape(sv, a, f, l, c)) continue;
+                    float t;
+                    if ((rc = time_calls(filt, reps, s, ev, &t))) return rc;
+                    if (t < kGain * tbest) {
+                        tbest = t;
+                        best[0] = f, best[1] = l, best[2] = c;
+                    }
+                }
+        if ((rc = filter_shape(sv, a, best[0], best[1], best[2]))) return rc;
+        set_tuned_filter(a, sv->ax[a].N, sv->ax[a].seam, best);
+    }
+    return CS_OK;
+}
+
 int cs_solver_set_precision(cs_solver* sv, int mode) {
     if (!sv || (mode != 0 && mode != 1)) return fail(CS_ERR_ARG, "precision mode is 0 or 1");
     sv->prec = mode;
+    apply_tuned(sv, false);
     return CS_OK;
 }
 

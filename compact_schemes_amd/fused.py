@@ -84,14 +84,6 @@ class FusedSolver:
         self.de = torch.zeros(5 * lmx, dtype=torch.float64, device="cuda")
         self.num = None
         self.nbc_flat = flat
-        # launch shapes of the RHS sweeps per axis (cs_solver_create's defaults:
-        # z lines of 32..256 points, a multiple of 16, run 16-point chunks)
-        nz = self.dims[2]
-        self._tuned = {0: (0, 16, 0), 1: (0, 16, 1), 2: (2 if (nz % 16 == 0 and 32 <= nz <= 256) else 0, 16, 0)}
-        # filter sweeps: (fields per workgroup, lines, points per chunk); y/z
-        # lines of 32..256 points, a multiple of 16, run 16-point chunks
-        fch = [16 if (n % 16 == 0 and 32 <= n <= 512) else 8 for n in self.dims]
-        self._ftuned = {0: (5, 16, 8), 1: (1, 16, fch[1]), 2: (1, 16, fch[2])}
         self.order = 1
         self.precision = 0
         # the filter of cs_solver_rk4_step runs one fused sweep per axis where the
@@ -124,11 +116,22 @@ class FusedSolver:
 
     def tune(self, axis, variant, lines=16, xcd=0):
         check(lib().cs_solver_tune(self.h, int(axis), int(variant), int(lines), int(xcd)))
-        self._tuned[int(axis)] = (int(variant), int(lines), int(xcd))
 
     def tune_filter(self, axis, fields, lines=16, chunk=8):
         check(lib().cs_solver_tune_filter(self.h, int(axis), int(fields), int(lines), int(chunk)))
-        self._ftuned[int(axis)] = (int(fields), int(lines), int(chunk))
+
+    def get_tune(self, axis):
+        """((variant, lines, xcd) of the RHS sweeps, (fields, lines, chunk) of
+        the fused filter sweep or (0, 0, 0)) along axis."""
+        out = (ctypes.c_int * 6)()
+        check(lib().cs_solver_get_tune(self.h, int(axis), out))
+        return tuple(out[:3]), tuple(out[3:])
+
+    def autotune(self, q, reps=3, stream=None):
+        """Times the admissible sweep and filter shapes on q, applies the
+        fastest and records them in the process-wide cache
+        (compact_schemes_amd.tune); overwrites the solver's stage buffers."""
+        check(lib().cs_solver_autotune(self.h, _p(q), _p(self.sbcc), int(reps), _stream(stream)))
 
     def set_precision(self, mode):
         """1: viscous intermediates stored in FP32; 0: FP64 (cs_solver_set_precision)."""
@@ -146,14 +149,14 @@ class FusedSolver:
             return " + ".join(self.kernel_name(w) for w in FILTER_SWEEPS)
         if which >= 8:
             ax = which - 8
-            fields, lines, chunk = self._ftuned[ax]
+            fields, lines, chunk = self.get_tune(ax)[1]
             sf = f"cs::fused::SF{'1' if fields == 1 else ''}<{ax}>"
             if ax == 0:
                 return f"cs::fused::xt_kernel<8, {sf}, false>"
             return f"cs::fused::direct_kernel<{chunk}, {ax}, {lines}, {sf}, false>"
         ax = S1_AXES[self.order][which] if which < 3 else S2_AXES[self.order][min(which, 5) - 3]
         n = self.dims[ax]
-        variant, lines, _ = self._tuned[ax]
+        variant, lines, _ = self.get_tune(ax)[0]
         K = 8 if (n <= 256 or (ax == 0 and n % 8 == 0)) and variant != 2 else 16
         P = (n + K - 1) // K
         exact = n == P * K

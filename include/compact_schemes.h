@@ -269,6 +269,21 @@ int cs_solver_tune(cs_solver* sv, int axis, int variant, int lines, int xcd);
  * to 256 points, a multiple of 16; the default there).  Results agree to
  * rounding whatever the choice. */
 int cs_solver_tune_filter(cs_solver* sv, int axis, int fields, int lines, int chunk);
+/* launch shapes in use along axis (no reference counterpart): shape6 =
+ * {variant, lines, xcd} of the right-hand-side sweeps, then {fields, lines,
+ * chunk} of the fused filter sweep ({0, 0, 0} when that filter runs the line
+ * solver). */
+int cs_solver_get_tune(const cs_solver* sv, int axis, int* shape6);
+/* autotuner of the fused sweeps (no reference counterpart; the reference
+ * fixes its block shapes per build, src/numerics_rtc.hpp:120-126): times
+ * every admissible shape of cs_solver_tune for the gradient and flux sweeps
+ * of each axis, and of cs_solver_tune_filter for each fused filter sweep, on
+ * qa (reps timed launches each, 1..1000), applies the fastest and records it
+ * in the process-wide cache (cs_tune_set_sweep, cs_tune_set_filter_sweep),
+ * which every solver created afterwards in the process applies.  A shape
+ * replaces the one in use only when it is 2% faster.  Overwrites the solver's
+ * stage buffers; synchronises the stream. */
+int cs_solver_autotune(cs_solver* sv, const double* qa, const double* sbcc, int reps, cs_stream stream);
 /* storage precision of the viscous intermediates between the sweeps (no
  * reference counterpart; the reference templates everything on one Type):
  * 0 = FP64 (default); 1 = the velocity and c^2 gradients of the first two
@@ -362,17 +377,43 @@ int cs_batch_replay(cs_batch* b, cs_stream stream);
 /* work items and kernel launches of the four phases (deriv, filter x, y, z) */
 int cs_batch_info(const cs_batch* b, int* items4, int* launches4);
 /* autotuner: times the admissible chunk lengths (8, 16, 32 points) of every
- * (operator, line length) of the batch, records the fastest, rebuilds the
+ * (operator, line length, start/end rows, precision) of the batch, records
+ * the fastest (cs_tune_set), rebuilds the
  * batch and picks the fastest grid mode (cs_batch_set_persistent).  The
  * choices hold for every operator table built afterwards in
  * the process (the numerics contexts created later included), so batched and
  * single-subdomain results stay identical. */
 int cs_batch_autotune(cs_batch* b, int reps, cs_stream stream);
-/* the chunk-length choices: op 0 = derivative, 1 = filter; K = 0 clears one */
-int cs_tune_set(int op, int n, int K);
-int cs_tune_get(int op, int n, int* K);
-int cs_tune_clear(void);
-/* persisted as text, one "op length K" line per choice */
+/* the process-wide cache of launch shapes (no reference counterpart; the
+ * reference specialises its NVRTC kernels per line length, boundary rows and
+ * block shape with -D macros, src/numerics_rtc.hpp:105-126).  Line entries:
+ * chunk length K (8, 16, 32; 0 clears the entry) of the per-operator and
+ * batched line kernels for op (0 = derivative, 1 = filter), line length n,
+ * start/end = the first/last rows of the line (0 = closure rows, 1 = seam
+ * halo rows) and precision (cs_set_precision's mode); operator tables built
+ * afterwards use it. */
+int cs_tune_set(int op, int n, int start, int end, int precision, int K);
+int cs_tune_get(int op, int n, int start, int end, int precision, int* K);   /* K = 0: none */
+/* fused right-hand-side sweeps: shape of cs_solver_tune for (axis, line
+ * length n, seam = 1 for a periodic or inter-subdomain axis, precision =
+ * cs_solver_set_precision's mode); solvers apply it at creation and when
+ * their precision is set */
+int cs_tune_set_sweep(int axis, int n, int seam, int precision, int variant, int lines, int xcd);
+int cs_tune_get_sweep(int axis, int n, int seam, int precision, int* found, int* shape3);
+/* fused filter sweeps: shape of cs_solver_tune_filter for (axis, n, seam);
+ * solvers apply it at creation */
+int cs_tune_set_filter_sweep(int axis, int n, int seam, int fields, int lines, int chunk);
+int cs_tune_get_filter_sweep(int axis, int n, int seam, int* found, int* shape3);
+int cs_tune_clear(void);   /* all three kinds */
+/* persisted as text, one entry per line ('#' starts a comment):
+ *   line <op> <n> <start> <end> <precision> <K>
+ *   sweep <axis> <n> <seam> <precision> <variant> <lines> <xcd>
+ *   filter <axis> <n> <seam> <fields> <lines> <chunk>
+ * cs_tune_load adds the entries of a file to the cache; a line of three
+ * integers "<op> <n> <K>" sets K for FP64 lines with closure rows at both ends
+ * and with seam rows at both ends.  A malformed or inadmissible entry stops
+ * the load with CS_ERR_ARG (or CS_ERR_SHAPE) and a message naming file:line;
+ * the entries before it stay. */
 int cs_tune_save(const char* path);
 int cs_tune_load(const char* path);
 

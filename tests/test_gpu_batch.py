@@ -143,7 +143,8 @@ def test_batch_autotune_keeps_identity(tmp_path):
         torch.cuda.synchronize()
         for s in range(len(f)):                       # tuning reads the bound fields only
             assert torch.equal(f[s], fields[s].to(f[s].device)), s
-        choices = {(op, n): bm.tune_get(op, n) for dm in dims for n in dm for op in (0, 1)}
+        # periodic subdomains: seam rows at both ends (start = end = 1), FP64
+        choices = {(op, n): bm.tune_get(op, n, 1, 1) for dm in dims for n in dm for op in (0, 1)}
         assert all(k in (8, 16, 32) for k in choices.values())
         for (op, n), k in choices.items():
             assert (n + k - 1) // k <= (16 if k == 32 else 64)      # chunks per line
@@ -160,7 +161,8 @@ def test_batch_autotune_keeps_identity(tmp_path):
         bm.tune_save(path)
         bm.tune_clear()
         bm.tune_load(path)
-        assert {(op, n): bm.tune_get(op, n) for (op, n) in choices} == choices
+        assert {(op, n): bm.tune_get(op, n, 1, 1) for (op, n) in choices} == choices
+        assert all(bm.tune_get(op, n, 0, 0) == 0 for (op, n) in choices)     # closure ends: not tuned
     finally:
         bm.tune_clear()
 

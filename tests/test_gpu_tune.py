@@ -1,0 +1,95 @@
+"""The fused solver's autotuner and the launch-shape cache on the GPU: the
+tuned shapes give the default shapes' RHS and filtered state to rounding, are
+recorded in the cache under (axis, length, seam, precision), are applied by
+solvers created afterwards, and survive a save/load."""
+import numpy as np
+import pytest
+
+from gpu_util import case, dev, host, nrel
+
+PER = [[45, 45, 45], [45, 45, 45]]
+MIX = [[45, 10, 10], [45, 10, 10]]
+
+
+def _state(dims, seed):
+    from compact_schemes_amd import workloads as wl
+    return wl.flow_state(dims, seed=seed, noise=1e-2, width=3.0)
+
+
+@pytest.fixture
+def tune():
+    from compact_schemes_amd import tune as t
+    t.clear()
+    yield t
+    t.clear()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims, nbc", [((64, 64, 64), PER), ((48, 32, 96), MIX)])
+def test_solver_autotune_records_and_reuses(tune, tmp_path, dims, nbc):
+    import torch
+    from compact_schemes_amd.fused import FusedSolver
+    st = _state(dims, 21)
+    q = dev(st["qa"])
+    base = FusedSolver(dims, nbc, st)                   # default shapes (cache empty)
+    d0 = host(base.rhs(q)).copy()
+    f0 = torch.zeros_like(q)
+    base.sweep(7, q, f0)
+    f0 = host(f0).copy()
+    defaults = [base.get_tune(a) for a in range(3)]
+    sv = FusedSolver(dims, nbc, st)
+    case(f"solver autotune dims={dims} nbc={nbc[0]}")
+    sv.autotune(q, reps=2)
+    torch.cuda.synchronize()
+    assert np.array_equal(host(q), st["qa"])            # tuning reads qa only
+    seam = [int(nbc[0][a] == 45) for a in range(3)]
+    for a in range(3):
+        shape, fshape = sv.get_tune(a)
+        assert tune.get_sweep(a, dims[a], seam[a], 0) == shape
+        assert tune.get_filter_sweep(a, dims[a], seam[a]) == fshape
+        assert tune.get_sweep(a, dims[a], seam[a], 1) is None      # another precision: untouched
+    d1 = host(sv.rhs(q))
+    assert nrel(d1, d0) < 1e-13
+    f1 = torch.zeros_like(q)
+    sv.sweep(7, q, f1)
+    assert nrel(host(f1) - st["qa"], f0 - st["qa"]) < 5e-12
+    # a solver created afterwards applies the cache
+    again = FusedSolver(dims, nbc, st)
+    assert [again.get_tune(a) for a in range(3)] == [sv.get_tune(a) for a in range(3)]
+    assert np.array_equal(host(again.rhs(q)), d1)
+    # saved and loaded: the same shapes for a new solver
+    path = tmp_path / "tune.txt"
+    tune.save(path)
+    tune.clear()
+    assert [FusedSolver(dims, nbc, st).get_tune(a) for a in range(3)] == defaults
+    tune.load(path)
+    assert [FusedSolver(dims, nbc, st).get_tune(a) for a in range(3)] == [sv.get_tune(a) for a in range(3)]
+
+
+@pytest.mark.gpu
+def test_cached_shapes_applied_per_precision(tune):
+    """A sweep entry for FP32 intermediates applies when the solver's
+    precision is set to 1, the FP64 one at creation; a shape the solver cannot
+    run (variant 2 on x) is skipped."""
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (32, 48, 64)
+    st = _state(dims, 5)
+    tune.set_sweep(1, 48, 1, 0, 1, 8, 1)
+    tune.set_sweep(1, 48, 1, 1, 2, 8, 0)
+    tune.set_sweep(0, 32, 1, 0, 2, 16, 0)               # not admissible along x
+    tune.set_filter_sweep(2, 64, 1, 5, 8, 8)
+    sv = FusedSolver(dims, PER, st)
+    assert sv.get_tune(1)[0] == (1, 8, 1)
+    assert sv.get_tune(0)[0] == (0, 16, 0)              # the default stays
+    assert sv.get_tune(2)[1] == (5, 8, 8)
+    sv.set_precision(1)
+    assert sv.get_tune(1)[0] == (2, 8, 0)
+    q = dev(st["qa"])
+    ref = FusedSolver(dims, PER, st)
+    ref.set_precision(1)
+    tune.clear()
+    plain = FusedSolver(dims, PER, st)
+    plain.set_precision(1)
+    case("cached shapes per precision")
+    a, b = host(ref.rhs(q)).copy(), host(plain.rhs(q))
+    assert nrel(a, b) < 1e-13
