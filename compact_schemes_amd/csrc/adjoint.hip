@@ -76,10 +76,11 @@ void forward_system(int op, int n, int start, int end, double af, const double* 
         }
     } else {
         // closure filter (the cyclic one never gets here): identity end rows,
-        // reduced-order centred rows near the ends, 10th order inside
+        // reduced-order centred rows with graded alpha near the ends, 10th
+        // order inside
         double an[6][6] = {};
-        for (int M = 1; M <= 5; M++) filter_coefficients(M, af, an[M]);
-        for (int i = 0; i < n; i++) { a[i] = af; c[i] = af; }
+        for (int M = 1; M <= 5; M++) filter_coefficients(M, filter_row_alpha(M, af), an[M]);
+        for (int i = 0; i < n; i++) a[i] = c[i] = filter_row_alpha(std::min(i, n - 1 - i), af);
         a[0] = 0; c[0] = 0; a[n - 1] = 0; c[n - 1] = 0;
         for (int j = 0; j < n; j++) {
             const int dist = std::min(j, n - 1 - j);

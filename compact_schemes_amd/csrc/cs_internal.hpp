@@ -124,6 +124,8 @@ int build_adjoint_table(int op, int n, int start, int end, double alpha_f, const
 int build_line_table(int op, int n, int start, int end, double alpha_f, int K,
                      std::vector<double>& host);
 void filter_coefficients(int M, double af, double* an);
+// alpha of the closure filter's row at distance dist >= 1 from the nearer end
+double filter_row_alpha(int dist, double af);
 
 struct LineArgs {
     const double* in;

@@ -213,7 +213,8 @@ struct S2F : S2T<AX, MODE, VISC, float> {};
 
 // SF: the compact filter of the five conserved variables along AX, in place
 // (the filter of cs_filter: 10th-order interior rows, reduced-order centred
-// rows near closure ends, identity end rows; cyclic on seam axes)
+// rows with graded alpha near closure ends, identity end rows; cyclic on seam
+// axes)
 template <int AX>
 struct SF {
     static constexpr int NF = 5;
@@ -299,7 +300,8 @@ __device__ __forceinline__ void window_rhs(const double (&w)[NF][W], double (&r)
 
 // filter rows from a window w[f][u] holding rows s-5+u: 10th order inside,
 // order 2*dist at distance dist < 5 from a closure end, identity at the end
-// (an = the order-2M coefficient sets, an[6M..6M+M]); cyclic lines are all
+// (an = the order-2M coefficient sets, an[6M..6M+M], each for its row's
+// alpha; the alphas enter through the LU tables); cyclic lines are all
 // inside rows
 template <int K, int NF, int W>
 __device__ __forceinline__ void filter_rhs(const double (&w)[NF][W], double (&r)[NF][K], int s, int N, bool cyclic,

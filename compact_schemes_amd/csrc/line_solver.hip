@@ -92,6 +92,16 @@ void filter_coefficients(int M, double af, double* an) {
     }
 }
 
+// Closure lines of the filter: the row at distance d = 1..4 from the nearer
+// end is the order-2d centred filter with alpha 0.2, 0.3, 0.4, 0.45 (never
+// above alpha_f), so that the rows next to the one-sided derivative closures
+// damp grid-scale modes more than the interior's; the end point itself is not
+// filtered.  Inside (d >= 5) the 10th-order filter with alpha_f.
+double filter_row_alpha(int dist, double af) {
+    static constexpr double kEdge[5] = {0.0, 0.2, 0.3, 0.4, 0.45};
+    return dist >= 5 ? af : std::min(af, kEdge[dist < 1 ? 1 : dist]);
+}
+
 static void thomas_lu(int n, const std::vector<double>& a, const std::vector<double>& b,
                       const std::vector<double>& c, double* nu, double* mu, double* rho) {
     double cp_prev = 0.0;
@@ -120,7 +130,7 @@ int build_line_table(int op, int n, int start, int end, double af, int K, std::v
         cyclic = (start == 1 && end == 1);
         if (!cyclic && (start == 1 || end == 1))
             return fail(CS_ERR_UNSUPPORTED, "filter: a seam on one face only has no single-GPU wrap");
-        for (int i = 0; i < n; i++) { a[i] = af; c[i] = af; }
+        for (int i = 0; i < n; i++) a[i] = c[i] = cyclic ? af : filter_row_alpha(std::min(i, n - 1 - i), af);
         if (cyclic) {
             // Sherman-Morrison: A = A' + u v^T, gamma = -b_0.
             double gamma = -1.0;
@@ -130,7 +140,8 @@ int build_line_table(int op, int n, int start, int end, double af, int K, std::v
         } else {
             a[0] = 0; c[0] = 0; a[n - 1] = 0; c[n - 1] = 0;   // identity end rows
         }
-        for (int M = 1; M <= 5; M++) filter_coefficients(M, af, &h[6 * (size_t)n + 8 + 6 * M]);
+        // coefficient set M serves the rows at distance M (M = 5: every row inside)
+        for (int M = 1; M <= 5; M++) filter_coefficients(M, filter_row_alpha(M, af), &h[6 * (size_t)n + 8 + 6 * M]);
     }
     double* nu = &h[0];
     double* mu = &h[n];

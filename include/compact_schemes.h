@@ -114,7 +114,9 @@ int cs_metrics(cs_numerics* num, const double* xyz, const double* period9, doubl
 int cs_deriv(cs_numerics* num, int axis, const double* in, double* out, int var, cs_stream stream);
 /* compact filter along `axis` (absent upstream; 10th-order tridiagonal
  * filter, alpha_f from cs_set_filter_alpha, default 0.49).  Cyclic when both
- * faces of the axis are seam codes, reduced-order centred closures otherwise.
+ * faces of the axis are seam codes; otherwise the end points are left as they
+ * are and the rows at distance d = 1..4 from an end run the order-2d centred
+ * filter with alpha = min(alpha_f, 0.2, 0.3, 0.4, 0.45 for d = 1..4).
  * in == out is allowed. */
 int cs_filter(cs_numerics* num, int axis, const double* in, double* out, cs_stream stream);
 int cs_set_filter_alpha(cs_numerics* num, double alpha_f);

@@ -284,8 +284,16 @@ void filter_coeffs(int M, double af, double* an) {
     for (int j = 0; j <= M; j++) an[j] = x[j];
 }
 
+// alpha of the non-periodic filter's row at distance d from the nearer end
+// (build-defined): 0.2, 0.3, 0.4, 0.45 at d = 1..4, at most af; af inside.
+double filter_row_alpha(int d, double af) {
+    const double edge[5] = {0.0, 0.2, 0.3, 0.4, 0.45};
+    return d >= 5 ? af : std::min(af, edge[d < 1 ? 1 : d]);
+}
+
 // Rows of the non-periodic filter: identity at the end points, reduced-order
-// centred stencils (orders 2,4,6,8) on the next four rows, order 10 inside.
+// centred stencils (orders 2,4,6,8) with graded alpha on the next four rows,
+// order 10 inside.  an[M] = coefficients of order 2M for filter_row_alpha(M).
 void build_filter_rows(int N, const double* f, double af, const double (*an)[6],
                        double* a, double* b, double* c, double* r) {
     for (int t = 0; t < N; t++) {
@@ -293,9 +301,10 @@ void build_filter_rows(int N, const double* f, double af, const double (*an)[6],
         if (dist == 0) { a[t] = 0.0; b[t] = 1.0; c[t] = 0.0; r[t] = f[t]; continue; }
         int M = std::min(dist, 5);
         const double* cf = an[M];
+        const double al = filter_row_alpha(M, af);
         double s = cf[0] * f[t];
         for (int m = 1; m <= M; m++) s += 0.5 * cf[m] * (f[t + m] + f[t - m]);
-        a[t] = af; b[t] = 1.0; c[t] = af; r[t] = s;
+        a[t] = al; b[t] = 1.0; c[t] = al; r[t] = s;
     }
 }
 
@@ -480,7 +489,7 @@ orc_ctx* orc_create(int lxi, int let, int lze, const int* nbc, int table, int so
     orc_interface_table(table, c->pbco, c->pbci);
     c->af = af;
     std::memset(c->an, 0, sizeof c->an);
-    for (int M = 1; M <= 5; M++) filter_coeffs(M, af, c->an[M]);
+    for (int M = 1; M <= 5; M++) filter_coeffs(M, filter_row_alpha(M, af), c->an[M]);
     for (int a = 0; a < 3; a++) {
         long fs = c->g.face_size(a);
         c->drva[a].assign(2 * NV * fs, 0.0);
@@ -581,7 +590,7 @@ void orc_deriv_field(orc_ctx* c, int axis, const double* in, double* out, int va
 
 // Compact filter along one axis (build-defined, absent upstream).  Periodic
 // when both faces of the axis are seam codes (single subdomain wraps), else
-// reduced-order centred closures.
+// reduced-order centred closures with graded alpha (build_filter_rows).
 void orc_filter(orc_ctx* c, int axis, const double* in, double* out) {
     const Geo& g = c->g;
     int N = (int)g.n[axis];

@@ -221,14 +221,111 @@ def test_fill_and_wrap(oracle):
 
 # ---- compact filter (build-defined) ------------------------------------------------
 
-def test_filter_coefficients_closed_form(oracle):
-    af = 0.49
-    a10 = oracle.filter_coeffs(5, af)
-    closed = [(193 + 126 * af) / 256, (105 + 302 * af) / 256, 15 * (2 * af - 1) / 64,
-              45 * (1 - 2 * af) / 512, 5 * (2 * af - 1) / 256, (1 - 2 * af) / 512]
-    assert np.max(np.abs(a10 - closed)) < 1e-15
-    a2 = oracle.filter_coeffs(1, af)
-    assert np.max(np.abs(a2 - (0.5 + af))) < 1e-15
+def _gv_closed_form(M, af):
+    """Gaitonde & Visbal's centred filters of order 2M (AIAA J. 38(11), 2000,
+    the table of filter coefficients): a_0..a_M of
+    af x[i-1] + x[i] + af x[i+1] = sum_n a_n/2 (f[i+n] + f[i-n])."""
+    return {1: [0.5 + af, 0.5 + af],
+            2: [5 / 8 + 3 * af / 4, 0.5 + af, -1 / 8 + af / 4],
+            3: [11 / 16 + 5 * af / 8, 15 / 32 + 17 * af / 16, -3 / 16 + 3 * af / 8, 1 / 32 - af / 16],
+            4: [(93 + 70 * af) / 128, (7 + 18 * af) / 16, (-7 + 14 * af) / 32, (1 - 2 * af) / 16,
+                (-1 + 2 * af) / 128],
+            5: [(193 + 126 * af) / 256, (105 + 302 * af) / 256, 15 * (2 * af - 1) / 64,
+                45 * (1 - 2 * af) / 512, 5 * (2 * af - 1) / 256, (1 - 2 * af) / 512]}[M]
+
+
+# alpha of the closure filter's rows at distance 1..4 from an end (capped at
+# alpha_f), the build's choice
+EDGE_ALPHA = [0.2, 0.3, 0.4, 0.45]
+
+
+@pytest.mark.parametrize("af", [0.49, 0.45, 0.3, 0.2, -0.1])
+def test_filter_coefficients_closed_form(oracle, af):
+    for M in range(1, 6):
+        got = oracle.filter_coeffs(M, af)[:M + 1]
+        assert np.max(np.abs(got - _gv_closed_form(M, af))) < 4e-15, M      # rounding of the moment solve
+
+
+def _dense_closure_filter(n, af):
+    """A^-1 B of a closure line assembled from the closed forms: identity end
+    rows, order 2d with alpha min(af, EDGE_ALPHA[d-1]) at distance d = 1..4,
+    order 10 with af inside."""
+    A, B = np.eye(n), np.zeros((n, n))
+    for i in range(n):
+        d = min(i, n - 1 - i)
+        if d == 0:
+            B[i, i] = 1.0
+            continue
+        M = min(d, 5)
+        al = af if M == 5 else min(af, EDGE_ALPHA[M - 1])
+        A[i, i - 1] = A[i, i + 1] = al
+        cf = _gv_closed_form(M, al)
+        B[i, i] += cf[0]
+        for m in range(1, M + 1):
+            B[i, i + m] += 0.5 * cf[m]
+            B[i, i - m] += 0.5 * cf[m]
+    return np.linalg.solve(A, B)
+
+
+@pytest.mark.parametrize("af", [0.49, 0.35])
+def test_closure_filter_rows_dense(oracle, af):
+    """The oracle's closure-line filter equals A^-1 B built from the published
+    closed forms with the graded alphas, along each axis."""
+    rng = np.random.default_rng(17)
+    for axis, dims in ((0, (24, 8, 8)), (1, (8, 21, 8)), (2, (8, 8, 30))):
+        ctx = oracle.Context(dims, NBC_CL, af=af)
+        f = rng.standard_normal(dims[::-1])                    # [z][y][x]
+        out = ctx.filter(axis, f.reshape(-1)).reshape(dims[::-1])
+        F = _dense_closure_filter(dims[axis], af)
+        want = np.moveaxis(np.tensordot(F, np.moveaxis(f, 2 - axis, 0), axes=(1, 0)), 0, 2 - axis)
+        assert rel(out, want) < 1e-13, axis
+
+
+def test_closure_filter_rows_damp_grid_modes():
+    """Each graded row d = 1..4 keeps T(0) = 1 and T(pi) = 0, and damps every
+    wavenumber at least as much as the same order with alpha 0.49 (|T| <= 1)."""
+    w = np.linspace(0.0, np.pi, 721)
+    for d in range(1, 5):
+        for al, ref in ((EDGE_ALPHA[d - 1], 0.49),):
+            def T(a):
+                cf = _gv_closed_form(d, a)
+                return (cf[0] + sum(cf[m] * np.cos(m * w) for m in range(1, d + 1))) / (1 + 2 * a * np.cos(w))
+            t, t49 = T(al), T(ref)
+            assert abs(t[0] - 1) < 1e-15 and abs(t[-1]) < 1e-14
+            assert np.all(t >= -1e-14) and np.all(t <= 1 + 1e-14)
+            assert np.all(t <= t49 + 1e-14)
+
+
+def _noise_run(oracle, dims, steps, filt, every):
+    from compact_schemes_amd import workloads as wl
+    st = wl.flow_state(dims, seed=1, amp=0.0, noise=1e-8)
+    base = wl.flow_state(dims, seed=1, amp=0.0, noise=0.0)["qa"]
+    ctx = oracle.Context(dims, [[10, 10, 10], [10, 10, 10]])
+    q = st["qa"].copy()
+    amp = []
+    for k in range(1, steps + 1):
+        ctx.rk4_step(0, q, st["ssk"], st["xim"], st["etm"], st["zem"], st["yaco"], st["cm"], st["sbcc"],
+                     st["umf"], cfl=0.95, filt=filt)
+        if k % every == 0:
+            amp.append(float(np.abs(q - base).max()))
+            if not np.isfinite(amp[-1]) or amp[-1] > 1e-3:
+                break
+    return amp
+
+
+def test_noise_bounded_nonreflective_faces(oracle):
+    """1e-8 white noise on uniform Mach-0.5 flow, 32^3, every face
+    non-reflective, CFL 0.95: 1000 RK4 steps with the filter stay finite and
+    small, and grow linearly (the drift of entropy and transverse velocity at
+    the inflow face that zero incoming characteristic amplitudes leave free),
+    not exponentially; without the filter the grid-scale modes grow past 1e-3
+    within 300 steps."""
+    amp = _noise_run(oracle, (32, 32, 32), 1000, True, 250)
+    assert len(amp) == 4 and all(np.isfinite(amp))
+    assert amp[-1] < 2e-5
+    assert amp[3] / amp[1] < 2.5                    # 500 -> 1000 steps: linear growth doubles
+    bare = _noise_run(oracle, (32, 32, 32), 300, False, 50)
+    assert bare[-1] > 1e-3 or not np.isfinite(bare[-1])
 
 
 def test_filter_transfer_function(oracle):
