@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--adjoint-probe", action="store_true",
                     help="only run --reps RK4 steps recording tapes and their adjoints on the 128^3 Euler "
                          "workload of --workload adjoint (the program rocprofv3 profiles the adjoint kernels with)")
+    ap.add_argument("--batch-probe", action="store_true",
+                    help="only run --reps steps of the 96-subdomain batch in the tuned grid mode (one launch per "
+                         "subdomain and operator) and a plain device copy of known size (the program rocprofv3 "
+                         "profiles the batched kernels with)")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--workload", default="rhs", choices=["rhs", "plain64", "batch96", "precision", "adjoint"],
                     help="rhs: the 256^3 viscous RK4 step (default); plain64: derivative x/y/z + filter x/y/z "
@@ -87,6 +91,8 @@ def main():
         return pmc_probe(args)
     if args.adjoint_probe:
         return adjoint_probe(args)
+    if args.batch_probe:
+        return batch_probe(args)
     if args.workload == "plain64":
         return plain_bench(args)
     if args.workload == "batch96":
@@ -639,6 +645,39 @@ def adjoint_probe(args, n=128):
             sv.rk4_vjp_tape(tapes[i], lam, dt)
     torch.cuda.synchronize()
     print(json.dumps({"adjoint_probe": "done", "steps": args.reps, "n": n}), flush=True)
+
+
+def batch_probe(args):
+    """--reps steps (derivative x, y, z, then filter x, y, z) of the
+    96-subdomain batch in grid mode 2, the mode the autotuner picks on MI355X
+    (one launch of the single-subdomain kernels per subdomain and operator),
+    through cs_batch_deriv and cs_batch_filter on one stream (the same
+    launches cs_batch_step deals over eight forked streams; counter
+    collection serialises dispatches, so the probe keeps one stream), and a
+    plain copy of known size; per-dispatch counters are summed per kernel."""
+    import torch
+    from compact_schemes_amd import batch as bm
+    from compact_schemes_amd.workloads import batch_fields
+    per = [[45, 45, 45], [45, 45, 45]]
+    dims, f0 = batch_fields()
+    f = [t.clone() for t in f0]
+    d = [torch.empty_like(f[s // 3]) for s in range(3 * len(dims))]
+    bm.tune_clear()
+    b = bm.Batch(dims, per)
+    b.bind(f, d)
+    b.set_persistent(2)
+    st = torch.cuda.current_stream()
+    for _ in range(args.reps):
+        b.deriv(st)
+        b.filter(st)
+    npts = sum(x[0] * x[1] * x[2] for x in dims)
+    src = torch.ones(npts, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"batch_probe": "done", "steps": args.reps, "subdomains": len(dims), "points": npts,
+                      "copy_bytes_read": 8 * npts, "copy_bytes_written": 8 * npts}), flush=True)
 
 
 def pmc_probe(args):
