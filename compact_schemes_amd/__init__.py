@@ -36,7 +36,7 @@ EXPORTS = [
     "cs_calc_time_step_noalloc", "cs_gcbc_go", "cs_wall_condition_update", "cs_rk_stage",
     "cs_set_precision", "cs_deriv_adjoint", "cs_filter_adjoint", "cs_solver_create", "cs_solver_destroy", "cs_solver_rhs", "cs_solver_sweep", "cs_solver_tune", "cs_solver_tune_filter",
     "cs_solver_get_tune", "cs_solver_autotune",
-    "cs_solver_set_order", "cs_solver_set_precision", "cs_solver_rhs_vjp", "cs_solver_rk4_vjp", "cs_solver_tape_size",
+    "cs_solver_set_order", "cs_solver_set_precision", "cs_solver_set_metrics", "cs_solver_rhs_vjp", "cs_solver_rk4_vjp", "cs_solver_tape_size",
     "cs_solver_rk4_step_tape", "cs_solver_rk4_vjp_tape", "cs_solver_adjoint_mode",
     "cs_batch_create", "cs_batch_destroy", "cs_batch_bind", "cs_batch_deriv", "cs_batch_filter", "cs_batch_step",
     "cs_batch_capture", "cs_batch_replay", "cs_batch_get_persistent", "cs_batch_set_persistent", "cs_batch_info", "cs_batch_autotune", "cs_tune_set", "cs_tune_get",
@@ -143,6 +143,7 @@ def lib():
             "cs_solver_tune": (I, [P, I, I, I, I]),
             "cs_solver_tune_filter": (I, [P, I, I, I, I]),
             "cs_solver_get_tune": (I, [P, I, P]),
+            "cs_solver_set_metrics": (I, [P, P, P, P, P, P, P, P]),
             "cs_solver_autotune": (I, [P, P, P, I, P]),
             "cs_solver_set_order": (I, [P, I]),
             "cs_solver_set_precision": (I, [P, I]),

@@ -68,6 +68,40 @@ __global__ void __launch_bounds__(kTB) dt_partial_kernel(const double* q, long l
     if (threadIdx.x == 0) { part[2 * blockIdx.x] = s0[0]; part[2 * blockIdx.x + 1] = s1[0]; }
 }
 
+// the same with per-point metric rows and Jacobian (curvilinear solvers;
+// calc_time_step_kernel's full form)
+__global__ void __launch_bounds__(kTB) dt_partial_curv_kernel(const double* q, long lmx, Phys ph, Metric mt,
+                                                              double* part) {
+    __shared__ double s0[kTB], s1[kTB];
+    double m0 = 0.0, m1 = 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < lmx; t += (long)gridDim.x * blockDim.x) {
+        double qq[5];
+        load_q(q, t, lmx, qq);
+        Prim p = primitive(qq);
+        const double u = p.u + ph.umf.x, v = p.v + ph.umf.y, w = p.w + ph.umf.z;
+        double rr1 = 0.0, rr2 = 0.0;
+        for (int a = 0; a < 3; a++) {
+            const double mx = mt.m[a][0][t], my = mt.m[a][1][t], mz = mt.m[a][2][t];
+            rr1 += mx * mx + my * my + mz * mz;
+            rr2 += fabs(mx * u + my * v + mz * w);
+        }
+        const double ssi = fabs(mt.J[t]);
+        m0 = fmax(m0, (sqrt(p.c2 * rr1) + rr2) * ssi);
+        m1 = fmax(m1, p.rhoi * ph.mu * rr1 * ssi * ssi);
+    }
+    s0[threadIdx.x] = m0;
+    s1[threadIdx.x] = m1;
+    __syncthreads();
+    for (int w = kTB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s0[threadIdx.x] = fmax(s0[threadIdx.x], s0[threadIdx.x + w]);
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = s0[0]; part[2 * blockIdx.x + 1] = s1[0]; }
+}
+
 // out[0] = dt = min(cfl/max0, 0.5/max1) (or dt_fixed when > 0), out[1..2] = maxima
 __global__ void __launch_bounds__(kTB) dt_final_kernel(const double* part, int n, double cfl, double dt_fixed,
                                                        double* out) {
@@ -97,6 +131,7 @@ __global__ void set_dt_kernel(double dt, double* out) {
 
 static int sweep_at(cs_solver* sv, int pos, const SweepCommon& cm, const double* q, double* out, const double* q0,
                     int div, hipStream_t s) {
+    if (sv->curv) return sweep_at_curv(sv, pos, cm, q, out, q0, div, s);
     if (sv->cfg.viscous && sv->prec == 1) return sweep_at_f32(sv, pos, cm, q, out, q0, div, s);
     if (pos < 3) return s1_sweep(sv, pos, cm, q, s);
     if (sv->cfg.viscous) return s2_sweep<true>(sv, pos - 3, cm, q, out, q0, div, s);
@@ -176,7 +211,11 @@ int time_step(cs_solver* sv, const double* q, double dt_fixed, hipStream_t s) {
         CS_HIP(hipGetLastError());
         return CS_OK;
     }
-    hipLaunchKernelGGL(dt_partial_kernel, dim3(sv->red_blocks), dim3(kTB), 0, s, q, lmx, phys_of(sv), sv->red);
+    if (sv->curv)
+        hipLaunchKernelGGL(dt_partial_curv_kernel, dim3(sv->red_blocks), dim3(kTB), 0, s, q, lmx, phys_of(sv), sv->met,
+                           sv->red);
+    else
+        hipLaunchKernelGGL(dt_partial_kernel, dim3(sv->red_blocks), dim3(kTB), 0, s, q, lmx, phys_of(sv), sv->red);
     CS_HIP(hipGetLastError());
     hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(kTB), 0, s, sv->red, sv->red_blocks, sv->cfg.cfl, 0.0, sv->dt);
     CS_HIP(hipGetLastError());

@@ -31,6 +31,16 @@ struct AxisDesc {
     int ftl;
 };
 
+// per-point geometry of a curvilinear solver (cs_solver_set_metrics; caller's
+// device arrays): m[a][k] = component k of the metric row of axis a (xim, etm,
+// zem of the reference, [3][lmx] each), J = yaco [lmx], cm[a] = the face
+// normals of axis a, [2 face][3][face] as gcbc_go takes them
+struct Metric {
+    const double* m[3][3];
+    const double* J;
+    const double* cm[3];
+};
+
 struct SweepCommon {
     int lxi;
     long plane;
@@ -75,6 +85,8 @@ struct cs_solver {
     double* rec = nullptr;     // faces of the RHS being evaluated go here when set ([3 axes][2][5][face])
     int adj_mode = 1;          // 1: fused transposed sweeps where the lines allow; 0: per-operator products
     int prec = 0;              // 1: the viscous intermediates (gradients, stress, heat flux) stored in FP32
+    int curv = 0;              // 1: per-point metrics, Jacobian and face normals (met)
+    cs::fused::Metric met{};
 };
 
 namespace cs {
@@ -89,6 +101,9 @@ int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, 
 // sweep `pos` of a viscous RHS with the intermediates in FP32 (fused_f32.hip)
 int sweep_at_f32(cs_solver* sv, int pos, const SweepCommon& cm, const double* q, double* out, const double* q0,
                  int div, hipStream_t s);
+// sweep `pos` of a curvilinear solver (fused_curv.hip)
+int sweep_at_curv(cs_solver* sv, int pos, const SweepCommon& cm, const double* q, double* out, const double* q0,
+                  int div, hipStream_t s);
 int time_step(cs_solver* sv, const double* q, double dt_fixed, hipStream_t s);
 int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, double* out, hipStream_t s);
 int build_axis_table(int n, int seam, int K, int P, std::vector<double>& h);

@@ -27,9 +27,10 @@ __device__ __forceinline__ void stv(float* p, uoff bo, double v) {
 // S1: velocity and c^2 gradients along AX.  The first two S1 sweeps of an
 // RHS store their index-space gradients; the third (CLOSE) combines its own
 // gradients with the stored ones into the stress tensor and heat flux.
-template <int AX, bool CLOSE, class TS>
+template <int AX, bool CLOSE, class TS, bool CURV = false>
 struct S1T {
     typedef TS Store;                      // storage type of the gradients, stress and heat flux
+    static constexpr bool kCurv = CURV;    // per-point metrics (curvilinear solvers)
     static constexpr int NF = 4;
     static constexpr bool kGcbc = false;   // gradients only; BC updates act on the flux sweeps
     static constexpr bool kPv = false;     // kernels without an LDS stash re-derive u, v, w from q at the store
@@ -45,6 +46,7 @@ struct S1T {
     TS* tau[6];               // xx, yy, zz, xy, yz, zx
     TS* hf[3];
     Phys ph;
+    Metric mt;                // CURV: the point's metric rows and Jacobian
 
     __device__ __forceinline__ void point(uoff bo, double* v) const {
         double qq[5];
@@ -77,16 +79,33 @@ struct S1T {
         for (int a = 0; a < 3; a++)
 #pragma unroll
             for (int f = 0; f < 4; f++) gr[a][f] = (a == AX) ? d[f] : ldv(gin[a][f], bo);
-        // chain rule of calc_viscous_shear_stress_post_compute_kernel with
-        // diagonal metrics (src/cuda/kernels/physics.hpp:267-281)
-        double g[9];
+        // chain rule of calc_viscous_shear_stress_post_compute_kernel
+        // (src/cuda/kernels/physics.hpp:267-281): d/dx_k = sum_a m_a[k] d/dxi_a
+        double g[9], c2g[3], J = ph.J;
+        if constexpr (CURV) {
+            double M[3][3];
 #pragma unroll
-        for (int f = 0; f < 3; f++)
+            for (int a = 0; a < 3; a++)
 #pragma unroll
-            for (int a = 0; a < 3; a++) g[3 * f + a] = ph.m[a] * gr[a][f];
-        double c2g[3] = {ph.m[0] * gr[0][3], ph.m[1] * gr[1][3], ph.m[2] * gr[2][3]};
+                for (int k = 0; k < 3; k++) M[a][k] = ldg(mt.m[a][k], bo);
+#pragma unroll
+            for (int f = 0; f < 3; f++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) g[3 * f + k] = M[0][k] * gr[0][f] + M[1][k] * gr[1][f] + M[2][k] * gr[2][f];
+#pragma unroll
+            for (int k = 0; k < 3; k++) c2g[k] = M[0][k] * gr[0][3] + M[1][k] * gr[1][3] + M[2][k] * gr[2][3];
+            J = ldg(mt.J, bo);
+        } else {
+            // diagonal metrics: m_a[k] = m[a] for k = a
+#pragma unroll
+            for (int f = 0; f < 3; f++)
+#pragma unroll
+                for (int a = 0; a < 3; a++) g[3 * f + a] = ph.m[a] * gr[a][f];
+#pragma unroll
+            for (int a = 0; a < 3; a++) c2g[a] = ph.m[a] * gr[a][3];
+        }
         double t6[6], h3[3], d4;
-        viscous_closure(ph.mu, ph.J, g, c2g, uvw, t6, h3, &d4);
+        viscous_closure(ph.mu, J, g, c2g, uvw, t6, h3, &d4);
 #pragma unroll
         for (int k = 0; k < 6; k++) stv(tau[k], bo, t6[k]);
 #pragma unroll
@@ -101,12 +120,16 @@ template <int AX, bool CLOSE>
 struct S1 : S1T<AX, CLOSE, double> {};
 template <int AX, bool CLOSE>
 struct S1F : S1T<AX, CLOSE, float> {};
+// curvilinear solvers (cs_solver_set_metrics): FP64 intermediates
+template <int AX, bool CLOSE>
+struct S1C : S1T<AX, CLOSE, double, true> {};
 
 // S2: flux divergence along AX.  MODE 0: write the partial RHS; 1: add to it;
 // 2: add and write the RHS; 3: add and write q0 - dtk*J*RHS (RK stage).
-template <int AX, int MODE, bool VISC, class TS>
+template <int AX, int MODE, bool VISC, class TS, bool CURV = false>
 struct S2T {
     typedef TS Store;
+    static constexpr bool kCurv = CURV;    // per-point metrics (curvilinear solvers)
     static constexpr int NF = 5;
     static constexpr bool kGcbc = true;
     static constexpr bool kPv = false;
@@ -117,6 +140,9 @@ struct S2T {
     const double* q[5];
     const TS* tr[3];          // row AX of the stress tensor
     const TS* hx;             // heat flux component AX
+    const TS* tau6[6];        // CURV: the whole stress tensor (xx, yy, zz, xy, yz, zx)
+    const TS* h3[3];          // CURV: the whole heat flux
+    Metric mt;                // CURV: the point's metric rows, Jacobian, face normals
     const double* din[5];
     double* dout[5];
     const double* q0[5];
@@ -133,6 +159,25 @@ struct S2T {
     __device__ __forceinline__ void point(uoff bo, double* F) const {
         double qq[5];
         Prim p = prim_at(q, bo, qq);
+        if constexpr (CURV) {
+            // the AX row with the point's metric row (m0, m1, m2)
+            const double m0 = ldg(mt.m[AX][0], bo), m1 = ldg(mt.m[AX][1], bo), m2 = ldg(mt.m[AX][2], bo);
+            const double ss = m0 * (p.u + ph.umf.x) + m1 * (p.v + ph.umf.y) + m2 * (p.w + ph.umf.z);
+            F[0] = qq[0] * ss;
+            F[1] = qq[1] * ss + m0 * p.p;
+            F[2] = qq[2] * ss + m1 * p.p;
+            F[3] = qq[3] * ss + m2 * p.p;
+            F[4] = (qq[4] + p.p) * ss - p.p * (ph.umf.x * m0 + ph.umf.y * m1 + ph.umf.z * m2);
+            if (VISC) {
+                const double txx = ldv(tau6[0], bo), tyy = ldv(tau6[1], bo), tzz = ldv(tau6[2], bo);
+                const double txy = ldv(tau6[3], bo), tyz = ldv(tau6[4], bo), tzx = ldv(tau6[5], bo);
+                F[1] -= m0 * txx + m1 * txy + m2 * tzx;
+                F[2] -= m0 * txy + m1 * tyy + m2 * tyz;
+                F[3] -= m0 * tzx + m1 * tyz + m2 * tzz;
+                F[4] -= m0 * ldv(h3[0], bo) + m1 * ldv(h3[1], bo) + m2 * ldv(h3[2], bo);
+            }
+            return;
+        }
         const double m = ph.m[AX];
         const double vel = (AX == 0) ? p.u : (AX == 1 ? p.v : p.w);
         const double uf = (AX == 0) ? ph.umf.x : (AX == 1 ? ph.umf.y : ph.umf.z);
@@ -161,7 +206,7 @@ struct S2T {
             for (int f = 0; f < 5; f++) d[f] = ldg(din[f], bo) + d[f];
         } else if (MODE == 3) {
             const double dtk = dtp[0] / (double)div;
-            const double fj = dtk * ph.J;
+            const double fj = dtk * (CURV ? ldg(mt.J, bo) : ph.J);
 #pragma unroll
             for (int f = 0; f < 5; f++) {
                 const double de = ldg(din[f], bo) + d[f];
@@ -177,23 +222,35 @@ struct S2T {
     // boundary derivatives of this sweep (gcbc_setup_kernel +
     // gcbc_update_non_reflective_kernel, src/cuda/kernels/gcbc.hpp:251-454)
     __device__ __forceinline__ void gcbc(uoff bo, int face, const double* drv, double* corr) const {
+        // face index of the face point (x: k*let+j, y: k*lxi+i, z: j*lxi+i)
+        const long o = bo / 8u;
+        const long fi = AX == 0 ? o / lxi : (AX == 1 ? (o / plane) * lxi + o % lxi : o % plane);
         if (rec) {
-            // face index of the face point (x: k*let+j, y: k*lxi+i, z: j*lxi+i)
-            const long o = bo / 8u;
-            const long fi = AX == 0 ? o / lxi : (AX == 1 ? (o / plane) * lxi + o % lxi : o % plane);
 #pragma unroll
             for (int v = 0; v < 5; v++) rec[(face * 5 + v) * nface + fi] = drv[v];
         }
         double qq[5];
         Prim p = prim_at(q, bo, qq);
-        // unit normal of the face: grad(xi_AX) / |grad(xi_AX)|, grad(xi_AX) = J * metric * e_AX
-        const double sg = ph.m[AX] * ph.J < 0.0 ? -1.0 : 1.0;
-        const double cm[3] = {AX == 0 ? sg : 0.0, AX == 1 ? sg : 0.0, AX == 2 ? sg : 0.0};
+        double cm[3], J = ph.J;
+        if constexpr (CURV) {
+            // the caller's face normals ([2 face][3][face], as gcbc_go takes them)
+            const double* c = mt.cm[AX] + (long)face * 3 * nface + fi;
+            cm[0] = c[0];
+            cm[1] = c[nface];
+            cm[2] = c[2 * nface];
+            J = ldg(mt.J, bo);
+        } else {
+            // unit normal of the face: grad(xi_AX) / |grad(xi_AX)|, grad(xi_AX) = J * metric * e_AX
+            const double sg = ph.m[AX] * ph.J < 0.0 ? -1.0 : 1.0;
+            cm[0] = AX == 0 ? sg : 0.0;
+            cm[1] = AX == 1 ? sg : 0.0;
+            cm[2] = AX == 2 ? sg : 0.0;
+        }
         Eig e;
         e.eleme(cm, qq[0], qq + 1, p.p, ph.umf);
         double cha[5], ch[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) cha[v] = ph.J * drv[v];
+        for (int v = 0; v < 5; v++) cha[v] = J * drv[v];
         e.l_apply(cm, cha, ch);
         const double ra0 = 1 - 2 * face;
         double dl[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -202,7 +259,7 @@ struct S2T {
         e.r_apply(cm, dl, corr);
         // D25: a change of J*de (the amplitudes were formed from J*drv), so de changes by corr / J
 #pragma unroll
-        for (int v = 0; v < 5; v++) corr[v] /= ph.J;
+        for (int v = 0; v < 5; v++) corr[v] /= J;
     }
 };
 
@@ -210,6 +267,8 @@ template <int AX, int MODE, bool VISC>
 struct S2 : S2T<AX, MODE, VISC, double> {};
 template <int AX, int MODE, bool VISC>
 struct S2F : S2T<AX, MODE, VISC, float> {};
+template <int AX, int MODE, bool VISC>
+struct S2C : S2T<AX, MODE, VISC, double, true> {};
 
 // SF: the compact filter of the five conserved variables along AX, in place
 // (the filter of cs_filter: 10th-order interior rows, reduced-order centred
@@ -1244,6 +1303,7 @@ static F<AX, CLOSE> make_s1(const cs_solver* sv, const double* q) {
         fields(store_of<TS>(sv->hf), lmx, a.hf);
     }
     a.ph = phys_of(sv);
+    if constexpr (F<AX, CLOSE>::kCurv) a.mt = sv->met;
     return a;
 }
 
@@ -1263,7 +1323,12 @@ static F<AX, MODE, V> make_s2(const cs_solver* sv, const double* q, const double
         a.tr[1] = tau + (long)t1 * lmx;
         a.tr[2] = tau + (long)t2 * lmx;
         a.hx = store_of<TS>(sv->hf) + (long)AX * lmx;
+        if constexpr (F<AX, MODE, V>::kCurv) {
+            fields(tau, lmx, a.tau6);
+            fields((const TS*)store_of<TS>(sv->hf), lmx, a.h3);
+        }
     }
+    if constexpr (F<AX, MODE, V>::kCurv) a.mt = sv->met;
     fields(din, lmx, a.din);
     fields(dout, lmx, a.dout);
     fields(q0, lmx, a.q0);

@@ -448,8 +448,35 @@ int cs_solver_autotune(cs_solver* sv, const double* qa, const double* sbcc, int 
     return CS_OK;
 }
 
+int cs_solver_set_metrics(cs_solver* sv, const double* xim, const double* etm, const double* zem,
+                          const double* yaco, const double* cm0, const double* cm1, const double* cm2) {
+    if (!sv) return fail(CS_ERR_ARG, "null solver");
+    const double* rows[3] = {xim, etm, zem};
+    const double* cms[3] = {cm0, cm1, cm2};
+    const int given = (xim != nullptr) + (etm != nullptr) + (zem != nullptr) + (yaco != nullptr) +
+                      (cm0 != nullptr) + (cm1 != nullptr) + (cm2 != nullptr);
+    if (given == 0) {            // back to the constants of the configuration
+        sv->curv = 0;
+        sv->met = Metric{};
+        return CS_OK;
+    }
+    if (given != 7) return fail(CS_ERR_ARG, "metrics: xim, etm, zem, yaco and the three face-normal arrays");
+    if (sv->prec != 0) return fail(CS_ERR_UNSUPPORTED, "curvilinear solvers store their intermediates in FP64");
+    Metric m{};
+    const long lmx = sv->g.lmx;
+    for (int a = 0; a < 3; a++) {
+        for (int k = 0; k < 3; k++) m.m[a][k] = rows[a] + (long)k * lmx;
+        m.cm[a] = cms[a];
+    }
+    m.J = yaco;
+    sv->met = m;
+    sv->curv = 1;
+    return CS_OK;
+}
+
 int cs_solver_set_precision(cs_solver* sv, int mode) {
     if (!sv || (mode != 0 && mode != 1)) return fail(CS_ERR_ARG, "precision mode is 0 or 1");
+    if (mode == 1 && sv->curv) return fail(CS_ERR_UNSUPPORTED, "curvilinear solvers store their intermediates in FP64");
     sv->prec = mode;
     apply_tuned(sv, false);
     return CS_OK;

@@ -824,6 +824,7 @@ int cs_solver_rk4_step_tape(cs_solver* sv, double* qa, const double* sbcc, doubl
     if (!sv || !qa || !tape) return cs::fail(CS_ERR_ARG, "null argument");
     if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
         return cs::fail(CS_ERR_ARG, "non-reflective faces need sbcc");
+    if (sv->curv) return cs::fail(CS_ERR_UNSUPPORTED, "the adjoint covers solvers with constant metrics");
     hipStream_t s = (hipStream_t)stream;
     int rc = cs::fused::rk4_step_tape(sv, qa, sbcc, dt, tape, s);
     if (rc || !dt_out) return rc;
@@ -838,6 +839,7 @@ int cs_solver_rk4_vjp_tape(cs_solver* sv, const double* tape, double* lam, doubl
     if (!sv || !tape || !lam) return cs::fail(CS_ERR_ARG, "null argument");
     if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
         return cs::fail(CS_ERR_ARG, "non-reflective faces need sbcc");
+    if (sv->curv) return cs::fail(CS_ERR_UNSUPPORTED, "the adjoint covers solvers with constant metrics");
     return cs::fused::rk4_vjp_tape(sv, tape, lam, dt, sbcc, (hipStream_t)stream);
 }
 

@@ -50,19 +50,26 @@ def _uniform(a, name):
 
 class FusedSolver:
     """cs_solver for one subdomain.  state: dict of numpy arrays as produced by
-    compact_schemes_amd.workloads.flow_state (metrics, yaco, ssk must be uniform)."""
+    compact_schemes_amd.workloads.flow_state (ssk must be uniform; metrics and
+    yaco uniform and diagonal unless curvilinear=True, which hands the
+    per-point metrics, yaco and face normals cm to cs_solver_set_metrics)."""
 
-    def __init__(self, dims, nbc, state, viscous=True, cfl=0.95, alpha_f=0.49, filt=True):
+    def __init__(self, dims, nbc, state, viscous=True, cfl=0.95, alpha_f=0.49, filt=True, curvilinear=False):
         import torch
         self.dims = tuple(int(d) for d in dims)
         lmx = self.dims[0] * self.dims[1] * self.dims[2]
         self.lmx = lmx
+        self.curvilinear = bool(curvilinear)
         xim, etm, zem = (np.asarray(state[k]) for k in ("xim", "etm", "zem"))
-        metric = (_uniform(xim[:lmx], "xi_x"), _uniform(etm[lmx:2 * lmx], "eta_y"), _uniform(zem[2 * lmx:], "zeta_z"))
-        for arr, zero in ((xim, (1, 2)), (etm, (0, 2)), (zem, (0, 1))):
-            for z in zero:
-                if np.any(arr[z * lmx:(z + 1) * lmx] != 0):
-                    raise ValueError("fused path needs diagonal metrics")
+        if self.curvilinear:
+            metric = (1.0, 1.0, 1.0)
+        else:
+            metric = (_uniform(xim[:lmx], "xi_x"), _uniform(etm[lmx:2 * lmx], "eta_y"),
+                      _uniform(zem[2 * lmx:], "zeta_z"))
+            for arr, zero in ((xim, (1, 2)), (etm, (0, 2)), (zem, (0, 1))):
+                for z in zero:
+                    if np.any(arr[z * lmx:(z + 1) * lmx] != 0):
+                        raise ValueError("fused path needs diagonal metrics unless curvilinear=True")
         cfg = SolverConfig()
         cfg.dims = Dims(*self.dims)
         flat = [int(v) for row in nbc for v in row]
@@ -71,7 +78,7 @@ class FusedSolver:
         cfg.umf = Point(*state["umf"])
         cfg.mu = _uniform(np.asarray(state["ssk"]), "ssk")
         cfg.metric = (ctypes.c_double * 3)(*metric)
-        cfg.yaco = _uniform(np.asarray(state["yaco"]), "yaco")
+        cfg.yaco = 1.0 if self.curvilinear else _uniform(np.asarray(state["yaco"]), "yaco")
         cfg.cfl = float(cfl)
         cfg.alpha_f = float(alpha_f)
         cfg.filter = int(filt)
@@ -81,6 +88,11 @@ class FusedSolver:
         check(lib().cs_solver_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
         self.sbcc = torch.as_tensor(np.asarray(state["sbcc"], dtype=np.float64)).cuda().contiguous()
+        if self.curvilinear:
+            # device copies the solver reads per point; kept alive with it
+            dev = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+            self.metric_fields = [dev(xim), dev(etm), dev(zem), dev(state["yaco"])] + [dev(c) for c in state["cm"]]
+            check(lib().cs_solver_set_metrics(self.h, *[_p(t) for t in self.metric_fields]))
         self.de = torch.zeros(5 * lmx, dtype=torch.float64, device="cuda")
         self.num = None
         self.nbc_flat = flat
@@ -162,7 +174,7 @@ class FusedSolver:
         exact = n == P * K
         nbc = self.nbc_flat
         gcbc = nbc[ax] == 10 and nbc[3 + ax] == 10
-        f32 = "F" if (self.viscous and self.precision == 1) else ""
+        f32 = "C" if self.curvilinear else ("F" if (self.viscous and self.precision == 1) else "")
         if which < 3:
             sw, gc = f"cs::fused::S1{f32}<{ax}, {str(which == 2).lower()}>", "false"
         else:

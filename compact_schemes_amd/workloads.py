@@ -158,3 +158,42 @@ def with_spacing(st, dims, spacing):
     out["yaco"] = np.full(lmx, J)
     out["cm"] = [c * np.sign(J * m[a]) for a, c in enumerate(face_normals(dims))]
     return out
+
+
+def curvilinear_state(dims, seed=2, skew=0.3, **flow):
+    """flow_state on the periodic sine-skewed grid x = i + s sin(2 pi j / Ny),
+    y = j + s sin(2 pi k / Nz), z = k + s sin(2 pi i / Nx) (unit index
+    spacing), with its analytic metrics as the reference stores them: xim,
+    etm, zem = the cofactor rows of the coordinate Jacobian (d x/d eta x
+    d x/d zeta, ...), yaco = 1 / det; cm = the unit normals grad(xi_a) /
+    |grad(xi_a)| on the two faces of each axis ([2 face][3][face], face points
+    ordered as the gcbc face loops run: x k*let+j, y k*lxi+i, z j*lxi+i)."""
+    st = flow_state(dims, seed=seed, **flow)
+    nx, ny, nz = dims
+    lmx = nx * ny * nz
+    x, y, z = (g.reshape(-1) for g in grid_index(dims))
+    tw = 2 * np.pi
+    one, zero = np.ones(lmx), np.zeros(lmx)
+    # M[c][a] = d x_c / d xi_a
+    M = [[one, skew * tw / ny * np.cos(tw * y / ny), zero],
+         [zero, one, skew * tw / nz * np.cos(tw * z / nz)],
+         [skew * tw / nx * np.cos(tw * x / nx), zero, one]]
+    u, v, w = ([M[c][a] for c in range(3)] for a in range(3))
+    cross = lambda p, q: [p[1] * q[2] - p[2] * q[1], p[2] * q[0] - p[0] * q[2], p[0] * q[1] - p[1] * q[0]]
+    rows = [cross(v, w), cross(w, u), cross(u, v)]
+    det = u[0] * rows[0][0] + u[1] * rows[0][1] + u[2] * rows[0][2]
+    st["xim"], st["etm"], st["zem"] = (np.concatenate(r) for r in rows)
+    st["yaco"] = 1.0 / det
+    cm = []
+    lxi, let, lze = dims
+    for a, n in enumerate(dims):
+        r = np.array(rows[a]).reshape(3, lze, let, lxi)
+        c = []
+        for face in range(2):
+            idx = face * (n - 1)
+            fr = r[:, :, :, idx] if a == 0 else (r[:, :, idx, :] if a == 1 else r[:, idx, :, :])
+            fr = fr.reshape(3, -1)
+            c.append(fr / np.sqrt((fr * fr).sum(0)))
+        cm.append(np.stack(c).reshape(-1))
+    st["cm"] = cm
+    return st

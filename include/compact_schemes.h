@@ -211,13 +211,14 @@ int cs_rk_stage(const cs_dims* dims, double* q, const double* q0, const double* 
  * The reference evaluates the RHS as calc_viscous_shear_stress + calc_fluxes
  * + gcbc_go (src/physics_pc.hpp:83-227, src/gcbc.hpp:86-186) and leaves the
  * stage update as a stub (driver/canard.cu:132-205, physics_base::movef).
- * cs_solver computes the same quantities with six fused line sweeps per RHS
- * on a uniform Cartesian grid: the metrics are the constant diagonals
+ * cs_solver computes the same quantities with six fused line sweeps per RHS.
+ * By default the grid is uniform and Cartesian: the metrics are the constant diagonals
  * xim = (metric[0],0,0), etm = (0,metric[1],0), zem = (0,0,metric[2]), yaco
  * and ssk (= mu) are constants, cm = sign(yaco * metric[a]) e_a (the unit
  * normal of grad xi_a = yaco * metric[a] e_a; it is -e_a on an axis whose
- * coordinate decreases with the index, as z of src/grid.hpp:103).  Per axis
- * both faces must be
+ * coordinate decreases with the index, as z of src/grid.hpp:103);
+ * cs_solver_set_metrics makes them per-point arrays (curvilinear grids).
+ * Per axis both faces must be
  * non-reflective (10) or both seam codes (35/40/45); walls and curved
  * interfaces use the per-operator API above. */
 typedef struct {
@@ -295,6 +296,19 @@ int cs_solver_autotune(cs_solver* sv, const double* qa, const double* sbcc, int 
  * accumulation stay FP64, and every sweep computes in FP64.  No effect on
  * inviscid solvers. */
 int cs_solver_set_precision(cs_solver* sv, int mode);
+/* curvilinear grid (the fused form of calc_fluxes + calc_viscous_shear_stress
+ * + gcbc_go with full metrics, src/cuda/kernels/physics.hpp:98-191,228-333,
+ * src/cuda/kernels/gcbc.hpp:251-454): device arrays the caller keeps alive
+ * while the solver uses them -- xim, etm, zem [3][lmx] (the metric rows of
+ * xi, eta, zeta as the per-operator calls take them), yaco [lmx], and the face
+ * normals cm0, cm1, cm2 of the x, y and z faces ([2 face][3][face], as
+ * cs_gcbc_go takes them).  Every sweep, the time step and the RK stage then
+ * read them per point instead of cfg.metric / cfg.yaco (ssk stays cfg.mu).
+ * All seven NULL: back to the constants.  Curvilinear solvers keep FP64
+ * intermediates (cs_solver_set_precision 1 is refused) and have no adjoint
+ * (the vjp and tape calls return CS_ERR_UNSUPPORTED). */
+int cs_solver_set_metrics(cs_solver* sv, const double* xim, const double* etm, const double* zem,
+                          const double* yaco, const double* cm0, const double* cm1, const double* cm2);
 /* order of the six sweeps (no reference counterpart): 0 = gradients along
  * x, y, then z with the stress closure, fluxes along x, y, then z with the
  * RHS/stage write; 1 (default) = gradients y, z, x (closure in the x sweep),

@@ -1,0 +1,151 @@
+"""The fused solver on a curvilinear grid (cs_solver_set_metrics): per-point
+metric rows, Jacobian and face normals in every sweep, the time step and the
+RK stage, against the oracle's calc_fluxes + calc_viscous_shear_stress +
+gcbc_go with the same arrays (src/cuda/kernels/physics.hpp:98-191,228-333,
+src/cuda/kernels/gcbc.hpp:251-454).  Grid: x = i + s sin(2 pi j/Ny),
+y = j + s sin(2 pi k/Nz), z = k + s sin(2 pi i/Nx), analytic cofactor
+metrics (every off-diagonal metric term non-zero)."""
+import numpy as np
+import pytest
+
+from gpu_util import case, dev, host, nrel
+
+CL = [[10, 10, 10], [10, 10, 10]]
+PER = [[45, 45, 45], [45, 45, 45]]
+MIX = [[45, 10, 10], [45, 10, 10]]
+
+
+def _state(dims, seed, skew=1.5):
+    from compact_schemes_amd import workloads as wl
+    return wl.curvilinear_state(dims, seed=seed, skew=skew, noise=1e-2, width=3.0)
+
+
+def _oracle_rhs(oracle, dims, nbc, st, visc):
+    lmx = dims[0] * dims[1] * dims[2]
+    ctx = oracle.Context(dims, nbc)
+    de, p = np.zeros(5 * lmx), np.zeros(lmx)
+    ctx.rhs(int(visc), st["qa"], de, p, st["ssk"], st["xim"], st["etm"], st["zem"], st["yaco"], st["cm"],
+            st["sbcc"], st["umf"])
+    return de
+
+
+def test_curvilinear_state_normals():
+    """The helper's face normals are unit vectors along grad(xi_a), and a grid
+    without skew gives the Cartesian metrics and normals exactly."""
+    from compact_schemes_amd import workloads as wl
+    dims = (12, 10, 8)
+    st = wl.curvilinear_state(dims, skew=1.5)
+    for a, c in enumerate(st["cm"]):
+        c = c.reshape(2, 3, -1)
+        assert np.allclose((c * c).sum(1), 1.0, atol=1e-14)
+        assert np.all(c[:, a, :] > 0.5)
+    flat = wl.curvilinear_state(dims, skew=0.0)
+    cart = wl.flow_state(dims)
+    for k in ("xim", "etm", "zem", "yaco"):
+        assert np.array_equal(flat[k], cart[k])
+    assert all(np.array_equal(a, b) for a, b in zip(flat["cm"], cart["cm"]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("visc", [True, False])
+@pytest.mark.parametrize("dims, nbc", [((40, 36, 32), CL), ((40, 36, 32), PER), ((48, 26, 33), MIX),
+                                       ((24, 64, 40), CL)])
+def test_curvilinear_rhs_matches_oracle(oracle, dims, nbc, visc):
+    from compact_schemes_amd.fused import FusedSolver
+    st = _state(dims, seed=12)
+    lmx = dims[0] * dims[1] * dims[2]
+    sv = FusedSolver(dims, nbc, st, viscous=visc, curvilinear=True)
+    q = dev(st["qa"])
+    case(f"curvilinear rhs dims={dims} nbc={nbc[0]} visc={visc}")
+    de = host(sv.rhs(q))
+    ref = _oracle_rhs(oracle, dims, nbc, st, visc)
+    errs = [nrel(de[m * lmx:(m + 1) * lmx], ref[m * lmx:(m + 1) * lmx]) for m in range(5)]
+    print("curvilinear rhs vs oracle:", " ".join(f"{e:.2e}" for e in errs))
+    assert max(errs) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nbc", [CL, MIX])
+def test_curvilinear_rk4_step_matches_oracle(oracle, nbc):
+    """One RK4 step (CFL time step, four stages, filter) against the oracle's
+    step with the same metric arrays: dt to 1e-14, the change of state to 1e-10."""
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (40, 36, 32)
+    st = _state(dims, seed=13)
+    lmx = dims[0] * dims[1] * dims[2]
+    sv = FusedSolver(dims, nbc, st, curvilinear=True)
+    q = dev(st["qa"])
+    case(f"curvilinear rk4 step nbc={nbc[0]}")
+    dt = sv.rk4_step(q, report_dt=True)
+    ctx = oracle.Context(dims, nbc)
+    qo = st["qa"].copy()
+    dto = ctx.rk4_step(1, qo, st["ssk"], st["xim"], st["etm"], st["zem"], st["yaco"], st["cm"], st["sbcc"],
+                       st["umf"])
+    assert abs(dt - dto) / dto < 1e-14
+    qh = host(q)
+    for v in range(5):
+        sl = slice(v * lmx, (v + 1) * lmx)
+        assert nrel(qh[sl] - st["qa"][sl], qo[sl] - st["qa"][sl]) < 1e-10, f"variable {v}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nbc", [CL, MIX])
+def test_curvilinear_with_cartesian_arrays_is_the_cartesian_solver(nbc):
+    """The per-point path fed the constant diagonal metrics and the unit normals
+    e_a gives the constant-metric solver's RHS and RK4 step bit for bit (the
+    off-diagonal products are exact zeros)."""
+    import torch
+    from compact_schemes_amd import workloads as wl
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (32, 28, 24)
+    st = wl.flow_state(dims, seed=14, noise=1e-2, width=3.0)
+    cart = FusedSolver(dims, nbc, st)
+    curv = FusedSolver(dims, nbc, st, curvilinear=True)
+    q = dev(st["qa"])
+    case(f"curvilinear = cartesian nbc={nbc[0]}")
+    a = cart.rhs(q).clone()
+    b = curv.rhs(q).clone()
+    assert torch.equal(a, b)
+    qa, qb = q.clone(), q.clone()
+    assert cart.rk4_step(qa, report_dt=True) == curv.rk4_step(qb, report_dt=True)
+    assert torch.equal(qa, qb)
+
+
+@pytest.mark.gpu
+def test_curvilinear_launch_shapes_agree(oracle):
+    """Every launch shape of the y/z sweeps gives the curvilinear RHS to rounding."""
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (32, 48, 64)
+    st = _state(dims, seed=15)
+    q = dev(st["qa"])
+    base = host(FusedSolver(dims, CL, st, curvilinear=True).rhs(q)).copy()
+    for axis in (1, 2):
+        for shape in ((0, 8, 0), (1, 16, 1), (2, 16, 0)):
+            sv = FusedSolver(dims, CL, st, curvilinear=True)
+            sv.tune(axis, *shape)
+            case(f"curvilinear launch shape axis={axis} shape={shape}")
+            assert nrel(host(sv.rhs(q)), base) < 1e-13
+
+
+@pytest.mark.gpu
+def test_curvilinear_refusals_and_reset():
+    """FP32 intermediates and the adjoint are refused on a curvilinear solver;
+    all-NULL metrics return it to the constants."""
+    import compact_schemes_amd as cs
+    from compact_schemes_amd import workloads as wl
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (24, 20, 18)
+    st = wl.flow_state(dims, seed=16, noise=1e-2, width=3.0)
+    sv = FusedSolver(dims, CL, st, viscous=False, curvilinear=True)
+    q = dev(st["qa"])
+    case("curvilinear refusals")
+    with pytest.raises(cs.CsError):
+        sv.set_precision(1)
+    with pytest.raises(cs.CsError):
+        sv.rhs_vjp(q, q.clone())
+    with pytest.raises(cs.CsError):
+        cs.check(cs.lib().cs_solver_set_metrics(sv.h, cs._p(q), None, None, None, None, None, None))
+    cs.check(cs.lib().cs_solver_set_metrics(sv.h, None, None, None, None, None, None, None))
+    ref = FusedSolver(dims, CL, st, viscous=False)
+    import torch
+    assert torch.equal(sv.rhs(q).clone(), ref.rhs(q).clone())
