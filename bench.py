@@ -46,13 +46,15 @@ def parse():
                          "subdomain and operator) and a plain device copy of known size (the program rocprofv3 "
                          "profiles the batched kernels with)")
     ap.add_argument("--reps", type=int, default=5)
-    ap.add_argument("--workload", default="rhs", choices=["rhs", "plain64", "batch96", "precision", "adjoint"],
+    ap.add_argument("--workload", default="rhs", choices=["rhs", "plain64", "batch96", "precision", "adjoint", "curv256"],
                     help="rhs: the 256^3 viscous RK4 step (default); plain64: derivative x/y/z + filter x/y/z "
                          "of a periodic 64^3 block, against the analytic derivative and the CPU oracle; batch96: "
                          "derivative x/y/z + filter "
                          "x/y/z of 96 periodic subdomains with seeded sizes in [48, 320]; precision: 50 RK4 "
                          "steps at 512x512x256 with FP64 and with FP32+refinement line solves; adjoint: gradient "
-                         "of an acoustic-energy loss after 40 Euler RK4 steps at 128^3 (checkpointed adjoint)")
+                         "of an acoustic-energy loss after 40 Euler RK4 steps at 128^3 (checkpointed adjoint); curv256: the "
+                         "RHS workload on a skewed curvilinear 256^3 grid, fused sweeps with per-point metrics "
+                         "against the per-operator path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-reps", type=int, default=3)
     return ap.parse_args()
@@ -101,6 +103,8 @@ def main():
         return precision_bench(args)
     if args.workload == "adjoint":
         return adjoint_bench(args)
+    if args.workload == "curv256":
+        return curvilinear_bench(args)
     n = args.n
     dims = (n, n, n)
     lmx = n ** 3
@@ -243,6 +247,67 @@ PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_rk_stage_256.txt")
 RHS_BYTES_PER_PT = 736                     # six sweeps of one viscous RHS
 STEP_BYTES_PER_PT = 4 * 776 + 3 * 80 + 40  # RK4 step: four stages, filter x/y/z, time step
 PROFILE_FP64 = os.path.join(ROOT, "profiles", "pmc_fp64_rk_stage_256.txt")
+
+
+# credited bytes of one curvilinear RHS: the 736 of the constant-metric RHS plus
+# the closure's nine metrics and Jacobian (80), each flux sweep's metric row
+# (24) and the five stress and heat-flux components beyond its own row (40)
+CURV_RHS_BYTES_PER_PT = RHS_BYTES_PER_PT + 80 + 3 * (24 + 40)
+
+
+def curvilinear_bench(args, n=256):
+    """The RHS workload (viscous, Mach 0.5 with the acoustic pulse,
+    non-reflective faces) on the skewed grid x = i + 1.5 sin(2 pi j/n), ...
+    with its analytic metrics: one RK4 step and one RHS of the fused solver
+    with per-point metrics (cs_solver_set_metrics), and one RHS of the
+    per-operator path (calc_viscous_shear_stress + calc_fluxes + gcbc_go of
+    the same arrays).  value = RHS grid points per second per RK4 step."""
+    import torch
+    from compact_schemes_amd import workloads as wl
+    from compact_schemes_amd.fused import FusedSolver
+    from compact_schemes_amd.stepper import Stepper
+    dims = (n, n, n)
+    lmx = n ** 3
+    st = wl.curvilinear_state(dims, seed=2, skew=1.5)
+    q0 = torch.from_numpy(st["qa"]).cuda()
+    q = q0.clone()
+    sv = FusedSolver(dims, wl.NBC_NON_REFLECTIVE, st, curvilinear=True)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps):
+        for _ in range(args.warmup):
+            fn()
+        q.copy_(q0)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+    ms_step = timed(lambda: sv.rk4_step(q), args.steps)
+    assert torch.isfinite(q).all().item(), "non-finite state after the timed steps"
+    ms_rhs = timed(lambda: sv.rhs(q), 5)
+    ref = Stepper(dims, wl.NBC_NON_REFLECTIVE, True, st["umf"], st)
+    ms_ref = timed(lambda: ref.rhs(q), 3)
+    gbs = CURV_RHS_BYTES_PER_PT * lmx / (ms_rhs * 1e-3) / 1e9
+    line = {
+        "metric": "curvilinear RHS Gpts/s at 256^3 FP64",
+        "value": round(4.0 * lmx / (ms_step * 1e-3) / 1e9, 4),
+        "unit": "Gpts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded acoustic pulse on uniform Mach 0.5 flow, sine-skewed grid, analytic metrics)",
+        "config": {"workload": "viscous compressible RHS, 256^3 FP64 curvilinear grid, non-reflective gcbc on all "
+                               "faces, one RK4 step (4 RHS + stage updates + GV10 filter)",
+                   "grid": [n, n, n], "path": "fused, per-point metrics", "timing": "eager launches",
+                   "ms_per_rhs": round(ms_rhs, 3), "rhs_gpts_s_rhs_only": round(lmx / (ms_rhs * 1e-3) / 1e9, 4),
+                   "per_operator_ms_per_rhs": round(ms_ref, 3),
+                   "fused_speedup_over_per_operator": round(ms_ref / ms_rhs, 3)},
+        "roofline": {"whole_rhs": {"bytes_per_pt": CURV_RHS_BYTES_PER_PT, "ms": round(ms_rhs, 4),
+                                   "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}},
+    }
+    print(json.dumps(line), flush=True)
 
 
 def plain_bench(args, n=64):
