@@ -48,11 +48,34 @@ __device__ __forceinline__ void flux_jt(const double* q, const double* nu, doubl
     g[4] = nu[4] * ss + cp * kGamm1;
 }
 
+// the same for a general metric row m (curvilinear grids): ss = m.(u + uf),
+// F_1+i = (rho u_i) ss + m_i p, F_4 = (E + p) ss - p (uf.m); with mu = m.u,
+//   g_0 = nu_0 (m.uf) - cs mu / rho + cp (gamma-1)|u|^2/2,
+//   g_1+i = nu_1+i ss + nu_0 m_i + cs m_i / rho - cp (gamma-1) u_i,
+//   g_4 = nu_4 ss + cp (gamma-1),
+// cs = nu_1..3 . (rho u) + nu_4 (E + p), cp = nu_1..3 . m + nu_4 mu
+__device__ __forceinline__ void flux_jt_m(const double* q, const double* nu, const double* m, cs_point umf,
+                                          double* g) {
+    const Prim p = primitive(q);
+    const double u[3] = {p.u, p.v, p.w};
+    const double mu = m[0] * p.u + m[1] * p.v + m[2] * p.w;
+    const double mf = m[0] * umf.x + m[1] * umf.y + m[2] * umf.z;
+    const double ss = m[0] * (p.u + umf.x) + m[1] * (p.v + umf.y) + m[2] * (p.w + umf.z);
+    const double cs = nu[1] * q[1] + nu[2] * q[2] + nu[3] * q[3] + nu[4] * (q[4] + p.p);
+    const double cp = m[0] * nu[1] + m[1] * nu[2] + m[2] * nu[3] + nu[4] * mu;
+    g[0] = nu[0] * mf - cs * (mu * p.rhoi) + cp * (kGamm1 * 0.5 * (p.u * p.u + p.v * p.v + p.w * p.w));
+#pragma unroll
+    for (int i = 0; i < 3; i++) g[1 + i] = nu[1 + i] * ss + nu[0] * m[i] + cs * (m[i] * p.rhoi) - cp * (kGamm1 * u[i]);
+    g[4] = nu[4] * ss + cp * kGamm1;
+}
+
 // AT: the transposed flux-divergence sweep along AX of one stage.
 // MODE 0: gout = J^T nu (+ G); 1: gout = gin + J^T nu (+ G); 2 (last sweep of
 // stages 4..2): tn = -ck (gin + J^T nu + G), tout = tn, lam += tn; 3 (last
 // sweep of stage 1): lam += -ck (gin + J^T nu + G)
-template <int AX, int MODE>
+// CURV: the point's metric row (curvilinear solvers; their per-point
+// Jacobian scales the seed before the stage, see backward_fused)
+template <int AX, int MODE, bool CURV = false>
 struct AT {
     static constexpr int NF = 5;
     static constexpr int kWpe = 3;        // waves per SIMD of the y/z kernels (168 VGPRs)
@@ -66,6 +89,8 @@ struct AT {
     long nface;
     double ck;
     double m, uf;
+    cs_point umf;
+    Metric mt;
     __device__ __forceinline__ void seed(uoff bo, double* v) const {
 #pragma unroll
         for (int f = 0; f < 5; f++) v[f] = ldg(t[f], bo);
@@ -76,9 +101,15 @@ struct AT {
     // compiler knows, so a load placed after a store waits for it.
     __device__ __forceinline__ void fetch(uoff bo, double* d, int face, long line) const {
         double qq[5], g[5];
+        const double c = ck;
 #pragma unroll
         for (int v = 0; v < 5; v++) qq[v] = ldg(q[v], bo);
-        flux_jt<AX>(qq, d, m, uf, g);
+        if constexpr (CURV) {
+            const double mr[3] = {ldg(mt.m[AX][0], bo), ldg(mt.m[AX][1], bo), ldg(mt.m[AX][2], bo)};
+            flux_jt_m(qq, d, mr, umf, g);
+        } else {
+            flux_jt<AX>(qq, d, m, uf, g);
+        }
         if (G && face >= 0) {
 #pragma unroll
             for (int v = 0; v < 5; v++) g[v] += G[(face * 5 + v) * nface + line];
@@ -87,8 +118,8 @@ struct AT {
         for (int v = 0; v < 5; v++) {
             if (MODE == 0) d[v] = g[v];
             else if (MODE == 1) d[v] = ldg(gin[v], bo) + g[v];
-            else if (MODE == 2) d[v] = -ck * (ldg(gin[v], bo) + g[v]);
-            else d[v] = ldg(lam[v], bo) - ck * (ldg(gin[v], bo) + g[v]);
+            else if (MODE == 2) d[v] = -c * (ldg(gin[v], bo) + g[v]);
+            else d[v] = ldg(lam[v], bo) - c * (ldg(gin[v], bo) + g[v]);
         }
     }
     __device__ __forceinline__ void store(uoff bo, const double* d) const {
@@ -514,7 +545,7 @@ __global__ void __launch_bounds__(TL * kMaxP) __attribute__((amdgpu_waves_per_eu
 __global__ void __launch_bounds__(256) face_adjoint_kernel(const double* q, const double* lam, const double* sbcc,
                                                            const double* drv, double* mu, double* G, int ax, int N,
                                                            long nface, long lxi, long plane, long st, EulerPhys ph,
-                                                           long lmx) {
+                                                           long lmx, const double* cmf, const double* yaco) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 2 * nface) return;
     const int face = (int)(t / nface);
@@ -538,12 +569,21 @@ __global__ void __launch_bounds__(256) face_adjoint_kernel(const double* q, cons
         qq[v] = q[pt + v * lmx];
         gd[v] = drv[((long)face * 5 + v) * nface + f];
     }
+    // face normal and Jacobian: the caller's arrays (curvilinear solvers) or the constants
+    double cm[3], J = ph.J;
+    if (yaco) {
+        for (int k = 0; k < 3; k++) cm[k] = cmf[((long)face * 3 + k) * nface + f];
+        J = yaco[pt];
+    } else {
+        const double sg = ph.m[ax] * ph.J < 0.0 ? -1.0 : 1.0;
+        for (int k = 0; k < 3; k++) cm[k] = k == ax ? sg : 0.0;
+    }
     typedef Dual<5> D;
     {
         D gdd[5], cr[5];
 #pragma unroll
         for (int v = 0; v < 5; v++) gdd[v] = D::seed(gd[v], v);
-        face_corr<double, D>(ax, face, qq, gdd, ph, cr);
+        face_corr_at<double, D>(face, qq, gdd, cm, J, ph.umf, cr);
         D phi(0.0);
 #pragma unroll
         for (int v = 0; v < 5; v++) phi = phi + cr[v] * lf[v];
@@ -557,7 +597,7 @@ __global__ void __launch_bounds__(256) face_adjoint_kernel(const double* q, cons
             qd[v] = D::seed(qq[v], v);
             gdd[v] = D(gd[v]);
         }
-        face_corr<D, D>(ax, face, qd, gdd, ph, cr);
+        face_corr_at<D, D>(face, qd, gdd, cm, J, ph.umf, cr);
         D phi(0.0);
 #pragma unroll
         for (int v = 0; v < 5; v++) phi = phi + cr[v] * lf[v];
@@ -651,12 +691,12 @@ int filter_t_axis(cs_solver* sv, double* lam, const SweepCommon& cm, hipStream_t
     return launch_t<AX, 1>(ax, cm, f, sv->fttabs[AX], nullptr, an, s);
 }
 
-template <int AX, int MODE>
-int stage_sweep(cs_solver* sv, const SweepCommon& cm, const double* q, const double* t, const double* gin,
-                double* gout, double* lam, double* tout, double ck, const double* fb, hipStream_t s) {
+template <int AX, int MODE, bool CURV>
+int stage_sweep_t(cs_solver* sv, const SweepCommon& cm, const double* q, const double* t, const double* gin,
+                  double* gout, double* lam, double* tout, double ck, const double* fb, hipStream_t s) {
     const AxisDesc& ax = sv->ax[AX];
     const long lmx = sv->g.lmx;
-    AT<AX, MODE> a{};
+    AT<AX, MODE, CURV> a{};
     fields(q, lmx, a.q);
     fields(t, lmx, a.t);
     fields(gin, lmx, a.gin);
@@ -672,7 +712,19 @@ int stage_sweep(cs_solver* sv, const SweepCommon& cm, const double* q, const dou
     a.ck = ck;
     a.m = sv->cfg.metric[AX];
     a.uf = uf_of(sv, AX);
+    a.umf = sv->cfg.umf;
+    if constexpr (CURV) a.mt = sv->met;
     return launch_t<AX, 0>(ax, cm, a, sv->ttabs[AX], mu, nullptr, s);
+}
+
+// ck = dt/div times the constant Jacobian; curvilinear solvers pass a seed
+// already scaled by the point's Jacobian (backward_fused) and ck = dt/div
+template <int AX, int MODE>
+int stage_sweep(cs_solver* sv, const SweepCommon& cm, const double* q, const double* t, const double* gin,
+                double* gout, double* lam, double* tout, double ck, const double* fb, hipStream_t s) {
+    if (sv->curv) return stage_sweep_t<AX, MODE, true>(sv, cm, q, t, gin, gout, lam, tout, ck, fb, s);
+    // (constant Jacobian: scaling the seed or the result is the same)
+    return stage_sweep_t<AX, MODE, false>(sv, cm, q, t, gin, gout, lam, tout, ck * sv->cfg.yaco, fb, s);
 }
 
 // face seeds of every non-reflective axis for the RHS evaluated at q with
@@ -687,10 +739,22 @@ int face_adjoints(cs_solver* sv, const double* q, const double* t, const double*
         double* mu = fb + 2 * face_offset(sv->g, a);
         hipLaunchKernelGGL(face_adjoint_kernel, dim3((unsigned)((2 * nf + 255) / 256)), dim3(256), 0, s, q, t, sbcc,
                            faces + face_offset(sv->g, a), mu, mu + 10 * nf, a, ax.N, nf, (long)sv->g.lxi,
-                           (long)sv->g.lxi * sv->g.let, ax.st, ph, sv->g.lmx);
+                           (long)sv->g.lxi * sv->g.let, ax.st, ph, sv->g.lmx, sv->curv ? sv->met.cm[a] : nullptr,
+                           sv->curv ? sv->met.J : nullptr);
         CS_HIP(hipGetLastError());
     }
     return CS_OK;
+}
+
+// out = J t, five fields: the seed of a stage's transposed RHS on a
+// curvilinear grid (stage k adds -dt/div * J(x) R(q) at each point x, so its
+// adjoint applies R^T to J t, the Jacobian on the output side of R)
+__global__ void __launch_bounds__(256) scale_seed_kernel(const double* t, const double* J, double* out, long lmx) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= lmx) return;
+    const double j = J[i];
+#pragma unroll
+    for (int v = 0; v < 5; v++) out[i + v * lmx] = j * t[i + v * lmx];
 }
 
 // lam <- step'(q0)^T lam from a tape (fused sweeps)
@@ -707,11 +771,18 @@ int backward_fused(cs_solver* sv, const double* tape, double* lam, double dt, co
         if ((rc = filter_t_axis<1>(sv, lam, cm, s))) return rc;
         if ((rc = filter_t_axis<0>(sv, lam, cm, s))) return rc;
     }
-    const double J = sv->cfg.yaco;
     for (int k = 4; k >= 1; k--) {
         const double* q = tape + (long)(k - 1) * n5;
         const double* t = k == 4 ? lam : tb;
-        const double ck = dt / (double)(5 - k) * J;
+        const double ck = dt / (double)(5 - k);   // times the Jacobian in stage_sweep
+        if (sv->curv) {
+            // the seed scaled by the point's Jacobian, in the stage buffer qc
+            // (unused outside cs_solver_rk4_step)
+            hipLaunchKernelGGL(scale_seed_kernel, dim3((unsigned)((lmx + 255) / 256)), dim3(256), 0, s, t, sv->met.J,
+                               sv->qc, lmx);
+            CS_HIP(hipGetLastError());
+            t = sv->qc;
+        }
         if ((rc = face_adjoints(sv, q, t, faces + (long)(k - 1) * F, sbcc, fb, s))) return rc;
         if ((rc = stage_sweep<2, 0>(sv, cm, q, t, nullptr, gb, nullptr, nullptr, 0.0, fb, s))) return rc;
         if ((rc = stage_sweep<1, 1>(sv, cm, q, t, gb, gb, nullptr, nullptr, 0.0, fb, s))) return rc;
@@ -824,7 +895,8 @@ int cs_solver_rk4_step_tape(cs_solver* sv, double* qa, const double* sbcc, doubl
     if (!sv || !qa || !tape) return cs::fail(CS_ERR_ARG, "null argument");
     if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
         return cs::fail(CS_ERR_ARG, "non-reflective faces need sbcc");
-    if (sv->curv) return cs::fail(CS_ERR_UNSUPPORTED, "the adjoint covers solvers with constant metrics");
+    if (sv->curv && !cs::fused::adjoint_fused(sv))
+        return cs::fail(CS_ERR_UNSUPPORTED, "curvilinear adjoints run the fused transposed sweeps (8-point chunks, mode 1)");
     hipStream_t s = (hipStream_t)stream;
     int rc = cs::fused::rk4_step_tape(sv, qa, sbcc, dt, tape, s);
     if (rc || !dt_out) return rc;
@@ -839,7 +911,8 @@ int cs_solver_rk4_vjp_tape(cs_solver* sv, const double* tape, double* lam, doubl
     if (!sv || !tape || !lam) return cs::fail(CS_ERR_ARG, "null argument");
     if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
         return cs::fail(CS_ERR_ARG, "non-reflective faces need sbcc");
-    if (sv->curv) return cs::fail(CS_ERR_UNSUPPORTED, "the adjoint covers solvers with constant metrics");
+    if (sv->curv && !cs::fused::adjoint_fused(sv))
+        return cs::fail(CS_ERR_UNSUPPORTED, "curvilinear adjoints run the fused transposed sweeps (8-point chunks, mode 1)");
     return cs::fused::rk4_vjp_tape(sv, tape, lam, dt, sbcc, (hipStream_t)stream);
 }
 

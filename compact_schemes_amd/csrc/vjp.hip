@@ -271,7 +271,8 @@ int cs_solver_rhs_vjp(cs_solver* sv, const double* qa, const double* lam, double
     if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
         return cs::fail(CS_ERR_ARG, "non-reflective faces need sbcc");
     if (sv->cfg.viscous) return cs::fail(CS_ERR_UNSUPPORTED, "the adjoint covers the inviscid right-hand side");
-    if (sv->curv) return cs::fail(CS_ERR_UNSUPPORTED, "the adjoint covers solvers with constant metrics");
+    if (sv->curv && !cs::fused::adjoint_fused(sv))
+        return cs::fail(CS_ERR_UNSUPPORTED, "curvilinear adjoints run the fused transposed sweeps (8-point chunks, mode 1)");
     if (cs::fused::adjoint_fused(sv)) return cs::fused::rhs_vjp_fused(sv, qa, lam, g, sbcc, (hipStream_t)stream);
     return cs::fused::solver_rhs_vjp(sv, qa, lam, g, sbcc, (hipStream_t)stream);
 }
@@ -281,7 +282,8 @@ int cs_solver_rk4_vjp(cs_solver* sv, const double* q0, double* lam, double dt, c
     if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
         return cs::fail(CS_ERR_ARG, "non-reflective faces need sbcc");
     if (sv->cfg.viscous) return cs::fail(CS_ERR_UNSUPPORTED, "the adjoint covers the inviscid right-hand side");
-    if (sv->curv) return cs::fail(CS_ERR_UNSUPPORTED, "the adjoint covers solvers with constant metrics");
+    if (sv->curv && !cs::fused::adjoint_fused(sv))
+        return cs::fail(CS_ERR_UNSUPPORTED, "curvilinear adjoints run the fused transposed sweeps (8-point chunks, mode 1)");
     if (!(dt > 0.0)) return cs::fail(CS_ERR_ARG, "the adjoint needs the step's dt > 0");
     if (cs::fused::adjoint_fused(sv)) return cs::fused::rk4_vjp_fused(sv, q0, lam, dt, sbcc, (hipStream_t)stream);
     return cs::fused::solver_rk4_vjp(sv, q0, lam, dt, sbcc, (hipStream_t)stream);

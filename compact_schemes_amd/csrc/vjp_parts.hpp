@@ -30,17 +30,17 @@ __device__ __forceinline__ void flux_row(int ax, const T* q, double m, cs_point 
     F[4] = (q[4] + p.p) * ss - p.p * (uf * m);
 }
 
-// characteristic correction of a face point (the S2 sweep's gcbc())
+// characteristic correction of a face point (the S2 sweep's gcbc()) with
+// the face normal cm and the point's Jacobian J
 template <class T, class X>
-__device__ __forceinline__ void face_corr(int ax, int face, const T* q, const X* drv, const EulerPhys& ph, X* corr) {
+__device__ __forceinline__ void face_corr_at(int face, const T* q, const X* drv, const double* cm, double J,
+                                             cs_point umf, X* corr) {
     PrimT<T> p = primitive(q);
-    const double sg = ph.m[ax] * ph.J < 0.0 ? -1.0 : 1.0;   // unit normal of grad(xi_ax) = J * metric * e_ax
-    const double cm[3] = {ax == 0 ? sg : 0.0, ax == 1 ? sg : 0.0, ax == 2 ? sg : 0.0};
     EigT<T> e;
-    e.eleme(cm, q[0], q + 1, p.p, ph.umf);
+    e.eleme(cm, q[0], q + 1, p.p, umf);
     X cha[5], ch[5];
 #pragma unroll
-    for (int v = 0; v < 5; v++) cha[v] = ph.J * drv[v];
+    for (int v = 0; v < 5; v++) cha[v] = J * drv[v];
     e.l_apply(cm, cha, ch);
     const double ra0 = 1 - 2 * face;
     X dl[5] = {X(0.0), X(0.0), X(0.0), X(0.0), X(0.0)};
@@ -48,7 +48,15 @@ __device__ __forceinline__ void face_corr(int ax, int face, const T* q, const X*
     if (ra0 * value_of(e.vn + e.vs - e.ao) > 0) dl[4] = -ch[4];
     e.r_apply(cm, dl, corr);
 #pragma unroll
-    for (int v = 0; v < 5; v++) corr[v] = corr[v] / ph.J;   // D25: de changes by R*cha / J
+    for (int v = 0; v < 5; v++) corr[v] = corr[v] / J;   // D25: de changes by R*cha / J
+}
+
+// the same with the constant metrics of a Cartesian solver
+template <class T, class X>
+__device__ __forceinline__ void face_corr(int ax, int face, const T* q, const X* drv, const EulerPhys& ph, X* corr) {
+    const double sg = ph.m[ax] * ph.J < 0.0 ? -1.0 : 1.0;   // unit normal of grad(xi_ax) = J * metric * e_ax
+    const double cm[3] = {ax == 0 ? sg : 0.0, ax == 1 ? sg : 0.0, ax == 2 ? sg : 0.0};
+    face_corr_at<T, X>(face, q, drv, cm, ph.J, ph.umf, corr);
 }
 
 }  // namespace fused

@@ -129,8 +129,9 @@ def test_curvilinear_launch_shapes_agree(oracle):
 
 @pytest.mark.gpu
 def test_curvilinear_refusals_and_reset():
-    """FP32 intermediates and the adjoint are refused on a curvilinear solver;
-    all-NULL metrics return it to the constants."""
+    """FP32 intermediates, and the adjoint on lines without 8-point chunks, are
+    refused on a curvilinear solver; all-NULL metrics return it to the
+    constants."""
     import compact_schemes_amd as cs
     from compact_schemes_amd import workloads as wl
     from compact_schemes_amd.fused import FusedSolver
@@ -141,7 +142,7 @@ def test_curvilinear_refusals_and_reset():
     case("curvilinear refusals")
     with pytest.raises(cs.CsError):
         sv.set_precision(1)
-    with pytest.raises(cs.CsError):
+    with pytest.raises(cs.CsError):          # 20 and 18 points: no 8-point chunks for the adjoint
         sv.rhs_vjp(q, q.clone())
     with pytest.raises(cs.CsError):
         cs.check(cs.lib().cs_solver_set_metrics(sv.h, cs._p(q), None, None, None, None, None, None))
@@ -149,3 +150,110 @@ def test_curvilinear_refusals_and_reset():
     ref = FusedSolver(dims, CL, st, viscous=False)
     import torch
     assert torch.equal(sv.rhs(q).clone(), ref.rhs(q).clone())
+
+
+def _euler(dims, nbc, seed, filt=True):
+    from compact_schemes_amd.fused import FusedSolver
+    st = _state(dims, seed)
+    return st, FusedSolver(dims, nbc, st, viscous=False, filt=filt, curvilinear=True)
+
+
+CONFIG5 = [[45, 10, 10], [45, 10, 10]]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nbc", [CONFIG5, CL, PER], ids=["config5", "closure", "periodic"])
+def test_curvilinear_rhs_vjp_matches_finite_differences(nbc):
+    """<lam, dR/dq v> by fourth-order central differences of the curvilinear
+    RHS against <(dR/dq)^T lam, v> from the fused transposed sweeps, for four
+    seeded directions."""
+    import torch
+    dims = (24, 32, 16)
+    st, sv = _euler(dims, nbc, 41)
+    rng = np.random.default_rng(42)
+    n = st["qa"].size
+    q0 = dev(st["qa"])
+    lam = dev(rng.standard_normal(n))
+    case(f"curvilinear rhs vjp fd nbc={nbc[0]}")
+    g = sv.rhs_vjp(q0, lam).clone()
+
+    def R(q):
+        return sv.rhs(q).clone()
+    eps = 1e-4
+    for k in range(4):
+        v = dev(rng.standard_normal(n))
+        d = (-R(q0 + 2 * eps * v) + 8 * R(q0 + eps * v) - 8 * R(q0 - eps * v) + R(q0 - 2 * eps * v)) / (12 * eps)
+        fd = float(torch.dot(lam, d))
+        ad = float(torch.dot(g, v))
+        assert abs(fd - ad) <= 1e-8 * abs(ad), (k, fd, ad)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nbc", [CONFIG5, CL], ids=["config5", "closure"])
+def test_curvilinear_rk4_vjp_and_tape(nbc):
+    """The adjoint of one curvilinear RK4 step (dt fixed, filter on) against
+    central differences; the recorded tape reproduces the step and the
+    adjoint bit for bit."""
+    import torch
+    dims = (24, 16, 32)
+    st, sv = _euler(dims, nbc, 43)
+    rng = np.random.default_rng(44)
+    n = st["qa"].size
+    q0 = dev(st["qa"])
+    dt = sv.time_step(q0)
+    lam = dev(rng.standard_normal(n))
+    v = dev(rng.standard_normal(n))
+
+    def step(q):
+        x = q.clone()
+        sv.rk4_step(x, dt=dt)
+        return x
+    case(f"curvilinear rk4 vjp nbc={nbc[0]}")
+    g = sv.rk4_vjp(q0, lam.clone(), dt).clone()
+    eps = 1e-6
+    fd = float(torch.dot(lam, (step(q0 + eps * v) - step(q0 - eps * v)) / (2 * eps)))
+    ad = float(torch.dot(g, v))
+    assert abs(fd - ad) <= 1e-7 * abs(ad), (fd, ad)
+    tape = sv.new_tape()
+    q = q0.clone()
+    assert sv.rk4_step_tape(q, tape, dt=dt, report_dt=True) == dt
+    assert torch.equal(q, step(q0))
+    assert torch.equal(sv.rk4_vjp_tape(tape, lam.clone(), dt).clone(), g)
+
+
+@pytest.mark.gpu
+def test_curvilinear_adjoint_with_cartesian_arrays():
+    """Fed the constant diagonal metrics and unit normals, the per-point
+    adjoint equals the constant-metric adjoint to rounding (the general
+    metric-row Jacobian sums its terms in another order)."""
+    from compact_schemes_amd import workloads as wl
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (24, 16, 32)
+    st = wl.flow_state(dims, seed=45, noise=1e-2, width=3.0)
+    cart = FusedSolver(dims, CONFIG5, st, viscous=False)
+    curv = FusedSolver(dims, CONFIG5, st, viscous=False, curvilinear=True)
+    rng = np.random.default_rng(46)
+    q0 = dev(st["qa"])
+    lam = dev(rng.standard_normal(st["qa"].size))
+    dt = cart.time_step(q0)
+    case("curvilinear adjoint = cartesian")
+    assert nrel(host(curv.rhs_vjp(q0, lam)), host(cart.rhs_vjp(q0, lam))) < 1e-13
+    assert nrel(host(curv.rk4_vjp(q0, lam.clone(), dt)), host(cart.rk4_vjp(q0, lam.clone(), dt))) < 1e-13
+
+
+@pytest.mark.gpu
+def test_curvilinear_adjoint_needs_fused_sweeps():
+    """Lines that do not split into 8-point chunks (or adjoint mode 0) would
+    need the per-operator adjoint, which has no per-point metrics: refused."""
+    import compact_schemes_amd as cs
+    dims = (24, 20, 18)
+    st, sv = _euler(dims, CL, 47)
+    q = dev(st["qa"])
+    case("curvilinear adjoint refusal")
+    with pytest.raises(cs.CsError):
+        sv.rhs_vjp(q, q.clone())
+    st, sv = _euler((24, 16, 32), CL, 48)
+    sv.set_adjoint_mode(0)
+    q = dev(st["qa"])
+    with pytest.raises(cs.CsError):
+        sv.rhs_vjp(q, q.clone())
