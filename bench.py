@@ -55,6 +55,8 @@ def parse():
                          "of an acoustic-energy loss after 40 Euler RK4 steps at 128^3 (checkpointed adjoint); curv256: the "
                          "RHS workload on a skewed curvilinear 256^3 grid, fused sweeps with per-point metrics "
                          "against the per-operator path")
+    ap.add_argument("--curvilinear", action="store_true",
+                    help="--workload adjoint on the sine-skewed curvilinear grid (per-point metrics)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-reps", type=int, default=3)
     return ap.parse_args()
@@ -572,7 +574,8 @@ def adjoint_bench(args, n=128, nsteps=40, every=8, plane=96, ndir=32):
     from compact_schemes_amd import workloads as wl
     from compact_schemes_amd.rollout import AcousticRollout, checkpointed_vjp
     NBC5 = [[45, 10, 10], [45, 10, 10]]
-    ro = AcousticRollout((n, n, n), NBC5, nsteps=nsteps, every=every, plane=plane)
+    curv = bool(getattr(args, "curvilinear", False))
+    ro = AcousticRollout((n, n, n), NBC5, nsteps=nsteps, every=every, plane=plane, curvilinear=curv)
     sv = ro.solver
     q0 = ro.initial(ro.rho1)
     tapes = [sv.new_tape() for _ in range(every)]
@@ -638,7 +641,8 @@ def adjoint_bench(args, n=128, nsteps=40, every=8, plane=96, ndir=32):
                 "median_rel_err": float(np.median(rel)),
                 "normwise_rel_err": float(np.max(np.abs(fd - ad)) / np.max(np.abs(ad)))}
     fd_full = fd_summary(ro, 1e-5)
-    fd_per = fd_summary(AcousticRollout((n, n, n), wl.NBC_PERIODIC, nsteps=nsteps, every=every, plane=plane), 1e-5)
+    fd_per = fd_summary(AcousticRollout((n, n, n), wl.NBC_PERIODIC, nsteps=nsteps, every=every, plane=plane,
+                                        curvilinear=curv), 1e-5)
     line = {
         "metric": "adjoint/forward cost ratio, 40 Euler RK4 steps at 128^3 (checkpoints every 8)",
         "value": round(t_adj / t_fwd, 3), "unit": "ratio", "n_gpus": 1, "steps": nsteps, "warmup": 1,
@@ -646,7 +650,8 @@ def adjoint_bench(args, n=128, nsteps=40, every=8, plane=96, ndir=32):
         "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (uniform Mach 0.5 flow + seeded isentropic density pulse, amplitude 1e-4)",
         "config": {"workload": "Euler 128^3, periodic x, non-reflective y/z, GV10 filter, 40 RK4 steps, loss = "
-                               "acoustic energy on the plane z=96, gradient w.r.t. the initial density perturbation",
+                               "acoustic energy on the plane z=96, gradient w.r.t. the initial density perturbation"
+                               + (", sine-skewed curvilinear grid (per-point metrics)" if curv else ""),
                    "loss": L0, "dt": ro.dt, "forward_s": round(t_fwd, 4), "adjoint_s": round(t_adj, 4),
                    "adjoint": "per segment: the steps re-run with cs_solver_rk4_step_tape, then "
                               "cs_solver_rk4_vjp_tape backwards (fused transposed sweeps); best of 3",

@@ -42,23 +42,27 @@ def checkpointed_vjp(solver, q0, nsteps, every, dt, loss_grad, tapes=None):
 class AcousticRollout:
     """The acoustic-energy loss on the plane z = `plane` after `nsteps` RK4
     steps of inviscid flow (faces `nbc`, dt fixed from the CFL limit of the
-    initial state), as a function of the initial density perturbation."""
+    initial state), as a function of the initial density perturbation.
+    curvilinear=True runs it on the sine-skewed grid of
+    workloads.curvilinear_state (per-point metrics in every sweep)."""
 
-    def __init__(self, dims, nbc, nsteps=40, every=8, plane=96, amp=1e-4, seed=5, cfl=0.5):
+    def __init__(self, dims, nbc, nsteps=40, every=8, plane=96, amp=1e-4, seed=5, cfl=0.5, curvilinear=False,
+                 skew=1.5):
         import torch
         from . import workloads as wl
         from .fused import FusedSolver
         self.dims = tuple(dims)
         self.lmx = dims[0] * dims[1] * dims[2]
         self.nsteps, self.every, self.plane = nsteps, every, plane
-        st = wl.flow_state(dims, seed=seed, amp=amp)
+        st = (wl.curvilinear_state(dims, seed=seed, skew=skew, amp=amp) if curvilinear
+              else wl.flow_state(dims, seed=seed, amp=amp))
         st0 = wl.flow_state(dims, seed=seed, amp=0.0)
         self.base = torch.from_numpy(st0["qa"]).cuda()
         lmx = self.lmx
         self.rho1 = torch.from_numpy(st["qa"][:lmx] - st0["qa"][:lmx]).cuda()     # rho' of the pulse
         self.gam = wl.GAM
         self.p0, self.rho0, self.c0 = 1.0 / self.gam, 1.0, 1.0
-        self.solver = FusedSolver(dims, nbc, st, viscous=False, cfl=cfl)
+        self.solver = FusedSolver(dims, nbc, st, viscous=False, cfl=cfl, curvilinear=curvilinear)
         self.dt = self.solver.time_step(self.initial(self.rho1))
         self.tapes = None
 
