@@ -200,6 +200,11 @@ def main():
     if "all" in kern:
         roofline["sweeps"] = kern["all"]
     fl = fp64_flops_from_profile(roofline["kernel"], n) if args.path == "fused" else None
+    if roofline.get("traffic") is not None or fl is not None:
+        # the counter figures come from the committed summaries: say whether
+        # they were measured with these kernel sources
+        roofline["profile_matches_sources"] = (profile_digest(PROFILE_TRAFFIC) == source_digest() and
+                                               profile_digest(PROFILE_FP64) == source_digest())
     if fl is not None:
         # the second roof: FP64 vector rate of the same kernel (counter flops / HIP-event time)
         ach = fl / (kern["ms"] * 1e-3) / 1e12
@@ -675,6 +680,34 @@ def fp64_flops_from_profile(kernel, n):
         parts = line.strip().split("\t")
         if len(parts) == 3 and parts[0] == "fp64_flops_per_launch" and parts[1] == kernel:
             return float(parts[2])
+    return None
+
+
+def source_digest():
+    """sha256 of the kernel sources (compact_schemes_amd/csrc/*.hip, *.hpp and
+    include/compact_schemes.h, in name order): the committed counter summaries
+    record the digest of the sources they were measured with."""
+    import glob
+    import hashlib
+    h = hashlib.sha256()
+    files = sorted(glob.glob(os.path.join(ROOT, "compact_schemes_amd", "csrc", "*.hip")) +
+                   glob.glob(os.path.join(ROOT, "compact_schemes_amd", "csrc", "*.hpp")))
+    files.append(os.path.join(ROOT, "include", "compact_schemes.h"))
+    for f in files:
+        h.update(os.path.basename(f).encode())
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def profile_digest(path):
+    """the source digest a counter summary records, or None"""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.strip().split("\t")
+        if len(parts) == 2 and parts[0] == "source_sha256":
+            return parts[1]
     return None
 
 
