@@ -160,6 +160,25 @@ def with_spacing(st, dims, spacing):
     return out
 
 
+def face_normals_from_metrics(dims, xim, etm, zem):
+    """cm[a] = [2 face][3][face]: the unit normals grad(xi_a)/|grad(xi_a)| on
+    the two faces of axis a from metric rows as cs_metrics returns them
+    ([3][lmx] each, proportional to grad(xi_a) with the positive factor
+    1/yaco when yaco > 0), face points ordered as the gcbc face loops run."""
+    lxi, let, lze = dims
+    cm = []
+    for a, (rows, n) in enumerate(zip((xim, etm, zem), dims)):
+        r = np.asarray(rows).reshape(3, lze, let, lxi)
+        c = []
+        for face in range(2):
+            idx = face * (n - 1)
+            fr = r[:, :, :, idx] if a == 0 else (r[:, :, idx, :] if a == 1 else r[:, idx, :, :])
+            fr = fr.reshape(3, -1)
+            c.append(fr / np.sqrt((fr * fr).sum(0)))
+        cm.append(np.stack(c).reshape(-1))
+    return cm
+
+
 def curvilinear_state(dims, seed=2, skew=0.3, **flow):
     """flow_state on the periodic sine-skewed grid x = i + s sin(2 pi j / Ny),
     y = j + s sin(2 pi k / Nz), z = k + s sin(2 pi i / Nx) (unit index
@@ -184,16 +203,5 @@ def curvilinear_state(dims, seed=2, skew=0.3, **flow):
     det = u[0] * rows[0][0] + u[1] * rows[0][1] + u[2] * rows[0][2]
     st["xim"], st["etm"], st["zem"] = (np.concatenate(r) for r in rows)
     st["yaco"] = 1.0 / det
-    cm = []
-    lxi, let, lze = dims
-    for a, n in enumerate(dims):
-        r = np.array(rows[a]).reshape(3, lze, let, lxi)
-        c = []
-        for face in range(2):
-            idx = face * (n - 1)
-            fr = r[:, :, :, idx] if a == 0 else (r[:, :, idx, :] if a == 1 else r[:, idx, :, :])
-            fr = fr.reshape(3, -1)
-            c.append(fr / np.sqrt((fr * fr).sum(0)))
-        cm.append(np.stack(c).reshape(-1))
-    st["cm"] = cm
+    st["cm"] = face_normals_from_metrics(dims, st["xim"], st["etm"], st["zem"])
     return st
