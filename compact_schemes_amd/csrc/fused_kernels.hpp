@@ -123,6 +123,8 @@ struct S1F : S1T<AX, CLOSE, float> {};
 // curvilinear solvers (cs_solver_set_metrics): FP64 intermediates
 template <int AX, bool CLOSE>
 struct S1C : S1T<AX, CLOSE, double, true> {};
+template <int AX, bool CLOSE>
+struct S1CF : S1T<AX, CLOSE, float, true> {};
 
 // S2: flux divergence along AX.  MODE 0: write the partial RHS; 1: add to it;
 // 2: add and write the RHS; 3: add and write q0 - dtk*J*RHS (RK stage).
@@ -269,6 +271,8 @@ template <int AX, int MODE, bool VISC>
 struct S2F : S2T<AX, MODE, VISC, float> {};
 template <int AX, int MODE, bool VISC>
 struct S2C : S2T<AX, MODE, VISC, double, true> {};
+template <int AX, int MODE, bool VISC>
+struct S2CF : S2T<AX, MODE, VISC, float, true> {};
 
 // SF: the compact filter of the five conserved variables along AX, in place
 // (the filter of cs_filter: 10th-order interior rows, reduced-order centred

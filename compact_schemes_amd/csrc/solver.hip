@@ -461,7 +461,6 @@ int cs_solver_set_metrics(cs_solver* sv, const double* xim, const double* etm, c
         return CS_OK;
     }
     if (given != 7) return fail(CS_ERR_ARG, "metrics: xim, etm, zem, yaco and the three face-normal arrays");
-    if (sv->prec != 0) return fail(CS_ERR_UNSUPPORTED, "curvilinear solvers store their intermediates in FP64");
     Metric m{};
     const long lmx = sv->g.lmx;
     for (int a = 0; a < 3; a++) {
@@ -476,7 +475,6 @@ int cs_solver_set_metrics(cs_solver* sv, const double* xim, const double* etm, c
 
 int cs_solver_set_precision(cs_solver* sv, int mode) {
     if (!sv || (mode != 0 && mode != 1)) return fail(CS_ERR_ARG, "precision mode is 0 or 1");
-    if (mode == 1 && sv->curv) return fail(CS_ERR_UNSUPPORTED, "curvilinear solvers store their intermediates in FP64");
     sv->prec = mode;
     apply_tuned(sv, false);
     return CS_OK;

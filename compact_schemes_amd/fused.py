@@ -174,7 +174,7 @@ class FusedSolver:
         exact = n == P * K
         nbc = self.nbc_flat
         gcbc = nbc[ax] == 10 and nbc[3 + ax] == 10
-        f32 = "C" if self.curvilinear else ("F" if (self.viscous and self.precision == 1) else "")
+        f32 = ("C" if self.curvilinear else "") + ("F" if (self.viscous and self.precision == 1) else "")
         if which < 3:
             sw, gc = f"cs::fused::S1{f32}<{ax}, {str(which == 2).lower()}>", "false"
         else:

@@ -304,8 +304,8 @@ int cs_solver_set_precision(cs_solver* sv, int mode);
  * normals cm0, cm1, cm2 of the x, y and z faces ([2 face][3][face], as
  * cs_gcbc_go takes them).  Every sweep, the time step and the RK stage then
  * read them per point instead of cfg.metric / cfg.yaco (ssk stays cfg.mu).
- * All seven NULL: back to the constants.  Curvilinear solvers keep FP64
- * intermediates (cs_solver_set_precision 1 is refused); their adjoint (vjp
+ * All seven NULL: back to the constants.  cs_solver_set_precision applies as
+ * on Cartesian grids.  The adjoint (vjp
  * and tape calls, inviscid) runs the fused transposed sweeps with the same
  * per-point arrays, which need every axis cut into 8-point chunks
  * (cs_solver_adjoint_mode 1); otherwise those calls return

@@ -129,9 +129,8 @@ def test_curvilinear_launch_shapes_agree(oracle):
 
 @pytest.mark.gpu
 def test_curvilinear_refusals_and_reset():
-    """FP32 intermediates, and the adjoint on lines without 8-point chunks, are
-    refused on a curvilinear solver; all-NULL metrics return it to the
-    constants."""
+    """The adjoint on lines without 8-point chunks is refused on a
+    curvilinear solver; all-NULL metrics return it to the constants."""
     import compact_schemes_amd as cs
     from compact_schemes_amd import workloads as wl
     from compact_schemes_amd.fused import FusedSolver
@@ -140,8 +139,6 @@ def test_curvilinear_refusals_and_reset():
     sv = FusedSolver(dims, CL, st, viscous=False, curvilinear=True)
     q = dev(st["qa"])
     case("curvilinear refusals")
-    with pytest.raises(cs.CsError):
-        sv.set_precision(1)
     with pytest.raises(cs.CsError):          # 20 and 18 points: no 8-point chunks for the adjoint
         sv.rhs_vjp(q, q.clone())
     with pytest.raises(cs.CsError):
@@ -295,3 +292,29 @@ def test_curvilinear_from_coordinates_matches_per_operator(oracle, visc):
         sl = slice(m * lmx, (m + 1) * lmx)
         assert nrel(de[sl], dr[sl]) < 1e-12, f"per-operator, variable {m}"
         assert nrel(de[sl], orc[sl]) < 1e-12, f"oracle, variable {m}"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nbc", [CL, MIX])
+def test_curvilinear_fp32_intermediates(oracle, nbc):
+    """FP32 storage of the viscous intermediates on the skewed grid: the RHS
+    within 1e-9 of the FP64 oracle, and the FP64 mode unchanged by a round
+    trip through the FP32 one."""
+    import torch
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (40, 36, 32)
+    st = _state(dims, seed=18)
+    lmx = dims[0] * dims[1] * dims[2]
+    sv = FusedSolver(dims, nbc, st, curvilinear=True)
+    q = dev(st["qa"])
+    case(f"curvilinear fp32 intermediates nbc={nbc[0]}")
+    d64 = sv.rhs(q).clone()
+    sv.set_precision(1)
+    d32 = host(sv.rhs(q))
+    ref = _oracle_rhs(oracle, dims, nbc, st, True)
+    errs = [nrel(d32[m * lmx:(m + 1) * lmx], ref[m * lmx:(m + 1) * lmx]) for m in range(5)]
+    print("curvilinear fp32 intermediates vs fp64 oracle:", " ".join(f"{e:.2e}" for e in errs))
+    assert max(errs) < 1e-9
+    assert max(errs) > 0.0                    # the FP32 kernels ran
+    sv.set_precision(0)
+    assert torch.equal(sv.rhs(q).clone(), d64)
