@@ -895,8 +895,6 @@ int cs_solver_rk4_step_tape(cs_solver* sv, double* qa, const double* sbcc, doubl
     if (!sv || !qa || !tape) return cs::fail(CS_ERR_ARG, "null argument");
     if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
         return cs::fail(CS_ERR_ARG, "non-reflective faces need sbcc");
-    if (sv->curv && !cs::fused::adjoint_fused(sv))
-        return cs::fail(CS_ERR_UNSUPPORTED, "curvilinear adjoints run the fused transposed sweeps (8-point chunks, mode 1)");
     hipStream_t s = (hipStream_t)stream;
     int rc = cs::fused::rk4_step_tape(sv, qa, sbcc, dt, tape, s);
     if (rc || !dt_out) return rc;
@@ -911,8 +909,6 @@ int cs_solver_rk4_vjp_tape(cs_solver* sv, const double* tape, double* lam, doubl
     if (!sv || !tape || !lam) return cs::fail(CS_ERR_ARG, "null argument");
     if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
         return cs::fail(CS_ERR_ARG, "non-reflective faces need sbcc");
-    if (sv->curv && !cs::fused::adjoint_fused(sv))
-        return cs::fail(CS_ERR_UNSUPPORTED, "curvilinear adjoints run the fused transposed sweeps (8-point chunks, mode 1)");
     return cs::fused::rk4_vjp_tape(sv, tape, lam, dt, sbcc, (hipStream_t)stream);
 }
 

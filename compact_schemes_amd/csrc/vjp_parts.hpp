@@ -30,6 +30,19 @@ __device__ __forceinline__ void flux_row(int ax, const T* q, double m, cs_point 
     F[4] = (q[4] + p.p) * ss - p.p * (uf * m);
 }
 
+// flux row of axis a with a general metric row m (curvilinear solvers; the
+// S2C sweep's point(), Euler part)
+template <class T>
+__device__ __forceinline__ void flux_row_m(const T* q, const double* m, cs_point umf, T* F) {
+    PrimT<T> p = primitive(q);
+    const T ss = m[0] * (p.u + umf.x) + m[1] * (p.v + umf.y) + m[2] * (p.w + umf.z);
+    F[0] = q[0] * ss;
+    F[1] = q[1] * ss + m[0] * p.p;
+    F[2] = q[2] * ss + m[1] * p.p;
+    F[3] = q[3] * ss + m[2] * p.p;
+    F[4] = (q[4] + p.p) * ss - p.p * (umf.x * m[0] + umf.y * m[1] + umf.z * m[2]);
+}
+
 // characteristic correction of a face point (the S2 sweep's gcbc()) with
 // the face normal cm and the point's Jacobian J
 template <class T, class X>

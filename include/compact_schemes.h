@@ -305,11 +305,9 @@ int cs_solver_set_precision(cs_solver* sv, int mode);
  * cs_gcbc_go takes them).  Every sweep, the time step and the RK stage then
  * read them per point instead of cfg.metric / cfg.yaco (ssk stays cfg.mu).
  * All seven NULL: back to the constants.  cs_solver_set_precision applies as
- * on Cartesian grids.  The adjoint (vjp
- * and tape calls, inviscid) runs the fused transposed sweeps with the same
- * per-point arrays, which need every axis cut into 8-point chunks
- * (cs_solver_adjoint_mode 1); otherwise those calls return
- * CS_ERR_UNSUPPORTED. */
+ * on Cartesian grids, and so do the adjoint calls (vjp and tape,
+ * inviscid: the fused transposed sweeps or the per-operator products, with
+ * the same per-point arrays). */
 int cs_solver_set_metrics(cs_solver* sv, const double* xim, const double* etm, const double* zem,
                           const double* yaco, const double* cm0, const double* cm1, const double* cm2);
 /* order of the six sweeps (no reference counterpart): 0 = gradients along

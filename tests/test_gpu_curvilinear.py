@@ -129,8 +129,8 @@ def test_curvilinear_launch_shapes_agree(oracle):
 
 @pytest.mark.gpu
 def test_curvilinear_refusals_and_reset():
-    """The adjoint on lines without 8-point chunks is refused on a
-    curvilinear solver; all-NULL metrics return it to the constants."""
+    """Partial metric arguments are refused; all-NULL metrics return a
+    curvilinear solver to the constants."""
     import compact_schemes_amd as cs
     from compact_schemes_amd import workloads as wl
     from compact_schemes_amd.fused import FusedSolver
@@ -139,8 +139,6 @@ def test_curvilinear_refusals_and_reset():
     sv = FusedSolver(dims, CL, st, viscous=False, curvilinear=True)
     q = dev(st["qa"])
     case("curvilinear refusals")
-    with pytest.raises(cs.CsError):          # 20 and 18 points: no 8-point chunks for the adjoint
-        sv.rhs_vjp(q, q.clone())
     with pytest.raises(cs.CsError):
         cs.check(cs.lib().cs_solver_set_metrics(sv.h, cs._p(q), None, None, None, None, None, None))
     cs.check(cs.lib().cs_solver_set_metrics(sv.h, None, None, None, None, None, None, None))
@@ -239,22 +237,44 @@ def test_curvilinear_adjoint_with_cartesian_arrays():
 
 
 @pytest.mark.gpu
-def test_curvilinear_adjoint_needs_fused_sweeps():
-    """Lines that do not split into 8-point chunks (or adjoint mode 0) would
-    need the per-operator adjoint, which has no per-point metrics: refused."""
-    import compact_schemes_amd as cs
-    dims = (24, 20, 18)
-    st, sv = _euler(dims, CL, 47)
-    q = dev(st["qa"])
-    case("curvilinear adjoint refusal")
-    with pytest.raises(cs.CsError):
-        sv.rhs_vjp(q, q.clone())
-    st, sv = _euler((24, 16, 32), CL, 48)
-    sv.set_adjoint_mode(0)
-    q = dev(st["qa"])
-    with pytest.raises(cs.CsError):
-        sv.rhs_vjp(q, q.clone())
+@pytest.mark.parametrize("dims", [(24, 20, 18), (24, 16, 32)], ids=["ragged", "chunks8"])
+def test_curvilinear_per_operator_adjoint(dims):
+    """The per-operator adjoint (lines without 8-point chunks, and adjoint
+    mode 0) with per-point metrics: the RHS product against fourth-order
+    central differences, the RK4-step product against central differences,
+    and on 8-point lines against the fused transposed sweeps to 1e-12."""
+    import torch
+    st, sv = _euler(dims, CONFIG5, 47)
+    rng = np.random.default_rng(49)
+    n = st["qa"].size
+    q0 = dev(st["qa"])
+    lam = dev(rng.standard_normal(n))
+    v = dev(rng.standard_normal(n))
+    dt = sv.time_step(q0)
+    case(f"curvilinear per-operator adjoint dims={dims}")
+    fused = None
+    if dims == (24, 16, 32):
+        fused = (sv.rhs_vjp(q0, lam).clone(), sv.rk4_vjp(q0, lam.clone(), dt).clone())
+        sv.set_adjoint_mode(0)
+    g = sv.rhs_vjp(q0, lam).clone()
+    R = lambda q: sv.rhs(q).clone()
+    eps = 1e-4
+    d = (-R(q0 + 2 * eps * v) + 8 * R(q0 + eps * v) - 8 * R(q0 - eps * v) + R(q0 - 2 * eps * v)) / (12 * eps)
+    fd, ad = float(torch.dot(lam, d)), float(torch.dot(g, v))
+    assert abs(fd - ad) <= 1e-8 * abs(ad), (fd, ad)
+    a = sv.rk4_vjp(q0, lam.clone(), dt).clone()
 
+    def step(q):
+        x = q.clone()
+        sv.rk4_step(x, dt=dt)
+        return x
+    e = 1e-6
+    fd = float(torch.dot(lam, (step(q0 + e * v) - step(q0 - e * v)) / (2 * e)))
+    ad = float(torch.dot(a, v))
+    assert abs(fd - ad) <= 1e-7 * abs(ad), (fd, ad)
+    if fused is not None:
+        assert nrel(host(g), host(fused[0])) < 1e-12
+        assert nrel(host(a), host(fused[1])) < 1e-12
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("visc", [True, False])
