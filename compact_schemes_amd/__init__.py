@@ -167,8 +167,8 @@ def lib():
             "cs_batch_autotune": (I, [P, I, P]),
             "cs_tune_set": (I, [I, I, I, I, I, I]),
             "cs_tune_get": (I, [I, I, I, I, I, P]),
-            "cs_tune_set_sweep": (I, [I, I, I, I, I, I, I]),
-            "cs_tune_get_sweep": (I, [I, I, I, I, P, P]),
+            "cs_tune_set_sweep": (I, [I, I, I, I, I, I, I, I]),
+            "cs_tune_get_sweep": (I, [I, I, I, I, I, P, P]),
             "cs_tune_set_filter_sweep": (I, [I, I, I, I, I, I]),
             "cs_tune_get_filter_sweep": (I, [I, I, I, P, P]),
             "cs_tune_clear": (I, []),

@@ -165,12 +165,12 @@ int launch_batch_lines(int op, int K, int axis, int P, const LineArgs* args, con
 // launch-shape choices (tune.hip): chunk length per (operator, line length,
 // start/end rows, precision), 0 = none (the default rule of choose_chunk);
 // fused right-hand-side sweep shapes per (axis, line length, seam axis,
-// precision of the intermediates), fused filter shapes per (axis, line
-// length, seam axis)
+// precision of the intermediates, constant 0 / per-point 1 metrics), fused
+// filter shapes per (axis, line length, seam axis)
 int tuned_chunk(int op, int n, int start, int end, int prec);
 void set_tuned_chunk(int op, int n, int start, int end, int prec, int K);
-bool tuned_sweep(int axis, int n, int seam, int prec, int* variant_lines_xcd);
-void set_tuned_sweep(int axis, int n, int seam, int prec, const int* variant_lines_xcd);
+bool tuned_sweep(int axis, int n, int seam, int prec, int metrics, int* variant_lines_xcd);
+void set_tuned_sweep(int axis, int n, int seam, int prec, int metrics, const int* variant_lines_xcd);
 bool tuned_filter(int axis, int n, int seam, int* fields_lines_chunk);
 void set_tuned_filter(int axis, int n, int seam, const int* fields_lines_chunk);
 

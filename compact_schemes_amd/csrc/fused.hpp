@@ -59,6 +59,8 @@ struct cs_solver {
     // the right-hand-side sweeps: ax, or the same axis cut into 16-point
     // chunks (cs_solver_tune variant 2, with tabs16)
     cs::fused::AxisDesc rax[3];
+    // default (variant, lines, xcd) of the right-hand-side sweeps (rax), then of ax
+    int shape0[3][6];
     // the filter sweeps: ax, or (y/z, cs_solver_tune_filter) 16-point chunks with ftabs16
     cs::fused::AxisDesc fax[3];
     double* tabs[3] = {nullptr, nullptr, nullptr};

@@ -286,6 +286,11 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
         cs_solver_destroy(sv);
         return fail(CS_ERR_HIP, "allocation of solver buffers failed");
     }
+    for (int a = 0; a < 3; a++) {
+        const AxisDesc &r = sv->rax[a], &x = sv->ax[a];
+        const int v[6] = {r.variant, r.tl, r.xcd, x.variant, x.tl, x.xcd};
+        for (int i = 0; i < 6; i++) sv->shape0[a][i] = v[i];
+    }
     apply_tuned(sv, true);
     *out = sv;
     return CS_OK;
@@ -332,14 +337,22 @@ static int filter_shape(cs_solver* sv, int axis, int fields, int lines, int chun
     return CS_OK;
 }
 
-// applies the process-wide launch-shape choices for this solver's axes
-// (sweeps at the current precision; filters when `filters`); a choice that
-// this solver cannot run (a cache written for another shape) is skipped
+// applies the process-wide launch-shape choices for this solver's axes: the
+// right-hand-side sweeps take the entry of the current precision and metric
+// kind, or their default shape when there is none; the filter sweeps take
+// theirs when `filters`.  A choice that this solver cannot run (a cache
+// written for another shape) is skipped.
 static void apply_tuned(cs_solver* sv, bool filters) {
     for (int a = 0; a < 3; a++) {
-        const AxisDesc& ax = sv->ax[a];
+        AxisDesc& ax = sv->ax[a];
+        const int* d = sv->shape0[a];
+        ax.variant = d[3];
+        ax.tl = d[4];
+        ax.xcd = d[5];
         int v[3];
-        if (tuned_sweep(a, ax.N, ax.seam, sv->cfg.viscous ? sv->prec : 0, v)) (void)sweep_shape(sv, a, v[0], v[1], v[2]);
+        if (!tuned_sweep(a, ax.N, ax.seam, sv->cfg.viscous ? sv->prec : 0, sv->curv, v) ||
+            sweep_shape(sv, a, v[0], v[1], v[2]))
+            (void)sweep_shape(sv, a, d[0], d[1], d[2]);
         if (filters && filter_axis_fused(sv, a) && tuned_filter(a, ax.N, ax.seam, v))
             (void)filter_shape(sv, a, v[0], v[1], v[2]);
     }
@@ -418,7 +431,7 @@ int cs_solver_autotune(cs_solver* sv, const double* qa, const double* sbcc, int 
                 }
         sv->ax[a] = ax0;
         if ((rc = sweep_shape(sv, a, best[0], best[1], best[2]))) return rc;
-        set_tuned_sweep(a, sv->ax[a].N, sv->ax[a].seam, prec, best);
+        set_tuned_sweep(a, sv->ax[a].N, sv->ax[a].seam, prec, sv->curv, best);
     }
     if (!sv->cfg.filter) return CS_OK;
     CS_HIP(hipMemcpyAsync(out, qa, sizeof(double) * 5 * sv->g.lmx, hipMemcpyDeviceToDevice, s));
@@ -456,8 +469,10 @@ int cs_solver_set_metrics(cs_solver* sv, const double* xim, const double* etm, c
     const int given = (xim != nullptr) + (etm != nullptr) + (zem != nullptr) + (yaco != nullptr) +
                       (cm0 != nullptr) + (cm1 != nullptr) + (cm2 != nullptr);
     if (given == 0) {            // back to the constants of the configuration
+        const bool was = sv->curv != 0;
         sv->curv = 0;
         sv->met = Metric{};
+        if (was) apply_tuned(sv, false);   // the sweep shapes of constant metrics
         return CS_OK;
     }
     if (given != 7) return fail(CS_ERR_ARG, "metrics: xim, etm, zem, yaco and the three face-normal arrays");
@@ -468,13 +483,16 @@ int cs_solver_set_metrics(cs_solver* sv, const double* xim, const double* etm, c
         m.cm[a] = cms[a];
     }
     m.J = yaco;
+    const bool was = sv->curv != 0;
     sv->met = m;
     sv->curv = 1;
+    if (!was) apply_tuned(sv, false);   // the sweep shapes of per-point metrics
     return CS_OK;
 }
 
 int cs_solver_set_precision(cs_solver* sv, int mode) {
     if (!sv || (mode != 0 && mode != 1)) return fail(CS_ERR_ARG, "precision mode is 0 or 1");
+    if (mode == sv->prec) return CS_OK;
     sv->prec = mode;
     apply_tuned(sv, false);
     return CS_OK;

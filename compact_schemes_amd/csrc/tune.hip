@@ -8,19 +8,20 @@
 // kernels' speed depends on.
 //   line:   (operator, line length, start/end rows, precision) -> chunk length
 //           of the per-operator and batched line kernels;
-//   sweep:  (axis, line length, seam axis, precision of the intermediates)
-//           -> kernel variant, lines per workgroup, XCD grouping of the fused
-//           right-hand-side sweeps;
+//   sweep:  (axis, line length, seam axis, precision of the intermediates,
+//           metrics: 0 constant per axis, 1 per point) -> kernel variant, lines
+//           per workgroup, XCD grouping of the fused right-hand-side sweeps;
 //   filter: (axis, line length, seam axis) -> fields per workgroup, lines per
 //           workgroup, chunk length of the fused filter sweeps.
 // Entries are written by the autotuners (cs_batch_autotune,
 // cs_solver_autotune) or by hand, and read when operator tables and solvers
 // are built.  File format, one entry per line ('#' starts a comment):
 //   line <op> <length> <start> <end> <precision> <K>
-//   sweep <axis> <length> <seam> <precision> <variant> <lines> <xcd>
+//   sweep <axis> <length> <seam> <precision> <metrics> <variant> <lines> <xcd>
 //   filter <axis> <length> <seam> <fields> <lines> <chunk>
 // A line of three integers "<op> <length> <K>" (the earlier format) is a line
-// entry for FP64 and both closure and seam ends.
+// entry for FP64 and both closure and seam ends; a sweep entry of seven
+// integers (no metrics field) is one for constant metrics.
 #include "cs_internal.hpp"
 
 #include <cstdio>
@@ -32,7 +33,7 @@ namespace {
 
 std::mutex g_lock;
 std::map<std::array<int, 5>, int> g_line;                   // op, n, start, end, prec -> K
-std::map<std::array<int, 4>, std::array<int, 3>> g_sweep;   // axis, n, seam, prec -> variant, lines, xcd
+std::map<std::array<int, 5>, std::array<int, 3>> g_sweep;   // axis, n, seam, prec, metrics -> variant, lines, xcd
 std::map<std::array<int, 3>, std::array<int, 3>> g_filter;  // axis, n, seam -> fields, lines, chunk
 
 int check_line(int op, int n, int start, int end, int prec, int K) {
@@ -44,11 +45,11 @@ int check_line(int op, int n, int start, int end, int prec, int K) {
     return CS_OK;
 }
 
-int check_sweep(int axis, int n, int seam, int prec, int variant, int lines, int xcd) {
-    if (axis < 0 || axis > 2 || n < 16 || n > 512 || seam < 0 || seam > 1 || prec < 0 || prec > 1 || variant < 0 ||
-        variant > 2 || (lines != 8 && lines != 16) || xcd < 0 || xcd > 1)
-        return fail(CS_ERR_ARG,
-                    "sweep entry: axis 0..2, length 16..512, seam 0/1, precision 0/1, variant 0..2, lines 8/16, xcd 0/1");
+int check_sweep(int axis, int n, int seam, int prec, int metrics, int variant, int lines, int xcd) {
+    if (axis < 0 || axis > 2 || n < 16 || n > 512 || seam < 0 || seam > 1 || prec < 0 || prec > 1 || metrics < 0 ||
+        metrics > 1 || variant < 0 || variant > 2 || (lines != 8 && lines != 16) || xcd < 0 || xcd > 1)
+        return fail(CS_ERR_ARG, "sweep entry: axis 0..2, length 16..512, seam 0/1, precision 0/1, metrics 0/1, "
+                                "variant 0..2, lines 8/16, xcd 0/1");
     return CS_OK;
 }
 
@@ -73,17 +74,17 @@ void set_tuned_chunk(int op, int n, int start, int end, int prec, int K) {
     else g_line[{op, n, start, end, prec}] = K;
 }
 
-bool tuned_sweep(int axis, int n, int seam, int prec, int* v3) {
+bool tuned_sweep(int axis, int n, int seam, int prec, int metrics, int* v3) {
     std::lock_guard<std::mutex> g(g_lock);
-    auto it = g_sweep.find({axis, n, seam, prec});
+    auto it = g_sweep.find({axis, n, seam, prec, metrics});
     if (it == g_sweep.end()) return false;
     for (int i = 0; i < 3; i++) v3[i] = it->second[i];
     return true;
 }
 
-void set_tuned_sweep(int axis, int n, int seam, int prec, const int* v3) {
+void set_tuned_sweep(int axis, int n, int seam, int prec, int metrics, const int* v3) {
     std::lock_guard<std::mutex> g(g_lock);
-    g_sweep[{axis, n, seam, prec}] = {v3[0], v3[1], v3[2]};
+    g_sweep[{axis, n, seam, prec, metrics}] = {v3[0], v3[1], v3[2]};
 }
 
 bool tuned_filter(int axis, int n, int seam, int* v3) {
@@ -118,17 +119,17 @@ int cs_tune_get(int op, int n, int start, int end, int precision, int* K) {
     return CS_OK;
 }
 
-int cs_tune_set_sweep(int axis, int n, int seam, int precision, int variant, int lines, int xcd) {
-    const int rc = check_sweep(axis, n, seam, precision, variant, lines, xcd);
+int cs_tune_set_sweep(int axis, int n, int seam, int precision, int metrics, int variant, int lines, int xcd) {
+    const int rc = check_sweep(axis, n, seam, precision, metrics, variant, lines, xcd);
     if (rc) return rc;
     const int v[3] = {variant, lines, xcd};
-    set_tuned_sweep(axis, n, seam, precision, v);
+    set_tuned_sweep(axis, n, seam, precision, metrics, v);
     return CS_OK;
 }
 
-int cs_tune_get_sweep(int axis, int n, int seam, int precision, int* found, int* shape3) {
+int cs_tune_get_sweep(int axis, int n, int seam, int precision, int metrics, int* found, int* shape3) {
     if (!found || !shape3) return fail(CS_ERR_ARG, "null argument");
-    *found = tuned_sweep(axis, n, seam, precision, shape3) ? 1 : 0;
+    *found = tuned_sweep(axis, n, seam, precision, metrics, shape3) ? 1 : 0;
     return CS_OK;
 }
 
@@ -160,7 +161,7 @@ int cs_tune_save(const char* path) {
     if (!fp) return fail(CS_ERR_ARG, std::string("cannot write ") + path);
     std::fprintf(fp, "# compact_schemes_amd launch-shape choices\n");
     std::fprintf(fp, "# line <op> <length> <start> <end> <precision> <K>\n");
-    std::fprintf(fp, "# sweep <axis> <length> <seam> <precision> <variant> <lines> <xcd>\n");
+    std::fprintf(fp, "# sweep <axis> <length> <seam> <precision> <metrics> <variant> <lines> <xcd>\n");
     std::fprintf(fp, "# filter <axis> <length> <seam> <fields> <lines> <chunk>\n");
     {
         std::lock_guard<std::mutex> g(g_lock);
@@ -168,8 +169,8 @@ int cs_tune_save(const char* path) {
             std::fprintf(fp, "line %d %d %d %d %d %d\n", e.first[0], e.first[1], e.first[2], e.first[3], e.first[4],
                          e.second);
         for (auto& e : g_sweep)
-            std::fprintf(fp, "sweep %d %d %d %d %d %d %d\n", e.first[0], e.first[1], e.first[2], e.first[3],
-                         e.second[0], e.second[1], e.second[2]);
+            std::fprintf(fp, "sweep %d %d %d %d %d %d %d %d\n", e.first[0], e.first[1], e.first[2], e.first[3],
+                         e.first[4], e.second[0], e.second[1], e.second[2]);
         for (auto& e : g_filter)
             std::fprintf(fp, "filter %d %d %d %d %d %d\n", e.first[0], e.first[1], e.first[2], e.second[0],
                          e.second[1], e.second[2]);
@@ -190,20 +191,24 @@ int cs_tune_load(const char* path) {
         char* hash = std::strchr(buf, '#');
         if (hash) *hash = '\0';
         char kind[16] = {0};
-        int v[8], k = 0;
+        int v[9], k = 0;
         if (std::sscanf(buf, "%15s", kind) != 1) continue;      // blank line
         if (!std::strcmp(kind, "line") || !std::strcmp(kind, "sweep") || !std::strcmp(kind, "filter")) {
             const char* rest = std::strstr(buf, kind) + std::strlen(kind);
-            const int want = kind[0] == 's' ? 7 : 6;
-            k = std::sscanf(rest, "%d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7]);
-            if (k != want) {
-                rc = fail(CS_ERR_ARG,
-                          std::string(path) + ":" + std::to_string(lineno) + ": expected " + std::to_string(want) +
-                              " integers after '" + kind + "'");
+            const bool sweep = kind[0] == 's';
+            k = std::sscanf(rest, "%d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7],
+                            &v[8]);
+            if (sweep ? (k != 7 && k != 8) : k != 6) {
+                rc = fail(CS_ERR_ARG, std::string(path) + ":" + std::to_string(lineno) + ": expected " +
+                                          (sweep ? "7 or 8" : "6") + " integers after '" + kind + "'");
                 break;
             }
+            if (sweep && k == 7) {   // no metrics field: constant metrics
+                for (int i = 7; i > 4; i--) v[i] = v[i - 1];
+                v[4] = 0;
+            }
             if (kind[0] == 'l') rc = cs_tune_set(v[0], v[1], v[2], v[3], v[4], v[5]);
-            else if (kind[0] == 's') rc = cs_tune_set_sweep(v[0], v[1], v[2], v[3], v[4], v[5], v[6]);
+            else if (sweep) rc = cs_tune_set_sweep(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]);
             else rc = cs_tune_set_filter_sweep(v[0], v[1], v[2], v[3], v[4], v[5]);
         } else if (std::sscanf(buf, "%d %d %d %d", &v[0], &v[1], &v[2], &v[3]) == 3) {
             // "<op> <length> <K>": FP64, closure and seam ends

@@ -2,10 +2,10 @@
 
 Three kinds of entries, each keyed by what the kernels' speed depends on:
 line kernels (operator, line length, start/end rows, precision) -> chunk
-length; fused right-hand-side sweeps (axis, line length, seam, precision) ->
-(variant, lines, xcd); fused filter sweeps (axis, line length, seam) ->
-(fields, lines, chunk).  The autotuners (Batch.autotune,
-FusedSolver.autotune) fill it; operator tables and solvers built afterwards
+length; fused right-hand-side sweeps (axis, line length, seam, precision,
+metrics: 0 constant, 1 per point) -> (variant, lines, xcd); fused filter
+sweeps (axis, line length, seam) -> (fields, lines, chunk).  The autotuners
+(Batch.autotune, FusedSolver.autotune) fill it; operator tables and solvers built afterwards
 read it; save/load keep it across runs as text.
 """
 import ctypes
@@ -27,15 +27,18 @@ def get_line(op, n, start=0, end=0, precision=0):
     return k.value
 
 
-def set_sweep(axis, n, seam, precision, variant, lines, xcd):
-    check(lib().cs_tune_set_sweep(int(axis), int(n), int(seam), int(precision), int(variant), int(lines), int(xcd)))
+def set_sweep(axis, n, seam, precision, variant, lines, xcd, metrics=0):
+    """metrics: 0 = solvers with constant metrics, 1 = per-point metrics."""
+    check(lib().cs_tune_set_sweep(int(axis), int(n), int(seam), int(precision), int(metrics), int(variant),
+                                  int(lines), int(xcd)))
 
 
-def get_sweep(axis, n, seam, precision=0):
+def get_sweep(axis, n, seam, precision=0, metrics=0):
     """(variant, lines, xcd) or None."""
     found = ctypes.c_int()
     out = (ctypes.c_int * 3)()
-    check(lib().cs_tune_get_sweep(int(axis), int(n), int(seam), int(precision), ctypes.byref(found), out))
+    check(lib().cs_tune_get_sweep(int(axis), int(n), int(seam), int(precision), int(metrics), ctypes.byref(found),
+                                  out))
     return tuple(out) if found.value else None
 
 

@@ -413,10 +413,11 @@ int cs_tune_set(int op, int n, int start, int end, int precision, int K);
 int cs_tune_get(int op, int n, int start, int end, int precision, int* K);   /* K = 0: none */
 /* fused right-hand-side sweeps: shape of cs_solver_tune for (axis, line
  * length n, seam = 1 for a periodic or inter-subdomain axis, precision =
- * cs_solver_set_precision's mode); solvers apply it at creation and when
- * their precision is set */
-int cs_tune_set_sweep(int axis, int n, int seam, int precision, int variant, int lines, int xcd);
-int cs_tune_get_sweep(int axis, int n, int seam, int precision, int* found, int* shape3);
+ * cs_solver_set_precision's mode, metrics = 0 for constant metrics, 1 for
+ * per-point metrics (cs_solver_set_metrics)); solvers apply it at creation
+ * and when their precision or metrics are set */
+int cs_tune_set_sweep(int axis, int n, int seam, int precision, int metrics, int variant, int lines, int xcd);
+int cs_tune_get_sweep(int axis, int n, int seam, int precision, int metrics, int* found, int* shape3);
 /* fused filter sweeps: shape of cs_solver_tune_filter for (axis, n, seam);
  * solvers apply it at creation */
 int cs_tune_set_filter_sweep(int axis, int n, int seam, int fields, int lines, int chunk);
@@ -424,11 +425,12 @@ int cs_tune_get_filter_sweep(int axis, int n, int seam, int* found, int* shape3)
 int cs_tune_clear(void);   /* all three kinds */
 /* persisted as text, one entry per line ('#' starts a comment):
  *   line <op> <n> <start> <end> <precision> <K>
- *   sweep <axis> <n> <seam> <precision> <variant> <lines> <xcd>
+ *   sweep <axis> <n> <seam> <precision> <metrics> <variant> <lines> <xcd>
  *   filter <axis> <n> <seam> <fields> <lines> <chunk>
  * cs_tune_load adds the entries of a file to the cache; a line of three
  * integers "<op> <n> <K>" sets K for FP64 lines with closure rows at both ends
- * and with seam rows at both ends.  A malformed or inadmissible entry stops
+ * and with seam rows at both ends, a sweep line without the metrics field is
+ * an entry for constant metrics.  A malformed or inadmissible entry stops
  * the load with CS_ERR_ARG (or CS_ERR_SHAPE) and a message naming file:line;
  * the entries before it stay. */
 int cs_tune_save(const char* path);

@@ -1,7 +1,7 @@
 """The fused solver's autotuner and the launch-shape cache on the GPU: the
 tuned shapes give the default shapes' RHS and filtered state to rounding, are
-recorded in the cache under (axis, length, seam, precision), are applied by
-solvers created afterwards, and survive a save/load."""
+recorded in the cache under (axis, length, seam, precision, metrics), are
+applied by solvers created afterwards, and survive a save/load."""
 import numpy as np
 import pytest
 
@@ -93,3 +93,38 @@ def test_cached_shapes_applied_per_precision(tune):
     case("cached shapes per precision")
     a, b = host(ref.rhs(q)).copy(), host(plain.rhs(q))
     assert nrel(a, b) < 1e-13
+
+
+@pytest.mark.gpu
+def test_curvilinear_autotune_keyed_by_metrics(tune):
+    """A solver with per-point metrics records its sweep shapes under
+    metrics = 1 and reads only those: the constant-metric entries stay
+    untouched and do not apply to it, and a Cartesian solver created after
+    ignores the per-point entries."""
+    from compact_schemes_amd import workloads as wl
+    from compact_schemes_amd.fused import FusedSolver
+    dims = (48, 32, 40)
+    st = wl.curvilinear_state(dims, seed=3, skew=1.5, noise=1e-2, width=3.0)
+    flat = _state(dims, 3)
+    q = dev(st["qa"])
+    default = FusedSolver(dims, PER, st, curvilinear=True).get_tune(1)[0]
+    assert default != (1, 8, 1)
+    tune.set_sweep(1, 32, 1, 0, 1, 8, 1)                # constant metrics only
+    cart = FusedSolver(dims, PER, flat)
+    assert cart.get_tune(1)[0] == (1, 8, 1)
+    sv = FusedSolver(dims, PER, st, curvilinear=True)
+    assert sv.get_tune(1)[0] == default
+    d0 = host(sv.rhs(q)).copy()
+    case("curvilinear autotune dims=(48, 32, 40)")
+    sv.autotune(q, reps=2)
+    for a in range(3):
+        assert tune.get_sweep(a, dims[a], 1, 0, metrics=1) == sv.get_tune(a)[0]
+    assert tune.get_sweep(1, 32, 1, 0) == (1, 8, 1)
+    assert nrel(host(sv.rhs(q)), d0) < 1e-13
+    again = FusedSolver(dims, PER, st, curvilinear=True)
+    assert [again.get_tune(a)[0] for a in range(3)] == [sv.get_tune(a)[0] for a in range(3)]
+    assert FusedSolver(dims, PER, flat).get_tune(1)[0] == (1, 8, 1)
+    # switching the metrics of one solver selects the other key's shape
+    import compact_schemes_amd as cs
+    cs.check(cs.lib().cs_solver_set_metrics(again.h, *([None] * 7)))   # all null: constant metrics
+    assert again.get_tune(1)[0] == (1, 8, 1)

@@ -35,6 +35,10 @@ def test_sweep_and_filter_entries(tune):
     assert tune.get_sweep(2, 256, 0, 0) == (2, 16, 0)
     assert tune.get_sweep(2, 256, 0, 1) == (0, 8, 1)
     assert tune.get_sweep(2, 256, 1, 0) is None
+    assert tune.get_sweep(2, 256, 0, 0, metrics=1) is None     # per-point metrics: a key of its own
+    tune.set_sweep(2, 256, 0, 0, 1, 8, 0, metrics=1)
+    assert tune.get_sweep(2, 256, 0, 0, metrics=1) == (1, 8, 0)
+    assert tune.get_sweep(2, 256, 0, 0) == (2, 16, 0)
     tune.set_filter_sweep(1, 128, 1, 5, 8, 16)
     assert tune.get_filter_sweep(1, 128, 1) == (5, 8, 16)
     assert tune.get_filter_sweep(1, 128, 0) is None
@@ -50,6 +54,7 @@ def test_sweep_and_filter_entries(tune):
     lambda t: t.set_line(0, 4, 8),                  # shorter than the stencil
     lambda t: t.set_sweep(3, 64, 0, 0, 0, 16, 0),   # axis
     lambda t: t.set_sweep(1, 64, 0, 2, 0, 16, 0),   # precision
+    lambda t: t.set_sweep(1, 64, 0, 0, 0, 16, 0, metrics=2),
     lambda t: t.set_sweep(1, 64, 0, 0, 3, 16, 0),   # variant
     lambda t: t.set_sweep(1, 64, 0, 0, 0, 12, 0),   # lines
     lambda t: t.set_sweep(1, 1024, 0, 0, 0, 16, 0), # longer than a fused axis
@@ -75,13 +80,14 @@ def test_chunk_length_too_short_for_line_rejected(tune):
 def test_save_load_round_trip(tune, tmp_path):
     entries = {
         "line": [(0, 96, 0, 0, 0, 16), (1, 96, 1, 1, 0, 32), (0, 256, 0, 1, 2, 8)],
-        "sweep": [(0, 256, 0, 0, 1, 8, 0), (2, 256, 1, 1, 2, 16, 1)],
+        # axis, length, seam, precision, metrics, variant, lines, xcd
+        "sweep": [(0, 256, 0, 0, 0, 1, 8, 0), (2, 256, 1, 1, 0, 2, 16, 1), (2, 256, 1, 1, 1, 0, 16, 0)],
         "filter": [(1, 256, 0, 1, 16, 16), (0, 512, 1, 5, 16, 8)],
     }
     for e in entries["line"]:
         tune.set_line(e[0], e[1], e[5], e[2], e[3], e[4])
     for e in entries["sweep"]:
-        tune.set_sweep(*e)
+        tune.set_sweep(*e[:4], *e[5:], metrics=e[4])
     for e in entries["filter"]:
         tune.set_filter_sweep(*e)
     path = tmp_path / "tune.txt"
@@ -95,7 +101,7 @@ def test_save_load_round_trip(tune, tmp_path):
     for e in entries["line"]:
         assert tune.get_line(e[0], e[1], e[2], e[3], e[4]) == e[5]
     for e in entries["sweep"]:
-        assert tune.get_sweep(*e[:4]) == e[4:]
+        assert tune.get_sweep(*e[:4], metrics=e[4]) == e[5:]
     for e in entries["filter"]:
         assert tune.get_filter_sweep(*e[:3]) == e[3:]
     tune.save(tmp_path / "again.txt")                   # load then save: the same file
@@ -115,9 +121,21 @@ def test_load_earlier_three_integer_format(tune, tmp_path):
     assert tune.get_line(0, 96, 0, 0, 1) == 0
 
 
+def test_load_sweep_entry_without_metrics_field(tune, tmp_path):
+    """A sweep line of seven integers (no metrics field) is an entry for
+    constant metrics."""
+    path = tmp_path / "sweep7.txt"
+    path.write_text("sweep 1 128 0 1 2 8 1\n")
+    tune.load(path)
+    assert tune.get_sweep(1, 128, 0, 1) == (2, 8, 1)
+    assert tune.get_sweep(1, 128, 0, 1, metrics=1) is None
+
+
 @pytest.mark.parametrize("text, where", [
     ("line 0 64 0 0 0\n", ":1:"),                           # five integers
-    ("sweep 0 64 0 0 1 8\n", ":1:"),                        # six, a sweep takes seven
+    ("sweep 0 64 0 0 1 8\n", ":1:"),                        # six, a sweep takes seven or eight
+    ("sweep 0 64 0 0 0 1 8 0 1\n", ":1:"),                  # nine
+    ("sweep 0 64 0 0 2 1 8 0\n", ":1:"),                    # metrics 2
     ("# ok\nline 0 64 0 0 0 8\nbogus 1 2\n", ":3:"),         # unknown entry
     ("filter 1 64 0 5 16 24\n", ":1:"),                     # chunk length 24
     ("line 0 64 0 0 0 8\n7 8\n", ":2:"),                    # two integers
