@@ -214,7 +214,7 @@ def main():
     if args.path == "fused":
         # the whole path against the same roof: credited bytes of one RHS
         # (736 B/pt) and of one RK4 step (4 stages of 776 B/pt, three filter
-        # sweeps of 80, the time step's 40) over their measured times
+        # sweeps of 80) over their measured times
         rhs_gbs = RHS_BYTES_PER_PT * lmx / (ms_rhs * 1e-3) / 1e9
         step_gbs = STEP_BYTES_PER_PT * lmx / (ms_step * 1e-3) / 1e9
         roofline["whole_rhs"] = {"bytes_per_pt": RHS_BYTES_PER_PT, "ms": round(ms_rhs, 4),
@@ -252,7 +252,10 @@ def main():
 
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_rk_stage_256.txt")
 RHS_BYTES_PER_PT = 736                     # six sweeps of one viscous RHS
-STEP_BYTES_PER_PT = 4 * 776 + 3 * 80 + 40  # RK4 step: four stages, filter x/y/z, time step
+# RK4 step: four stages and the filter along x, y, z; the time step's maxima
+# are reduced inside the first gradient sweep of the first stage (the y sweep
+# of the 256^3 workload: 16 lines x 32 chunks, whole waves), which reads q anyway
+STEP_BYTES_PER_PT = 4 * 776 + 3 * 80
 PROFILE_FP64 = os.path.join(ROOT, "profiles", "pmc_fp64_rk_stage_256.txt")
 
 

@@ -43,17 +43,13 @@ constexpr int kTB = 256;
 
 __global__ void __launch_bounds__(kTB) dt_partial_kernel(const double* q, long lmx, Phys ph, double* part) {
     __shared__ double s0[kTB], s1[kTB];
-    const double rr1 = ph.m[0] * ph.m[0] + ph.m[1] * ph.m[1] + ph.m[2] * ph.m[2];
-    const double ssi = fabs(ph.J);
     double m0 = 0.0, m1 = 0.0;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < lmx; t += (long)gridDim.x * blockDim.x) {
         double qq[5];
         load_q(q, t, lmx, qq);
-        Prim p = primitive(qq);
-        const double rr2 = fabs(ph.m[0] * (p.u + ph.umf.x)) + fabs(ph.m[1] * (p.v + ph.umf.y)) +
-                           fabs(ph.m[2] * (p.w + ph.umf.z));
-        m0 = fmax(m0, (sqrt(p.c2 * rr1) + rr2) * ssi);
-        m1 = fmax(m1, p.rhoi * ph.mu * rr1 * ssi * ssi);
+        const Prim p = primitive(qq);
+        m0 = fmax(m0, dt_conv(ph, p.u, p.v, p.w, p.c2));
+        m1 = fmax(m1, dt_visc(ph, p.rhoi));
     }
     s0[threadIdx.x] = m0;
     s1[threadIdx.x] = m1;
@@ -138,12 +134,29 @@ static int sweep_at(cs_solver* sv, int pos, const SweepCommon& cm, const double*
     return s2_sweep<false>(sv, pos - 3, cm, q, out, q0, div, s);
 }
 
+bool dt_foldable(const cs_solver* sv) {
+    if (!sv->cfg.viscous || sv->curv || !sv->dtpart) return false;
+    const int a = kS1Axes[sv->order][0];
+    if (a == 0) return false;
+    const AxisDesc& r = sv->rax[a];
+    const int TL = r.tl == 8 ? 8 : 16;
+    return r.variant != 1 && r.N == r.P * r.K && (TL * r.P) % 64 == 0;
+}
+
 int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, const double* q0, int div,
                hipStream_t s) {
     const SweepCommon cm = common_of(sv, sbcc);
     for (int pos = sv->cfg.viscous ? 0 : 3; pos < 6; pos++) {
         const int rc = sweep_at(sv, pos, cm, q, out, pos == 5 ? q0 : nullptr, div, s);
         if (rc) return rc;
+        if (pos == 0 && sv->dt_fold) {
+            // the sweep wrote one pair of maxima per workgroup
+            const AxisDesc& r = sv->rax[kS1Axes[sv->order][0]];
+            const int TL = r.tl == 8 ? 8 : 16;
+            const int nb = (int)((r.nlines + TL - 1) / TL);
+            hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(kTB), 0, s, sv->dtpart, nb, sv->cfg.cfl, 0.0, sv->dt);
+            CS_HIP(hipGetLastError());
+        }
     }
     return CS_OK;
 }

@@ -77,6 +77,10 @@ struct cs_solver {
     double* qc = nullptr;
     double* red = nullptr;     // time-step partial maxima
     int red_blocks = 0;
+    // time-step maxima per workgroup of the first gradient sweep, when an RK4
+    // step reduces them there (dt_fold) instead of in a pass of its own
+    double* dtpart = nullptr;
+    int dt_fold = 0;
     double* dt = nullptr;      // [0] dt, [1..2] maxima
     double* dt_host = nullptr; // pinned
     double* vjp = nullptr;     // [41][lmx] adjoint scratch (allocated on the first adjoint call)
@@ -107,6 +111,9 @@ int sweep_at_f32(cs_solver* sv, int pos, const SweepCommon& cm, const double* q,
 int sweep_at_curv(cs_solver* sv, int pos, const SweepCommon& cm, const double* q, double* out, const double* q0,
                   int div, hipStream_t s);
 int time_step(cs_solver* sv, const double* q, double dt_fixed, hipStream_t s);
+// whether the first gradient sweep can reduce the time step (viscous,
+// constant metrics, a y/z direct sweep of whole waves)
+bool dt_foldable(const cs_solver* sv);
 int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, double* out, hipStream_t s);
 int build_axis_table(int n, int seam, int K, int P, std::vector<double>& h);
 // every axis filters with the 8-point fused sweep of ax (the adjoint's assumption)

@@ -4,6 +4,8 @@
 // fused_f32.hip (the viscous intermediates stored in FP32).
 #pragma once
 
+#include <type_traits>
+
 #include "sweep_parts.hpp"
 
 namespace cs {
@@ -27,7 +29,7 @@ __device__ __forceinline__ void stv(float* p, uoff bo, double v) {
 // S1: velocity and c^2 gradients along AX.  The first two S1 sweeps of an
 // RHS store their index-space gradients; the third (CLOSE) combines its own
 // gradients with the stored ones into the stress tensor and heat flux.
-template <int AX, bool CLOSE, class TS, bool CURV = false>
+template <int AX, bool CLOSE, class TS, bool CURV = false, bool DT = false>
 struct S1T {
     typedef TS Store;                      // storage type of the gradients, stress and heat flux
     static constexpr bool kCurv = CURV;    // per-point metrics (curvilinear solvers)
@@ -40,6 +42,10 @@ struct S1T {
     static constexpr int kH = 2;           // derivative stencil half-width
     static constexpr int kWpe = 3;         // waves per SIMD of the y/z kernels (168 VGPRs)
     static constexpr bool kFilter = false;
+    // DT: a y/z gradient sweep of constant metrics that also reduces the time
+    // step's maxima over the points it evaluates, one pair per workgroup
+    static constexpr bool kDt = DT;
+    static_assert(!DT || (!CLOSE && !CURV && AX != 0), "time-step maxima: y/z gradient sweeps, constant metrics");
     const double* q[5];
     TS* gout[4];              // !CLOSE: d/d(axis AX) of u, v, w, c^2
     const TS* gin[3][4];      // CLOSE: stored gradients along the other two axes
@@ -47,6 +53,7 @@ struct S1T {
     TS* hf[3];
     Phys ph;
     Metric mt;                // CURV: the point's metric rows and Jacobian
+    double* dtpart;           // kDt: the workgroup's two time-step maxima [2*blockIdx.x]
 
     __device__ __forceinline__ void point(uoff bo, double* v) const {
         double qq[5];
@@ -55,6 +62,16 @@ struct S1T {
         v[1] = p.v;
         v[2] = p.w;
         v[3] = p.c2;
+    }
+    // point() and the largest 1/rho so far (kDt)
+    __device__ __forceinline__ void point_dt(uoff bo, double* v, double& rhoi_max) const {
+        double qq[5];
+        Prim p = prim_at(q, bo, qq);
+        v[0] = p.u;
+        v[1] = p.v;
+        v[2] = p.w;
+        v[3] = p.c2;
+        rhoi_max = fmax(rhoi_max, p.rhoi);
     }
     __device__ __forceinline__ void fetch(uoff, double*) const {}
     __device__ __forceinline__ void store(uoff bo, const double* d, const double* pv) const {
@@ -125,6 +142,11 @@ template <int AX, bool CLOSE>
 struct S1C : S1T<AX, CLOSE, double, true> {};
 template <int AX, bool CLOSE>
 struct S1CF : S1T<AX, CLOSE, float, true> {};
+// the first gradient sweep of an RK4 step, reducing the time step as well
+template <int AX, bool CLOSE>
+struct S1D : S1T<AX, CLOSE, double, false, true> {};
+template <int AX, bool CLOSE>
+struct S1FD : S1T<AX, CLOSE, float, false, true> {};
 
 // S2: flux divergence along AX.  MODE 0: write the partial RHS; 1: add to it;
 // 2: add and write the RHS; 3: add and write q0 - dtk*J*RHS (RK stage).
@@ -552,6 +574,16 @@ __device__ __forceinline__ void face_corrections(const SW& sw, uoff base, uoff s
 // from the neighbouring chunks (shuffles within a wave segment for x lines,
 // LDS for y/z lines)
 // ---------------------------------------------------------------------------
+// functors that can reduce the time step's maxima in their point evaluation
+template <class SW, class = void>
+struct DtFold {
+    static constexpr bool value = false;
+};
+template <class SW>
+struct DtFold<SW, std::void_t<decltype(SW::kDt)>> {
+    static constexpr bool value = SW::kDt;
+};
+
 template <int K, int AX, int TL, class SW, bool GCBC>
 __global__ void __launch_bounds__(AX == 0 ? 256 : TL * kMaxP)
 __attribute__((amdgpu_waves_per_eu(AX == 0 ? 1 : (K == 16 ? 2 : SW::kWpe))))
@@ -585,18 +617,38 @@ direct_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
     const double* tab = SW::kFilter ? ax.ftab : ax.tab;
     const bool cyc = SW::kFilter && ax.seam;
 
+    // time-step maxima (DtFold functors; every lane of the workgroup's waves
+    // is a thread, the launcher checks)
+    constexpr bool kDt = DtFold<SW>::value;
+    double dm0 = 0.0, dm1 = 0.0;
     double w[NF][W];
 #pragma unroll
     for (int t = 0; t < K; t++) {
         double x[NF];
         if (work) {
-            sw.point(base + (uoff)(s + t) * st, x);
+            if constexpr (kDt) sw.point_dt(base + (uoff)(s + t) * st, x, dm1);
+            else sw.point(base + (uoff)(s + t) * st, x);
         } else {
 #pragma unroll
             for (int f = 0; f < NF; f++) x[f] = 0.0;
         }
 #pragma unroll
         for (int f = 0; f < NF; f++) w[f][t + H] = x[f];
+    }
+    if constexpr (kDt) {
+        // the convective maximum from u, v, w, c^2 of the rows; the viscous one
+        // at the largest 1/rho
+        if (work) {
+#pragma unroll
+            for (int t = 0; t < K; t++)
+                dm0 = fmax(dm0, dt_conv(sw.ph, w[0][t + H], w[1][t + H], w[2][t + H], w[3][t + H]));
+            dm1 = dt_visc(sw.ph, dm1);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            dm0 = fmax(dm0, __shfl_xor(dm0, o));
+            dm1 = fmax(dm1, __shfl_xor(dm1, o));
+        }
     }
     if (AX == 0) {
 #pragma unroll
@@ -668,7 +720,26 @@ direct_kernel(AxisDesc ax, SweepCommon cm, SW sw) {
         __syncthreads();                                // halo reads done: reuse LDS
 #pragma unroll
         for (int f = 0; f < NF; f++) lds[(f * P + c) * TL + lloc] = carry[f];
+        // the waves' time-step maxima go past the carries of both folds
+        double* dl = lds + 3 * NF * P * TL;
+        if constexpr (kDt) {
+            if ((threadIdx.x & 63) == 0) {
+                dl[2 * (threadIdx.x >> 6)] = dm0;
+                dl[2 * (threadIdx.x >> 6) + 1] = dm1;
+            }
+        }
         __syncthreads();
+        if constexpr (kDt) {
+            if (threadIdx.x == 0) {
+                double a0 = 0.0, a1 = 0.0;
+                for (int v = 0; v < (int)(blockDim.x >> 6); v++) {
+                    a0 = fmax(a0, dl[2 * v]);
+                    a1 = fmax(a1, dl[2 * v + 1]);
+                }
+                sw.dtpart[2 * blockIdx.x] = a0;
+                sw.dtpart[2 * blockIdx.x + 1] = a1;
+            }
+        }
         fold_forward<NF, true, K>(lds, tab, N, P, TL, c, lloc, G);
     }
     apply_carry<K, NF, true>(tab, 3 * N, s, K, r, G);
@@ -1366,6 +1437,26 @@ static int s2_sweep(cs_solver* sv, int k, const SweepCommon& cm, const double* q
     return dispatch<0>(sv->rax[0], cm, make_s2<F, 0, 2, V>(sv, q, sv->dpart, out, nullptr, 0), s);
 }
 
+// the gradient functor that also reduces the time step, per storage type
+template <template <int, bool> class F>
+struct DtOf {
+    static constexpr bool kOk = false;
+    template <int A, bool C>
+    using type = F<A, C>;
+};
+template <>
+struct DtOf<S1> {
+    static constexpr bool kOk = true;
+    template <int A, bool C>
+    using type = S1D<A, C>;
+};
+template <>
+struct DtOf<S1F> {
+    static constexpr bool kOk = true;
+    template <int A, bool C>
+    using type = S1FD<A, C>;
+};
+
 template <template <int, bool> class F = S1>
 static int s1_sweep(cs_solver* sv, int k, const SweepCommon& cm, const double* q, hipStream_t s) {
     if (sv->order == 0) {
@@ -1373,7 +1464,17 @@ static int s1_sweep(cs_solver* sv, int k, const SweepCommon& cm, const double* q
         if (k == 1) return dispatch<1>(sv->rax[1], cm, make_s1<F, 1, false>(sv, q), s);
         return dispatch<2>(sv->rax[2], cm, make_s1<F, 2, true>(sv, q), s);
     }
-    if (k == 0) return dispatch<1>(sv->rax[1], cm, make_s1<F, 1, false>(sv, q), s);
+    if (k == 0) {
+        // the first sweep of an RK4 step's first stage reduces the time step
+        if constexpr (DtOf<F>::kOk) {
+            if (sv->dt_fold) {
+                auto f = make_s1<DtOf<F>::template type, 1, false>(sv, q);
+                f.dtpart = sv->dtpart;
+                return dispatch<1>(sv->rax[1], cm, f, s);
+            }
+        }
+        return dispatch<1>(sv->rax[1], cm, make_s1<F, 1, false>(sv, q), s);
+    }
     if (k == 1) return dispatch<2>(sv->rax[2], cm, make_s1<F, 2, false>(sv, q), s);
     return dispatch<0>(sv->rax[0], cm, make_s1<F, 0, true>(sv, q), s);
 }

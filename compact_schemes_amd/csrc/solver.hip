@@ -117,7 +117,7 @@ int cs_solver_destroy(cs_solver* sv) {
                       sv->ftabs16[0], sv->ftabs16[1], sv->ftabs16[2],
                       sv->ftabs[0], sv->ftabs[1], sv->ftabs[2],
                       sv->ttabs[0], sv->ttabs[1], sv->ttabs[2], sv->fttabs[0], sv->fttabs[1], sv->fttabs[2],
-                      sv->adj};
+                      sv->adj, sv->dtpart};
     for (double* p : bufs)
         if (p) (void)hipFree(p);
     if (sv->dt_host) (void)hipHostFree(sv->dt_host);
@@ -277,11 +277,15 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
               hipMalloc(&sv->red, sizeof(double) * 2 * sv->red_blocks) == hipSuccess &&
               hipMalloc(&sv->dt, sizeof(double) * 4) == hipSuccess &&
               hipHostMalloc(&sv->dt_host, sizeof(double) * 4) == hipSuccess;
+    // one pair of time-step maxima per workgroup of a y or z gradient sweep
+    // (at least 8 lines per workgroup)
+    const long fold_blocks = (std::max(g.face_size(1), g.face_size(2)) + 7) / 8;
     if (ok && cfg->viscous)
         ok = hipMalloc(&sv->ga, sizeof(double) * 4 * lmx) == hipSuccess &&
              hipMalloc(&sv->gb, sizeof(double) * 4 * lmx) == hipSuccess &&
              hipMalloc(&sv->tau, sizeof(double) * 6 * lmx) == hipSuccess &&
-             hipMalloc(&sv->hf, sizeof(double) * 3 * lmx) == hipSuccess;
+             hipMalloc(&sv->hf, sizeof(double) * 3 * lmx) == hipSuccess &&
+             hipMalloc(&sv->dtpart, sizeof(double) * 2 * fold_blocks) == hipSuccess;
     if (!ok || hipMemset(sv->dt, 0, sizeof(double) * 4) != hipSuccess) {
         cs_solver_destroy(sv);
         return fail(CS_ERR_HIP, "allocation of solver buffers failed");
@@ -520,9 +524,15 @@ int cs_solver_rk4_step(cs_solver* sv, double* qa, const double* sbcc, double dt,
     if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
         return fail(CS_ERR_ARG, "non-reflective faces need sbcc");
     hipStream_t s = (hipStream_t)stream;
-    int rc = time_step(sv, qa, dt, s);
+    // a computed time step is reduced by the first gradient sweep of the first
+    // stage where that sweep can (dt is read by the stage update at its end)
+    const bool fold = !(dt > 0.0) && dt_foldable(sv);
+    int rc = fold ? CS_OK : time_step(sv, qa, dt, s);
     if (rc) return rc;
-    if ((rc = rhs_sweeps(sv, qa, sbcc, sv->qb, qa, 4, s))) return rc;
+    sv->dt_fold = fold ? 1 : 0;
+    rc = rhs_sweeps(sv, qa, sbcc, sv->qb, qa, 4, s);
+    sv->dt_fold = 0;
+    if (rc) return rc;
     if ((rc = rhs_sweeps(sv, sv->qb, sbcc, sv->qc, qa, 3, s))) return rc;
     if ((rc = rhs_sweeps(sv, sv->qc, sbcc, sv->qb, qa, 2, s))) return rc;
     if ((rc = rhs_sweeps(sv, sv->qb, sbcc, qa, qa, 1, s))) return rc;

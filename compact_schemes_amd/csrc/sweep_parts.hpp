@@ -31,6 +31,25 @@ __device__ __forceinline__ Prim prim_at(const double* const* q, uoff bo, double*
     return primitive(qq);
 }
 
+// the two quantities of the time step at one point with constant diagonal
+// metrics (calc_time_step_kernel, src/cuda/kernels/physics.hpp:335-427, D9
+// corrected): the convective (c |grad xi| + |u . grad xi|) |J| and the
+// viscous mu/rho |grad xi|^2 J^2.  One expression for every kernel that
+// reduces them, so that all give the same dt bit for bit.  The viscous one is
+// a product of non-negative factors rounded left to right, so its maximum
+// over points is its value at the largest 1/rho (rounding is monotone).
+__device__ __forceinline__ double dt_conv(const Phys& ph, double u, double v, double w, double c2) {
+    const double rr1 = ph.m[0] * ph.m[0] + ph.m[1] * ph.m[1] + ph.m[2] * ph.m[2];
+    const double rr2 = fabs(ph.m[0] * (u + ph.umf.x)) + fabs(ph.m[1] * (v + ph.umf.y)) +
+                       fabs(ph.m[2] * (w + ph.umf.z));
+    return (sqrt(c2 * rr1) + rr2) * fabs(ph.J);
+}
+__device__ __forceinline__ double dt_visc(const Phys& ph, double rhoi) {
+    const double rr1 = ph.m[0] * ph.m[0] + ph.m[1] * ph.m[1] + ph.m[2] * ph.m[2];
+    const double ssi = fabs(ph.J);
+    return rhoi * ph.mu * rr1 * ssi * ssi;
+}
+
 // byte offset of row 0 of line f of an axis (f runs over the face's points
 // with the first remaining index fastest)
 __device__ __forceinline__ uoff line_off(int axis, uoff f, uoff lxi, uoff plane) {

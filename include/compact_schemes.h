@@ -319,10 +319,13 @@ int cs_solver_set_order(cs_solver* sv, int order);
 /* time step of physics::calc_time_step on qa (D9 corrected); dte may be NULL
  * (then nothing synchronises and the value stays on the device). */
 int cs_solver_time_step(cs_solver* sv, const double* qa, double* dte, cs_stream stream);
-/* one RK4 step in place on qa: dt from cs_solver_time_step when dt <= 0,
- * stages q_k = q0 - dt/(5-k) * yaco * de(q_{k-1}), k = 1..4, then the compact
- * filter along x, y, z when cfg.filter.  dt_out (host, may be NULL)
- * receives the step used; passing it synchronises the stream. */
+/* one RK4 step in place on qa: dt from cs_solver_time_step when dt <= 0
+ * (the same value bit for bit; a viscous solver with constant metrics
+ * reduces it inside its first gradient sweep where that sweep runs whole
+ * waves of y or z lines), stages q_k = q0 - dt/(5-k) * yaco * de(q_{k-1}),
+ * k = 1..4, then the compact filter along x, y, z when cfg.filter.  dt_out
+ * (host, may be NULL) receives the step used; passing it synchronises the
+ * stream. */
 int cs_solver_rk4_step(cs_solver* sv, double* qa, const double* sbcc, double dt, double* dt_out,
                        cs_stream stream);
 /* adjoint of the inviscid right-hand side (no reference counterpart):

@@ -108,6 +108,34 @@ def test_fused_fixed_dt_and_time_step(oracle):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dims, nbc, order, prec", [
+    ((32, 64, 24), CL, 1, 0),     # y gradient sweep first, 16 lines x 8 chunks: reduces dt
+    ((32, 64, 24), PER, 1, 1),    # the same with FP32 intermediates
+    ((24, 48, 40), MIX, 1, 0),    # 16 lines x 6 chunks, not whole waves: a pass of its own
+    ((32, 64, 24), CL, 0, 0),     # x gradient sweep first: a pass of its own
+])
+def test_fused_step_time_step_reduction(dims, nbc, order, prec):
+    """Wherever an RK4 step reduces its time step (inside its first gradient
+    sweep, or in a pass of its own), the step's dt equals cs_solver_time_step's
+    bit for bit, and the step equals one with that dt given."""
+    import torch
+    from compact_schemes_amd.fused import FusedSolver
+    st = _state(dims, seed=9)
+    sv = FusedSolver(dims, nbc, st)
+    sv.set_order(order)
+    sv.set_precision(prec)
+    q = dev(st["qa"])
+    case(f"fused step dt dims={dims} nbc={nbc[0]} order={order} prec={prec}")
+    dt_ts = sv.time_step(q)
+    q1 = q.clone()
+    dt1 = sv.rk4_step(q1, report_dt=True)
+    assert dt1 == dt_ts
+    q2 = q.clone()
+    sv.rk4_step(q2, dt=dt1)
+    assert torch.equal(q1, q2)
+
+
+@pytest.mark.gpu
 def test_fused_rhs_256_matches_oracle(oracle):
     """The benchmark workload itself: 256^3, Mach 0.5 flow with the acoustic
     pulse, non-reflective characteristic BCs on all faces, viscous."""
