@@ -147,6 +147,17 @@ int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, 
                hipStream_t s) {
     const SweepCommon cm = common_of(sv, sbcc);
     for (int pos = sv->cfg.viscous ? 0 : 3; pos < 6; pos++) {
+        if (pos == 0) {
+            // the second gradient sweep on the aux stream, concurrently with
+            // the first; the closing sweep waits for both
+            CS_HIP(hipEventRecord(sv->ev_fork, s));
+            CS_HIP(hipStreamWaitEvent(sv->aux, sv->ev_fork, 0));
+            const int rc = sweep_at(sv, 1, cm, q, out, nullptr, div, sv->aux);
+            if (rc) return rc;
+            CS_HIP(hipEventRecord(sv->ev_join, sv->aux));
+        }
+        if (pos == 1) continue;
+        if (pos == 2) CS_HIP(hipStreamWaitEvent(s, sv->ev_join, 0));
         const int rc = sweep_at(sv, pos, cm, q, out, pos == 5 ? q0 : nullptr, div, s);
         if (rc) return rc;
         if (pos == 0 && sv->dt_fold) {

@@ -81,6 +81,10 @@ struct cs_solver {
     // step reduces them there (dt_fold) instead of in a pass of its own
     double* dtpart = nullptr;
     int dt_fold = 0;
+    // viscous solvers: the second gradient sweep of each RHS runs on aux,
+    // concurrently with the first (both read q and write their own gradients)
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double* dt = nullptr;      // [0] dt, [1..2] maxima
     double* dt_host = nullptr; // pinned
     double* vjp = nullptr;     // [41][lmx] adjoint scratch (allocated on the first adjoint call)

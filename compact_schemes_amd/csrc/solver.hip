@@ -121,6 +121,9 @@ int cs_solver_destroy(cs_solver* sv) {
     for (double* p : bufs)
         if (p) (void)hipFree(p);
     if (sv->dt_host) (void)hipHostFree(sv->dt_host);
+    if (sv->ev_fork) (void)hipEventDestroy(sv->ev_fork);
+    if (sv->ev_join) (void)hipEventDestroy(sv->ev_join);
+    if (sv->aux) (void)hipStreamDestroy(sv->aux);
     if (sv->num) cs_numerics_destroy(sv->num);
     delete sv;
     return CS_OK;
@@ -285,7 +288,10 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
              hipMalloc(&sv->gb, sizeof(double) * 4 * lmx) == hipSuccess &&
              hipMalloc(&sv->tau, sizeof(double) * 6 * lmx) == hipSuccess &&
              hipMalloc(&sv->hf, sizeof(double) * 3 * lmx) == hipSuccess &&
-             hipMalloc(&sv->dtpart, sizeof(double) * 2 * fold_blocks) == hipSuccess;
+             hipMalloc(&sv->dtpart, sizeof(double) * 2 * fold_blocks) == hipSuccess &&
+             hipStreamCreateWithFlags(&sv->aux, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&sv->ev_fork, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&sv->ev_join, hipEventDisableTiming) == hipSuccess;
     if (!ok || hipMemset(sv->dt, 0, sizeof(double) * 4) != hipSuccess) {
         cs_solver_destroy(sv);
         return fail(CS_ERR_HIP, "allocation of solver buffers failed");

@@ -236,6 +236,11 @@ typedef struct {
 
 typedef struct cs_solver cs_solver;
 
+/* a viscous solver owns one more stream (default priority) on which the
+ * second gradient sweep of every RHS runs concurrently with the first; events
+ * fork it from and join it back to the caller's stream, so the caller sees
+ * one stream's ordering and a capture of that stream records both.  Calls on
+ * one solver from several streams at once race, as for every solver call. */
 int cs_solver_create(const cs_solver_config* cfg, cs_solver** out);
 int cs_solver_destroy(cs_solver* sv);
 /* de = the reference's de after calc_fluxes + gcbc_go (dq/dt = -yaco*de).
