@@ -108,6 +108,27 @@ def test_fused_fixed_dt_and_time_step(oracle):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("visc", [True, False], ids=["viscous", "euler"])
+@pytest.mark.parametrize("dims, nbc", [((32, 28, 24), CL), ((48, 40, 36), MIX), ((24, 64, 40), PER)])
+def test_fused_concurrent_sweeps_bit_identical(dims, nbc, visc):
+    """cs_solver_rhs runs the two first gradient sweeps of a viscous solver on
+    two streams; the result equals the six sweeps run one after another
+    (cs_solver_sweep) bit for bit."""
+    import torch
+    from compact_schemes_amd.fused import FusedSolver
+    st = _state(dims, seed=13)
+    sv = FusedSolver(dims, nbc, st, viscous=visc)
+    q = dev(st["qa"])
+    case(f"concurrent sweeps dims={dims} nbc={nbc[0]} visc={visc}")
+    de = sv.rhs(q).clone()
+    chained = torch.zeros_like(de)
+    for w in (range(6) if visc else range(3, 6)):
+        sv.sweep(w, q, chained)
+    torch.cuda.synchronize()
+    assert torch.equal(de, chained)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("dims, nbc, order, prec", [
     ((32, 64, 24), CL, 1, 0),     # y gradient sweep first, 16 lines x 8 chunks: reduces dt
     ((32, 64, 24), PER, 1, 1),    # the same with FP32 intermediates
