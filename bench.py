@@ -10,7 +10,10 @@ conserved variables along x, y, z.  value = RHS grid points per second =
 4 * 256^3 / t_step (filter and time step included, i.e. conservative).
 
 Prints ONE JSON line.  --gpus N > 1 is not supported: this build runs the
-subdomain of one GPU.
+subdomain of one GPU.  The default run also measures the other workloads
+(--workload plain64, batch96, precision, adjoint, curv256), each in a child
+process of this script, and carries their lines under "other_workloads"
+(--no-workloads skips them).
 """
 import argparse
 import json
@@ -58,6 +61,8 @@ def parse():
     ap.add_argument("--curvilinear", action="store_true",
                     help="--workload adjoint on the sine-skewed curvilinear grid (per-point metrics)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="default workload only: do not run the other workloads' lines")
     ap.add_argument("--cpu-reps", type=int, default=3)
     return ap.parse_args()
 
@@ -247,7 +252,33 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if not args.no_workloads and args.path == "fused" and n == 256:
+        line["other_workloads"] = other_workloads()
     print(json.dumps(line), flush=True)
+
+
+OTHER_WORKLOADS = ("plain64", "batch96", "precision", "adjoint", "curv256")
+
+
+def other_workloads(timeout=300):
+    """The JSON lines of the other workloads, each from a child process of
+    this script (its own GPU context and memory; a failure or a time-out is
+    recorded, not raised)."""
+    out = {}
+    for w in OTHER_WORKLOADS:
+        t0 = time.perf_counter()
+        try:
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--workload", w, "--no-cpu"],
+                               capture_output=True, text=True, timeout=timeout, cwd=ROOT)
+            lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+            if r.returncode == 0 and lines:
+                out[w] = json.loads(lines[-1])
+            else:
+                out[w] = {"error": f"exit status {r.returncode}", "stderr": r.stderr[-400:]}
+        except subprocess.TimeoutExpired:
+            out[w] = {"error": f"no line within {timeout} s"}
+        print(f"workload {w}: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    return out
 
 
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_rk_stage_256.txt")
