@@ -73,17 +73,10 @@ __global__ void __launch_bounds__(kTB) dt_partial_curv_kernel(const double* q, l
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < lmx; t += (long)gridDim.x * blockDim.x) {
         double qq[5];
         load_q(q, t, lmx, qq);
-        Prim p = primitive(qq);
-        const double u = p.u + ph.umf.x, v = p.v + ph.umf.y, w = p.w + ph.umf.z;
-        double rr1 = 0.0, rr2 = 0.0;
-        for (int a = 0; a < 3; a++) {
-            const double mx = mt.m[a][0][t], my = mt.m[a][1][t], mz = mt.m[a][2][t];
-            rr1 += mx * mx + my * my + mz * mz;
-            rr2 += fabs(mx * u + my * v + mz * w);
-        }
-        const double ssi = fabs(mt.J[t]);
-        m0 = fmax(m0, (sqrt(p.c2 * rr1) + rr2) * ssi);
-        m1 = fmax(m1, p.rhoi * ph.mu * rr1 * ssi * ssi);
+        double M[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int k = 0; k < 3; k++) M[a][k] = mt.m[a][k][t];
+        dt_curv_point(M, mt.J[t], ph, primitive(qq), m0, m1);
     }
     s0[threadIdx.x] = m0;
     s1[threadIdx.x] = m1;
@@ -134,13 +127,25 @@ static int sweep_at(cs_solver* sv, int pos, const SweepCommon& cm, const double*
     return s2_sweep<false>(sv, pos - 3, cm, q, out, q0, div, s);
 }
 
-bool dt_foldable(const cs_solver* sv) {
-    if (!sv->cfg.viscous || sv->curv || !sv->dtpart) return false;
-    const int a = kS1Axes[sv->order][0];
-    if (a == 0) return false;
-    const AxisDesc& r = sv->rax[a];
+int dt_fold_pos(const cs_solver* sv) {
+    if (!sv->cfg.viscous || !sv->dtpart || sv->order != 1) return -1;
+    if (sv->curv) {
+        // the closing x sweep on the transposing kernel (8-point chunks)
+        const AxisDesc& r = sv->rax[kS1Axes[1][2]];
+        return r.variant != 1 && r.K == 8 && r.N == r.P * 8 ? 2 : -1;
+    }
+    // the first (y) sweep on the direct kernel with whole waves
+    const AxisDesc& r = sv->rax[kS1Axes[1][0]];
     const int TL = r.tl == 8 ? 8 : 16;
-    return r.variant != 1 && r.N == r.P * r.K && (TL * r.P) % 64 == 0;
+    return r.variant != 1 && r.N == r.P * r.K && (TL * r.P) % 64 == 0 ? 0 : -1;
+}
+
+// workgroups of the sweep at position pos (0: the direct y/z kernel, 2: the
+// transposing x kernel), one pair of maxima each
+static int fold_blocks(const cs_solver* sv, int pos) {
+    const AxisDesc& r = sv->rax[kS1Axes[sv->order][pos]];
+    const int lines = pos == 2 ? 64 / r.GS : (r.tl == 8 ? 8 : 16);
+    return (int)((r.nlines + lines - 1) / lines);
 }
 
 int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, const double* q0, int div,
@@ -160,12 +165,10 @@ int rhs_sweeps(cs_solver* sv, const double* q, const double* sbcc, double* out, 
         if (pos == 2) CS_HIP(hipStreamWaitEvent(s, sv->ev_join, 0));
         const int rc = sweep_at(sv, pos, cm, q, out, pos == 5 ? q0 : nullptr, div, s);
         if (rc) return rc;
-        if (pos == 0 && sv->dt_fold) {
+        if (sv->dt_fold == pos + 1) {
             // the sweep wrote one pair of maxima per workgroup
-            const AxisDesc& r = sv->rax[kS1Axes[sv->order][0]];
-            const int TL = r.tl == 8 ? 8 : 16;
-            const int nb = (int)((r.nlines + TL - 1) / TL);
-            hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(kTB), 0, s, sv->dtpart, nb, sv->cfg.cfl, 0.0, sv->dt);
+            hipLaunchKernelGGL(dt_final_kernel, dim3(1), dim3(kTB), 0, s, sv->dtpart, fold_blocks(sv, pos),
+                               sv->cfg.cfl, 0.0, sv->dt);
             CS_HIP(hipGetLastError());
         }
     }

@@ -77,8 +77,8 @@ struct cs_solver {
     double* qc = nullptr;
     double* red = nullptr;     // time-step partial maxima
     int red_blocks = 0;
-    // time-step maxima per workgroup of the first gradient sweep, when an RK4
-    // step reduces them there (dt_fold) instead of in a pass of its own
+    // time-step maxima per workgroup of the gradient sweep that reduces them
+    // inside an RK4 step (dt_fold = its position + 1, 0 = a pass of its own)
     double* dtpart = nullptr;
     int dt_fold = 0;
     // viscous solvers: the second gradient sweep of each RHS runs on aux,
@@ -115,9 +115,11 @@ int sweep_at_f32(cs_solver* sv, int pos, const SweepCommon& cm, const double* q,
 int sweep_at_curv(cs_solver* sv, int pos, const SweepCommon& cm, const double* q, double* out, const double* q0,
                   int div, hipStream_t s);
 int time_step(cs_solver* sv, const double* q, double dt_fixed, hipStream_t s);
-// whether the first gradient sweep can reduce the time step (viscous,
-// constant metrics, a y/z direct sweep of whole waves)
-bool dt_foldable(const cs_solver* sv);
+// the position of the gradient sweep that can reduce the time step of an RK4
+// step (viscous solvers in the default order: 0 = the y sweep on the direct
+// kernel with whole waves, constant metrics; 2 = the closing x sweep on the
+// transposing kernel, per-point metrics), -1 = none (a pass of its own)
+int dt_fold_pos(const cs_solver* sv);
 int one_sweep(cs_solver* sv, int which, const double* q, const double* sbcc, double* out, hipStream_t s);
 int build_axis_table(int n, int seam, int K, int P, std::vector<double>& h);
 // every axis filters with the 8-point fused sweep of ax (the adjoint's assumption)

@@ -42,10 +42,13 @@ struct S1T {
     static constexpr int kH = 2;           // derivative stencil half-width
     static constexpr int kWpe = 3;         // waves per SIMD of the y/z kernels (168 VGPRs)
     static constexpr bool kFilter = false;
-    // DT: a y/z gradient sweep of constant metrics that also reduces the time
-    // step's maxima over the points it evaluates, one pair per workgroup
+    // DT: a sweep that also reduces the time step's maxima over the points it
+    // evaluates, one pair per workgroup: a y/z gradient sweep
+    // (constant metrics), or the closing x sweep with per-point metrics, which
+    // loads the metric rows and J of every point anyway
     static constexpr bool kDt = DT;
-    static_assert(!DT || (!CLOSE && !CURV && AX != 0), "time-step maxima: y/z gradient sweeps, constant metrics");
+    static_assert(!DT || (!CLOSE && !CURV && AX != 0) || (CLOSE && CURV && AX == 0),
+                  "time-step maxima: y/z gradient sweeps of constant metrics, closing x sweep of per-point metrics");
     const double* q[5];
     TS* gout[4];              // !CLOSE: d/d(axis AX) of u, v, w, c^2
     const TS* gin[3][4];      // CLOSE: stored gradients along the other two axes
@@ -90,7 +93,15 @@ struct S1T {
         if constexpr (CLOSE) close(bo, d, uvw);
         else store(bo, d, nullptr);
     }
+    // the same, updating the time step's maxima mm[0..1] at the point (kDt, CURV)
+    __device__ __forceinline__ void store_pv_dt(uoff bo, const double* d, const double* uvw, double* mm) const {
+        close_dt<true>(bo, d, uvw, mm);
+    }
     __device__ __forceinline__ void close(uoff bo, const double* d, const double* uvw) const {
+        close_dt<false>(bo, d, uvw, nullptr);
+    }
+    template <bool WITH_DT>
+    __device__ __forceinline__ void close_dt(uoff bo, const double* d, const double* uvw, double* mm) const {
         double gr[3][4];
 #pragma unroll
         for (int a = 0; a < 3; a++)
@@ -99,8 +110,8 @@ struct S1T {
         // chain rule of calc_viscous_shear_stress_post_compute_kernel
         // (src/cuda/kernels/physics.hpp:267-281): d/dx_k = sum_a m_a[k] d/dxi_a
         double g[9], c2g[3], J = ph.J;
+        double M[3][3];
         if constexpr (CURV) {
-            double M[3][3];
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -127,6 +138,11 @@ struct S1T {
         for (int k = 0; k < 6; k++) stv(tau[k], bo, t6[k]);
 #pragma unroll
         for (int k = 0; k < 3; k++) stv(hf[k], bo, h3[k]);
+        if constexpr (WITH_DT && CURV) {
+            // the point's q again (this sweep read it for the gradients)
+            double qq[5];
+            dt_curv_point(M, J, ph, prim_at(q, bo, qq), mm[0], mm[1]);
+        }
     }
     __device__ __forceinline__ void gcbc(uoff, int, const double*, double*) const {}
 };
@@ -147,6 +163,12 @@ template <int AX, bool CLOSE>
 struct S1D : S1T<AX, CLOSE, double, false, true> {};
 template <int AX, bool CLOSE>
 struct S1FD : S1T<AX, CLOSE, float, false, true> {};
+// the closing sweep of an RK4 step's first stage with per-point metrics,
+// reducing the time step as well
+template <int AX, bool CLOSE>
+struct S1CD : S1T<AX, CLOSE, double, true, true> {};
+template <int AX, bool CLOSE>
+struct S1CFD : S1T<AX, CLOSE, float, true, true> {};
 
 // S2: flux divergence along AX.  MODE 0: write the partial RHS; 1: add to it;
 // 2: add and write the RHS; 3: add and write q0 - dtk*J*RHS (RK stage).
@@ -574,14 +596,18 @@ __device__ __forceinline__ void face_corrections(const SW& sw, uoff base, uoff s
 // from the neighbouring chunks (shuffles within a wave segment for x lines,
 // LDS for y/z lines)
 // ---------------------------------------------------------------------------
-// functors that can reduce the time step's maxima in their point evaluation
+// functors that reduce the time step's maxima: value = in the point
+// evaluation of a y/z sweep (constant metrics), close = in the closing x
+// sweep's stores (per-point metrics)
 template <class SW, class = void>
 struct DtFold {
     static constexpr bool value = false;
+    static constexpr bool close = false;
 };
 template <class SW>
 struct DtFold<SW, std::void_t<decltype(SW::kDt)>> {
-    static constexpr bool value = SW::kDt;
+    static constexpr bool value = SW::kDt && !SW::kCurv;
+    static constexpr bool close = SW::kDt && SW::kCurv;
 };
 
 template <int K, int AX, int TL, class SW, bool GCBC>
@@ -935,6 +961,7 @@ __global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW 
 #pragma unroll
         for (int j = 0; j < J; j++) xo[f][j] = lds[slot[j]];
     }
+    double mm[2] = {0.0, 0.0};           // DtFold<SW>::close: the time step's maxima
     // every point's epilogue loads before the first store
 #pragma unroll
     for (int j = 0; j < J; j++) {
@@ -959,13 +986,26 @@ __global__ void __launch_bounds__(64) xt_kernel(AxisDesc ax, SweepCommon cm, SW 
                 double sv[SW::kStash > 0 ? SW::kStash : 1];
 #pragma unroll
                 for (int f = 0; f < SW::kStash; f++) sv[f] = stash[(f * J + j) * 64 + lane];
-                sw.store_pv(run + (uoff)p * st, d, sv);
+                if constexpr (DtFold<SW>::close) sw.store_pv_dt(run + (uoff)p * st, d, sv, mm);
+                else sw.store_pv(run + (uoff)p * st, d, sv);
             } else {
                 double pv[NF];
 #pragma unroll
                 for (int f = 0; f < NF; f++) pv[f] = SW::kPv ? xq[f][j] : 0.0;
                 sw.store(run + (uoff)p * st, d, pv);
             }
+        }
+    }
+    if constexpr (DtFold<SW>::close) {
+        // the workgroup is one wave: reduce across it, one pair per workgroup
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            mm[0] = fmax(mm[0], __shfl_xor(mm[0], o));
+            mm[1] = fmax(mm[1], __shfl_xor(mm[1], o));
+        }
+        if (lane == 0) {
+            sw.dtpart[2 * blockIdx.x] = mm[0];
+            sw.dtpart[2 * blockIdx.x + 1] = mm[1];
         }
     }
 }
@@ -1456,6 +1496,18 @@ struct DtOf<S1F> {
     template <int A, bool C>
     using type = S1FD<A, C>;
 };
+template <>
+struct DtOf<S1C> {
+    static constexpr bool kOk = true;
+    template <int A, bool C>
+    using type = S1CD<A, C>;
+};
+template <>
+struct DtOf<S1CF> {
+    static constexpr bool kOk = true;
+    template <int A, bool C>
+    using type = S1CFD<A, C>;
+};
 
 template <template <int, bool> class F = S1>
 static int s1_sweep(cs_solver* sv, int k, const SweepCommon& cm, const double* q, hipStream_t s) {
@@ -1464,10 +1516,12 @@ static int s1_sweep(cs_solver* sv, int k, const SweepCommon& cm, const double* q
         if (k == 1) return dispatch<1>(sv->rax[1], cm, make_s1<F, 1, false>(sv, q), s);
         return dispatch<2>(sv->rax[2], cm, make_s1<F, 2, true>(sv, q), s);
     }
+    // the sweep of an RK4 step's first stage that reduces the time step
+    // (dt_fold = its position + 1): the first with constant metrics, the
+    // closing x sweep with per-point metrics
     if (k == 0) {
-        // the first sweep of an RK4 step's first stage reduces the time step
-        if constexpr (DtOf<F>::kOk) {
-            if (sv->dt_fold) {
+        if constexpr (DtOf<F>::kOk && !F<1, false>::kCurv) {
+            if (sv->dt_fold == 1) {
                 auto f = make_s1<DtOf<F>::template type, 1, false>(sv, q);
                 f.dtpart = sv->dtpart;
                 return dispatch<1>(sv->rax[1], cm, f, s);
@@ -1476,6 +1530,13 @@ static int s1_sweep(cs_solver* sv, int k, const SweepCommon& cm, const double* q
         return dispatch<1>(sv->rax[1], cm, make_s1<F, 1, false>(sv, q), s);
     }
     if (k == 1) return dispatch<2>(sv->rax[2], cm, make_s1<F, 2, false>(sv, q), s);
+    if constexpr (DtOf<F>::kOk && F<0, true>::kCurv) {
+        if (sv->dt_fold == 3) {
+            auto f = make_s1<DtOf<F>::template type, 0, true>(sv, q);
+            f.dtpart = sv->dtpart;
+            return dispatch<0>(sv->rax[0], cm, f, s);
+        }
+    }
     return dispatch<0>(sv->rax[0], cm, make_s1<F, 0, true>(sv, q), s);
 }
 

@@ -281,8 +281,8 @@ int cs_solver_create(const cs_solver_config* cfg, cs_solver** out) {
               hipMalloc(&sv->dt, sizeof(double) * 4) == hipSuccess &&
               hipHostMalloc(&sv->dt_host, sizeof(double) * 4) == hipSuccess;
     // one pair of time-step maxima per workgroup of a y or z gradient sweep
-    // (at least 8 lines per workgroup)
-    const long fold_blocks = (std::max(g.face_size(1), g.face_size(2)) + 7) / 8;
+    // (at least 8 lines per workgroup) or of the x sweep (at least one line)
+    const long fold_blocks = std::max((std::max(g.face_size(1), g.face_size(2)) + 7) / 8, g.face_size(0));
     if (ok && cfg->viscous)
         ok = hipMalloc(&sv->ga, sizeof(double) * 4 * lmx) == hipSuccess &&
              hipMalloc(&sv->gb, sizeof(double) * 4 * lmx) == hipSuccess &&
@@ -532,10 +532,10 @@ int cs_solver_rk4_step(cs_solver* sv, double* qa, const double* sbcc, double dt,
     hipStream_t s = (hipStream_t)stream;
     // a computed time step is reduced by the first gradient sweep of the first
     // stage where that sweep can (dt is read by the stage update at its end)
-    const bool fold = !(dt > 0.0) && dt_foldable(sv);
-    int rc = fold ? CS_OK : time_step(sv, qa, dt, s);
+    const int fold = dt > 0.0 ? -1 : dt_fold_pos(sv);
+    int rc = fold >= 0 ? CS_OK : time_step(sv, qa, dt, s);
     if (rc) return rc;
-    sv->dt_fold = fold ? 1 : 0;
+    sv->dt_fold = fold + 1;
     rc = rhs_sweeps(sv, qa, sbcc, sv->qb, qa, 4, s);
     sv->dt_fold = 0;
     if (rc) return rc;

@@ -49,6 +49,23 @@ __device__ __forceinline__ double dt_visc(const Phys& ph, double rhoi) {
     const double ssi = fabs(ph.J);
     return rhoi * ph.mu * rr1 * ssi * ssi;
 }
+// the same two maxima updated at one point with the point's metric rows M[a]
+// = grad xi_a and Jacobian J (calc_time_step_kernel's full form); one
+// expression for the separate pass and the sweep that folds it
+__device__ __forceinline__ void dt_curv_point(const double (&M)[3][3], double J, const Phys& ph, const Prim& p,
+                                              double& m0, double& m1) {
+    const double u = p.u + ph.umf.x, v = p.v + ph.umf.y, w = p.w + ph.umf.z;
+    double rr1 = 0.0, rr2 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double mx = M[a][0], my = M[a][1], mz = M[a][2];
+        rr1 += mx * mx + my * my + mz * mz;
+        rr2 += fabs(mx * u + my * v + mz * w);
+    }
+    const double ssi = fabs(J);
+    m0 = fmax(m0, (sqrt(p.c2 * rr1) + rr2) * ssi);
+    m1 = fmax(m1, p.rhoi * ph.mu * rr1 * ssi * ssi);
+}
 
 // byte offset of row 0 of line f of an axis (f runs over the face's points
 // with the first remaining index fastest)

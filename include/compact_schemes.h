@@ -325,9 +325,10 @@ int cs_solver_set_order(cs_solver* sv, int order);
  * (then nothing synchronises and the value stays on the device). */
 int cs_solver_time_step(cs_solver* sv, const double* qa, double* dte, cs_stream stream);
 /* one RK4 step in place on qa: dt from cs_solver_time_step when dt <= 0
- * (the same value bit for bit; a viscous solver with constant metrics
- * reduces it inside its first gradient sweep where that sweep runs whole
- * waves of y or z lines), stages q_k = q0 - dt/(5-k) * yaco * de(q_{k-1}),
+ * (the same value bit for bit; a viscous solver in the default sweep order
+ * reduces it inside a gradient sweep of the first stage: with constant
+ * metrics the first one where it runs whole waves of y lines, with per-point
+ * metrics the closing x sweep where x lines split into 8-point chunks), stages q_k = q0 - dt/(5-k) * yaco * de(q_{k-1}),
  * k = 1..4, then the compact filter along x, y, z when cfg.filter.  dt_out
  * (host, may be NULL) receives the step used; passing it synchronises the
  * stream. */

@@ -338,3 +338,30 @@ def test_curvilinear_fp32_intermediates(oracle, nbc):
     assert max(errs) > 0.0                    # the FP32 kernels ran
     sv.set_precision(0)
     assert torch.equal(sv.rhs(q).clone(), d64)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims, nbc, prec", [
+    ((32, 40, 24), CL, 0),        # x lines of 4 chunks of 8: the closing x sweep reduces dt
+    ((32, 40, 24), PER, 1),       # the same with FP32 intermediates
+    ((36, 40, 24), CL, 0),        # ragged x lines: a pass of its own
+])
+def test_curvilinear_step_time_step_reduction(dims, nbc, prec):
+    """Wherever a curvilinear RK4 step reduces its time step (inside the
+    closing x gradient sweep, or in a pass of its own over q, the metric rows
+    and J), the step's dt equals cs_solver_time_step's bit for bit, and the
+    step equals one with that dt given."""
+    import torch
+    from compact_schemes_amd.fused import FusedSolver
+    st = _state(dims, seed=17)
+    sv = FusedSolver(dims, nbc, st, curvilinear=True)
+    sv.set_precision(prec)
+    q = dev(st["qa"])
+    case(f"curvilinear step dt dims={dims} nbc={nbc[0]} prec={prec}")
+    dt_ts = sv.time_step(q)
+    q1 = q.clone()
+    dt1 = sv.rk4_step(q1, report_dt=True)
+    assert dt1 == dt_ts
+    q2 = q.clone()
+    sv.rk4_step(q2, dt=dt1)
+    assert torch.equal(q1, q2)
