@@ -530,8 +530,9 @@ int cs_solver_rk4_step(cs_solver* sv, double* qa, const double* sbcc, double dt,
     if (!sbcc && (sv->ax[0].gcbc || sv->ax[1].gcbc || sv->ax[2].gcbc))
         return fail(CS_ERR_ARG, "non-reflective faces need sbcc");
     hipStream_t s = (hipStream_t)stream;
-    // a computed time step is reduced by the first gradient sweep of the first
-    // stage where that sweep can (dt is read by the stage update at its end)
+    // a computed time step is reduced inside a gradient sweep of the first
+    // stage where one can (dt_fold_pos; dt is first read by the stage update
+    // at the end of that stage), else in a pass of its own
     const int fold = dt > 0.0 ? -1 : dt_fold_pos(sv);
     int rc = fold >= 0 ? CS_OK : time_step(sv, qa, dt, s);
     if (rc) return rc;
