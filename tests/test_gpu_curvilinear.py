@@ -345,6 +345,7 @@ def test_curvilinear_fp32_intermediates(oracle, nbc):
     ((32, 40, 24), CL, 0),        # x lines of 4 chunks of 8: the closing x sweep reduces dt
     ((32, 40, 24), PER, 1),       # the same with FP32 intermediates
     ((36, 40, 24), CL, 0),        # ragged x lines: a pass of its own
+    ((32, 40, 24), MIX, -1),      # x sweep tuned to the tiled kernel: a pass of its own
 ])
 def test_curvilinear_step_time_step_reduction(dims, nbc, prec):
     """Wherever a curvilinear RK4 step reduces its time step (inside the
@@ -355,7 +356,10 @@ def test_curvilinear_step_time_step_reduction(dims, nbc, prec):
     from compact_schemes_amd.fused import FusedSolver
     st = _state(dims, seed=17)
     sv = FusedSolver(dims, nbc, st, curvilinear=True)
-    sv.set_precision(prec)
+    if prec < 0:
+        sv.tune(0, 1, 16, 0)     # the tiled x kernel
+    else:
+        sv.set_precision(prec)
     q = dev(st["qa"])
     case(f"curvilinear step dt dims={dims} nbc={nbc[0]} prec={prec}")
     dt_ts = sv.time_step(q)

@@ -134,6 +134,7 @@ def test_fused_concurrent_sweeps_bit_identical(dims, nbc, visc):
     ((32, 64, 24), PER, 1, 1),    # the same with FP32 intermediates
     ((24, 48, 40), MIX, 1, 0),    # 16 lines x 6 chunks, not whole waves: a pass of its own
     ((32, 64, 24), CL, 0, 0),     # x gradient sweep first: a pass of its own
+    ((32, 64, 24), MIX, 1, -1),   # y sweep tuned to the tiled kernel: a pass of its own
 ])
 def test_fused_step_time_step_reduction(dims, nbc, order, prec):
     """Wherever an RK4 step reduces its time step (inside its first gradient
@@ -144,7 +145,10 @@ def test_fused_step_time_step_reduction(dims, nbc, order, prec):
     st = _state(dims, seed=9)
     sv = FusedSolver(dims, nbc, st)
     sv.set_order(order)
-    sv.set_precision(prec)
+    if prec < 0:
+        sv.tune(1, 1, 16, 0)     # the tiled y kernel, 16 lines per workgroup
+    else:
+        sv.set_precision(prec)
     q = dev(st["qa"])
     case(f"fused step dt dims={dims} nbc={nbc[0]} order={order} prec={prec}")
     dt_ts = sv.time_step(q)
